@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the PBF substep on one MI355X.
+
+Workload: the 1,048,576-particle dam break (128 x 64 x 128 lattice, spacing
+0.5 h, h = 0.1, in a 12.8 x 6.4 x 6.4 box; rest density 1000, g = -9.8,
+dt = 1/120 per substep, 4 Jacobi iterations, eps = 600, XSPH c = 0.01,
+vorticity 0.0005, jitter +/- 0.01 h).  A "step" is one substep: the whole
+body of Simulation/Source/main.cpp:71-124 through ``pbf_step``.
+
+Prints one JSON line.  ``value`` is particle-substeps/s with the particles
+already resident in HBM.  ``roofline`` reports the dominant kernel's
+algorithmic bytes per launch over its average launch time (HIP events on the
+library's stream, taken inside the timed region).  ``cpu_baseline`` times the
+CPU oracle (a restatement of the reference's OpenMP path) on a bounded sample
+of the same scene.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-substeps/s @1M, <=1% density err: plain vs learned warm-start"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per particle for each kernel launch: the compulsory
+# per-particle streams of that pass (positions and velocities as 12-byte
+# Vec<float,3>, scalars 4 bytes); neighbour gathers are assumed served on chip
+# and neighbour lists are not counted.
+ALG_BYTES = {
+    "density_lambda": 12 + 4 + 4,          # R x*; W rho, lambda
+    "dp_update": 12 + 4 + 12,              # R x*, lambda; W x*
+    "fill_neighbors": 12 + 4,              # R x*; W count (lists not counted)
+    "viscosity": 12 + 12 + 4 + 12,         # R x*, v, rho; W v
+    "viscosity_positions": 12 + 12 + 4 + 12 + 12,
+    "vorticity": 12 + 12 + 12,             # R x*, v; W omega
+    "apply_vorticity": 12 + 12 + 12 + 12,  # R x*, omega, v; W v
+    "forces_predict": 12 + 12 + 12 + 12,   # R x, v; W v, x*
+    "bin": 12 + 4 + 4,
+    "sort": 4 * 4,
+    "reorder": 2 * (3 * 12 + 4 + 4 + 12 + 4 + 4),
+    "scan": 8,
+    "lambda_repair": 0,
+}
+
+
+def _load(name):
+    return importlib.import_module(name)
+
+
+def cpu_baseline(params, pos, vel, warm=1, timed=5):
+    """Oracle in the reference's in-place mode with OpenMP on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    O = _load("oracle_lib")
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = O.Oracle(O.params_from_pbf(params), mode=O.SERIAL, threads=threads)
+    o.set_state(pos, pos, vel)
+    o.step(warm)
+    times = []
+    for _ in range(timed):
+        t0 = time.perf_counter()
+        o.step(1)
+        times.append(time.perf_counter() - t0)
+    o.close()
+    med = statistics.median(times)
+    n = pos.shape[0]
+    return {
+        "value": n / med,
+        "unit": "particle-substeps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} particles (same scene), {warm} warm-up + {timed} timed substeps, median "
+                  f"{med:.3f} s/substep; oracle in the reference's OpenMP in-place mode",
+    }
+
+
+def _profile_traffic(kernel: str):
+    """HBM bytes per launch from committed rocprofv3 PMC summaries, if present."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--lattice", default="128x64x128", help="block size in particles (x by y by z)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--kernel", default="", help="kernel to time (default: the slowest in a short probe)")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the single-domain PBF step has no multi-GPU path in this build "
+              "(see the sharding design notes).", file=sys.stderr)
+        return 2
+
+    pbf = _load("sph-cpp_amd")
+    scenes = _load("sph-cpp_amd.scenes")
+    counts = tuple(int(c) for c in args.lattice.lower().split("x"))
+    params, pos, vel = scenes.config_c2(seed=args.seed, lattice_counts=counts)
+    n = pos.shape[0]
+
+    g = pbf.Particles(params)
+    g.add(pos, pos, vel)
+
+    # short probe: which kernel dominates a substep
+    kernels = ["forces_predict", "bin", "scan", "sort", "reorder", "fill_neighbors", "density_lambda",
+               "lambda_repair", "dp_update", "viscosity", "vorticity", "apply_vorticity", "viscosity_positions"]
+    g.profile_kernels(kernels)
+    g.step(3)
+    g.sync()
+    probe = {k: g.kernel_time(k)[1] for k in kernels}
+    dom = args.kernel or max(probe, key=probe.get)
+    g.profile_kernels("")
+
+    g.step(args.warmup)
+    g.sync()
+
+    # timed region; HIP events around the dominant kernel's launches
+    g.profile_kernels(dom)
+    g.sync()
+    t0 = time.perf_counter()
+    g.region_begin()
+    g.step(args.steps)
+    ev_ms = g.region_end()
+    g.sync()
+    wall = time.perf_counter() - t0
+    launches, kms = g.kernel_time(dom)
+    g.profile_kernels("")
+    stats = g.stats()
+
+    seconds = wall
+    value = n * args.steps / seconds
+    avg_launch_ms = kms / max(1, launches)
+    alg = ALG_BYTES.get(dom, 0) * n
+    achieved = alg / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    traffic = _profile_traffic(dom)
+    roof = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": traffic,
+        "alg_bytes_per_launch": alg,
+        "avg_launch_ms": round(avg_launch_ms, 5),
+        "launches": launches,
+    }
+    # whole-substep bookkeeping: B_alg = 260 + 44 I + 8 C/N bytes per particle-substep
+    I = params.solve_step_count
+    b_alg = 260 + 44 * I + 8.0 * g.num_cells() / n
+    step_roof = {"alg_bytes_per_particle_substep": round(b_alg, 2),
+                 "achieved_GBs": round(b_alg * value / 1e9, 2),
+                 "frac_of_hbm_peak": round(b_alg * value / 1e9 / HBM_PEAK_GBS, 5)}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": seconds * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain PBF dam break, {n} particles ({args.lattice} lattice, spacing 0.05, h 0.1) in "
+                        f"12.8x6.4x6.4, dt 1/120, {I} Jacobi iterations, eps 600, XSPH 0.01, vorticity 0.0005",
+            "particles": n,
+            "solver": "plain, fixed iterations",
+            "jacobi_iterations": I,
+            "neighbor_cap": int(params.max_neighbors),
+            "cells": g.num_cells(),
+        },
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "event_ms_total": ev_ms,
+        "probe_ms": {k: round(v, 4) for k, v in probe.items()},
+        "stats": {"neighbor_overflow": stats["neighbor_overflow"], "density_hacks": stats["density_hacks"],
+                  "max_density_error": stats["max_density_error"]},
+    }
+    g.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(params, pos, vel)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

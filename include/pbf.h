@@ -1,0 +1,206 @@
+/* pbf.h -- C ABI of the MI355X position-based-fluids library.
+ *
+ * This header is the drop-in boundary for the PBF step of AdamSimpson/SPH-CPP.
+ * The reference drives its step through header-only templates:
+ *   Common/sim_algorithms.h:43-258   for_each_index, sort_by_key,
+ *                                    lower_bound, upper_bound (Thrust)
+ *   Simulation/Source/particles.h    one for_each_index lambda per pass
+ *   Simulation/Source/neighbors.h    bins, stable sort, bounds, capped lists
+ * Device lambdas cannot cross an extern "C" boundary, so each lambda body
+ * becomes one entry point taking an index span, the Thrust primitives become
+ * concretely typed device-pointer entry points, and the sim::Array /
+ * ManagedAllocation storage (Common/array.h, Common/managed_allocation.h)
+ * becomes device buffers owned by a context.
+ *
+ * Conventions
+ *   - Every function returns PBF_OK (0) or an error code; pbf_last_error()
+ *     gives the message for the calling thread.  Nothing throws.
+ *   - Particle indices are the reference's indices: the order particles were
+ *     added with pbf_add (particles.h:184-199).  The library stores particles
+ *     sorted by cell internally; every getter returns original order.
+ *   - Vectors are 3 floats per particle (Vec<float,3>, 12 bytes, vec.h:184-256).
+ *   - Work is stream ordered on the context's stream; getters synchronise.
+ *     The reference synchronises after every call (sim_algorithms.h:49,60,79,98,110);
+ *     here only pbf_sync and the getters do.
+ *   - One context per host thread.
+ */
+#ifndef PBF_H
+#define PBF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pbf_status {
+  PBF_OK = 0,
+  PBF_ERR_ARG = 1,        /* bad argument (span out of range, null pointer) */
+  PBF_ERR_CAPACITY = 2,   /* "Not enough capacity to push_back" (array.h:127-158) */
+  PBF_ERR_HIP = 3,        /* HIP runtime error */
+  PBF_ERR_STATE = 4,      /* call order (e.g. a neighbour pass before find_neighbors) */
+  PBF_ERR_UNSUPPORTED = 5 /* configuration outside what the library handles */
+};
+
+/* IndexSpan = Vec<size_t,2>{begin, end} (vec.h:706), half open. */
+typedef struct pbf_index_span {
+  uint64_t begin;
+  uint64_t end;
+} pbf_index_span;
+
+/* Parameters<float,3> public fields used by the step (parameters.h:550-573). */
+typedef struct pbf_params {
+  uint64_t max_particles_local;  /* capacity (parameters.h:550) */
+  uint32_t solve_step_count;     /* Jacobi iterations per step (parameters.h:552) */
+  uint32_t max_neighbors;        /* list cap, MAX_NEIGHBORS = 60 (neighbors.h:39); 1..64 */
+  float particle_rest_spacing;
+  float particle_radius;
+  float smoothing_radius;        /* h */
+  float neighbor_bin_spacing;    /* cell size and neighbour radius (neighbors.h:198) */
+  float rest_density;
+  float rest_mass;
+  float gravity;                 /* applied to y (particles.h:268) */
+  float gamma;                   /* surface tension; 0 skips the pass */
+  float lambda_epsilon;
+  float k_stiff;                 /* parsed by the reference, never used */
+  float visc_c;
+  float time_step;
+  float max_speed;
+  float vorticity_coef;
+  float boundary_min[3];         /* must be the origin (neighbors.h:110-113) */
+  float boundary_max[3];
+  float mover_center[3];
+} pbf_params;
+
+typedef struct pbf_context pbf_context;
+
+/* Counters for the last step / pass. */
+typedef struct pbf_stats {
+  uint64_t neighbor_overflow;  /* candidates within the bin spacing dropped by the cap
+                                  (the reference truncates silently, neighbors.h:223) */
+  uint64_t outside_grid;       /* particles whose cell id is not in the grid */
+  uint64_t density_hacks;      /* triggers of the near-coincidence fix (particles.h:303-305) */
+  uint64_t iterations;         /* Jacobi iterations run in the last step */
+  double max_density_error;    /* max_p max(rho_p/rho0 - 1, 0) of the last density pass */
+} pbf_stats;
+
+/* Thread-local message for the last failing call. */
+const char* pbf_last_error(void);
+
+/* Fill *p with the INI-less defaults, then derive like
+ * Parameters::derive_from_input (parameters.h:123-148): particle_radius,
+ * smoothing_radius (1.8 spacing if <= 0), neighbor_bin_spacing (1.2 h if <= 0),
+ * rest_mass = 1, rest_density = 1/spacing^3, max_speed = 0.5 h I / dt. */
+int pbf_params_derive(pbf_params* p);
+
+/* Particles(parameters) (particles.h:47-57) + Neighbors(parameters)
+ * (neighbors.h:67-76): allocates capacity max_particles_local on the device. */
+int pbf_create(const pbf_params* params, pbf_context** out);
+int pbf_destroy(pbf_context* ctx);
+
+/* Replace the parameters (the renderer re-broadcasts them,
+ * Simulation/Source/main.cpp:62-63).  Capacity may not change; a new boundary or bin
+ * spacing re-sizes the grid. */
+int pbf_set_params(pbf_context* ctx, const pbf_params* params);
+int pbf_get_params(const pbf_context* ctx, pbf_params* out);
+
+/* Particles::add(positions, position_stars, velocities, count)
+ * (particles.h:184-199); densities, lambdas and scratch start at 0. */
+int pbf_add(pbf_context* ctx, const float* positions, const float* position_stars,
+            const float* velocities, uint64_t count);
+/* Particles::remove(count) (particles.h:146-154): drop the last count particles. */
+int pbf_remove(pbf_context* ctx, uint64_t count);
+uint64_t pbf_local_count(const pbf_context* ctx);   /* particles.h:76 */
+uint64_t pbf_available(const pbf_context* ctx);     /* particles.h:81 */
+uint64_t pbf_max_local_count(const pbf_context* ctx);
+
+/* Array getters (particles.h:91-141) in original order; 3 floats per
+ * particle for vectors, 1 for scalars.  Setters overwrite all local particles. */
+int pbf_get_positions(pbf_context* ctx, float* out);
+int pbf_get_position_stars(pbf_context* ctx, float* out);
+int pbf_get_velocities(pbf_context* ctx, float* out);
+int pbf_get_densities(pbf_context* ctx, float* out);
+int pbf_get_lambdas(pbf_context* ctx, float* out);
+int pbf_get_scratch(pbf_context* ctx, float* out);
+int pbf_set_positions(pbf_context* ctx, const float* in);
+int pbf_set_position_stars(pbf_context* ctx, const float* in);
+int pbf_set_velocities(pbf_context* ctx, const float* in);
+int pbf_set_densities(pbf_context* ctx, const float* in);
+int pbf_set_lambdas(pbf_context* ctx, const float* in);
+int pbf_set_scratch(pbf_context* ctx, const float* in);
+
+/* ---- one entry per for_each_index body (spans in original indices) ---- */
+int pbf_apply_external_forces(pbf_context* ctx, pbf_index_span span);   /* particles.h:263-270 */
+int pbf_predict_positions(pbf_context* ctx, pbf_index_span span);       /* particles.h:275-287 */
+/* Particles::find_neighbors -> Neighbors::find (particles.h:256-258,
+ * neighbors.h:241-248): calculate_bins, stable sort_by_key, lower/upper
+ * bound, fill_neighbors.  bin_span.begin must be 0 (neighbors.h:130-132). */
+int pbf_find_neighbors(pbf_context* ctx, pbf_index_span bin_span, pbf_index_span fill_span);
+int pbf_compute_densities(pbf_context* ctx, pbf_index_span span);       /* particles.h:292-313 */
+int pbf_compute_pressure_lambdas(pbf_context* ctx, pbf_index_span span);/* particles.h:318-342 */
+int pbf_compute_pressure_dps(pbf_context* ctx, pbf_index_span span, int substep); /* :348-358 */
+int pbf_update_position_stars(pbf_context* ctx, pbf_index_span span);  /* particles.h:363-370 */
+int pbf_update_velocities(pbf_context* ctx, pbf_index_span span);       /* particles.h:375-385 */
+int pbf_apply_surface_tension(pbf_context* ctx, pbf_index_span color_field_span,
+                              pbf_index_span surface_tension_span);     /* particles.h:400-431 */
+int pbf_apply_viscosity(pbf_context* ctx, pbf_index_span span);         /* particles.h:436-447 */
+int pbf_compute_vorticity(pbf_context* ctx, pbf_index_span span);       /* particles.h:452-466 */
+int pbf_apply_vorticity(pbf_context* ctx, pbf_index_span span);         /* particles.h:471-487 */
+int pbf_update_positions(pbf_context* ctx, pbf_index_span span);        /* particles.h:390-394 */
+
+/* The whole step body of Simulation/Source/main.cpp:71-124 on one domain
+ * (resident = local = all particles), nsteps times, with fused kernels. */
+int pbf_step(pbf_context* ctx, uint32_t nsteps);
+
+/* Block until the context's queued work is done. */
+int pbf_sync(pbf_context* ctx);
+
+int pbf_get_stats(pbf_context* ctx, pbf_stats* out);
+
+/* ---- introspection of the neighbour structures (neighbors.h:279-284) ----
+ * Valid after pbf_find_neighbors or pbf_step. */
+int pbf_grid_dims(const pbf_context* ctx, uint64_t dims[3]);     /* neighbors.h:253-255 */
+uint64_t pbf_num_cells(const pbf_context* ctx);
+/* bin_ids_ after sort_bins: sorted cell keys (n entries) */
+int pbf_get_bin_ids(pbf_context* ctx, uint64_t* out);
+/* particle_ids_ after sort_bins: original index of each sorted slot */
+int pbf_get_particle_ids(pbf_context* ctx, uint64_t* out);
+/* begin_indices_ / end_indices_ (num_cells entries each) */
+int pbf_get_bin_bounds(pbf_context* ctx, uint64_t* begin, uint64_t* end);
+/* neighbor_lists_: out[p*max_neighbors + k] = original index of the k-th
+ * neighbour of particle p; counts[p] = list length. */
+int pbf_get_neighbor_lists(pbf_context* ctx, uint64_t* out, int32_t* counts);
+
+/* ---- sim::algorithms primitives on device memory (sim_algorithms.h) ---- */
+/* sort_by_key (sim_algorithms.h:57-61,134-137,207-210): stable LSD radix sort
+ * of n (key, value) pairs in place.  key_bits bounds the key width (1..32). */
+int pbf_sort_by_key_u32(uint32_t* d_keys, uint32_t* d_values, uint64_t n, uint32_t key_bits);
+/* lower_bound / upper_bound of the counting range [search.begin, search.end)
+ * over sorted keys (sim_algorithms.h:69-99,145-173,218-246); one result per
+ * searched value. */
+int pbf_lower_bound_counting(const uint32_t* d_sorted_keys, uint64_t n, pbf_index_span search,
+                             uint32_t* d_result);
+int pbf_upper_bound_counting(const uint32_t* d_sorted_keys, uint64_t n, pbf_index_span search,
+                             uint32_t* d_result);
+/* Device memory for the primitives. */
+int pbf_device_alloc(uint64_t bytes, void** out);
+int pbf_device_free(void* ptr);
+int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes);
+int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes);
+
+/* ---- measurement ----
+ * Bracket every launch of the kernels named in `names` (comma separated,
+ * "" = none) with HIP events on the context stream; pbf_kernel_time returns
+ * the accumulated launches and milliseconds of one kernel since enabling. */
+int pbf_profile_kernels(pbf_context* ctx, const char* names);
+int pbf_kernel_time(pbf_context* ctx, const char* name, uint64_t* launches, double* total_ms);
+/* Events around a whole region on the context stream. */
+int pbf_region_begin(pbf_context* ctx);
+int pbf_region_end(pbf_context* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBF_H */

@@ -1,0 +1,730 @@
+// pbf_oracle.cpp -- CPU oracle for the position-based-fluids substep.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+// sph-cpp_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it; the product path never calls it and never
+// falls back to it.
+//
+// It restates, on the CPU, the algorithm of AdamSimpson/SPH-CPP for one
+// compute domain (no MPI, no renderer):
+//   Simulation/Source/particles.h   (all per-particle passes, boundary conditions)
+//   Simulation/Source/neighbors.h   (bin ids, stable sort, bin bounds, capped lists)
+//   Simulation/Source/kernels.h     (Poly6, Del_Spikey, C_Spline; norms in double)
+//   Common/vec.h, Common/utility_math.h (operation order of every vector op)
+//   Common/parameters.h:123-148     (derived constants)
+//   Common/sim_algorithms.h:116-258 (OPENMP / CPP_PAR semantics, stable sort)
+//   Simulation/Source/main.cpp:65-133 (order of the passes in one step)
+// Every arithmetic expression keeps the reference's operation order and
+// types (float vs double literals) so that, with the same inputs, the float
+// instance reproduces the reference bit for bit.  It is compiled with
+// -ffp-contract=off (no fused multiply-add) on x86-64 SSE.
+//
+// Modes
+//   ORC_SERIAL  the reference's CPP_PAR backend: each pass walks particles in
+//               index order and writes in place (Gauss-Seidel XSPH,
+//               particles.h:439-446; in-place density hack, particles.h:303-305).
+//   ORC_JACOBI  each pass reads the state as it was when the pass began and
+//               writes a separate buffer: the semantics of a data-parallel
+//               device.  Per-particle arithmetic is identical.
+//   threads > 1 runs the passes under OpenMP like the reference's OPENMP
+//   backend (ORC_SERIAL then races exactly where the reference races).
+//
+// Parity pins (what checks each function; tests/test_oracle_*.py):
+//   bin_id              neighbor_test.cpp:46-121 known answers (3D) + closed form
+//   sort / bounds       numpy stable argsort / searchsorted on the same keys
+//   fill_neighbors      neighbor_test.cpp:124-298 counts; O(N^2) brute force of
+//                       the accepted set, walk order and cap truncation
+//   Poly6 / Del_Spikey  kernels_test.cpp zeros and 101^3 normalisation; Spiky
+//                       gradient against a finite difference of the textbook
+//                       Spiky kernel (Mueller et al. 2003)
+//   densities           particle_test.cpp:227-267 (mean within 5% of rest
+//                       density); two-particle closed form; hack known answer
+//   pressure lambdas    Macklin & Mueller 2013 eq. 8-11 restated independently
+//                       in numpy float64 (compressed and stretched cases)
+//   pressure dps        sum of dp over a symmetric neighbourhood is 0
+//                       (momentum); two-particle closed form sign
+//   external forces /   particle_test.cpp:122-225 known answers
+//   predict / clamp
+//   update velocities   closed-form threshold cases
+//   viscosity           uniform velocity field is a fixed point; two-particle
+//                       exchange direction; serial vs jacobi order semantics
+//   vorticity           rigid rotation: omega = -(2/3) sum_j k_j |d_j|^2 Omega
+//   apply vorticity     uniform |omega| field leaves v unchanged; gradient sign
+//   whole step          isolated particles follow the closed-form ballistic
+//                       trajectory; float vs double spread
+//   surface tension     C_Spline zeros (kernels_test.cpp:209-239); gamma = 0
+//                       is a no-op; beyond that "parity unpinned"
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+enum { ORC_SERIAL = 0, ORC_JACOBI = 1 };
+
+template <class R>
+struct V3 {
+  R x, y, z;
+};
+
+// Common/vec.h:262-290 (vec+vec, vec+scalar), :300-320 (vec-vec)
+template <class R> static inline V3<R> add(const V3<R>& a, const V3<R>& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> static inline V3<R> adds(const V3<R>& a, R s) { return {a.x + s, a.y + s, a.z + s}; }
+template <class R> static inline V3<R> sub(const V3<R>& a, const V3<R>& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+// vec.h:344-373 (vec*scalar, scalar*vec) and :375-381 (vec/scalar): per component
+template <class R> static inline V3<R> muls(const V3<R>& a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> static inline V3<R> smul(R s, const V3<R>& a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> static inline V3<R> divs(const V3<R>& a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> static inline void acc(V3<R>& a, const V3<R>& b) { a.x += b.x; a.y += b.y; a.z += b.z; }
+template <class R> static inline void dec(V3<R>& a, const V3<R>& b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; }
+// vec.h:500-508: dot accumulates from 0, left to right
+template <class R> static inline R dot(const V3<R>& a, const V3<R>& b) {
+  R d = static_cast<R>(0);
+  d += a.x * b.x;
+  d += a.y * b.y;
+  d += a.z * b.z;
+  return d;
+}
+template <class R> static inline R magsq(const V3<R>& a) { return dot(a, a); }
+// vec.h:524-528: magnitude = sqrt(magnitude_squared)
+template <class R> static inline R mag(const V3<R>& a) { return std::sqrt(magsq(a)); }
+// vec.h:620-630
+template <class R> static inline V3<R> cross(const V3<R>& l, const V3<R>& r) {
+  return {l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x};
+}
+// utility_math.h:50-57, applied per component (vec.h:669-674)
+template <class R> static inline void clamp1(R& n, R lo, R hi) {
+  if (n < lo)
+    n = lo;
+  else if (n > hi)
+    n = hi;
+}
+
+struct Params {
+  float h, s, rho0, mass, g, gamma, eps, visc, dt, max_speed, vort;
+  float bmin[3], bmax[3], mover[3];
+  int iters;
+  int cap;
+};
+
+template <class R>
+struct Sim {
+  // parameters at precision R (Parameters<Real,3> fields, parameters.h:550-573)
+  R h, s, rho0, mass, g, gamma, eps, visc, dt, max_speed, vort;
+  V3<R> bmin, bmax, mover;
+  int iters, cap, mode, threads;
+  // kernel constants (kernels.h:62, :160-161, :218)
+  R poly6_norm, spiky_norm, cspline_norm, r_eps;
+  // neighbour grid (neighbors.h:67-71)
+  uint64_t dims[3];
+  uint64_t ncells;
+  // particle state (particles.h:494-503), array-of-Vec layout like sim::Array
+  size_t n = 0;
+  std::vector<V3<R>> x, xs, v, scratch, tmpv;
+  std::vector<R> rho, lam;
+  // neighbours (neighbors.h:279-284); list entries are size_t like NeighborList
+  std::vector<uint64_t> keys, ids, begin, end, lists;
+  std::vector<int> counts;
+  long long overflow = 0, outside = 0, hacks = 0;
+
+  void init(const Params& p, int mode_, int threads_) {
+    h = (R)p.h; s = (R)p.s; rho0 = (R)p.rho0; mass = (R)p.mass; g = (R)p.g; gamma = (R)p.gamma;
+    eps = (R)p.eps; visc = (R)p.visc; dt = (R)p.dt; max_speed = (R)p.max_speed; vort = (R)p.vort;
+    bmin = {(R)p.bmin[0], (R)p.bmin[1], (R)p.bmin[2]};
+    bmax = {(R)p.bmax[0], (R)p.bmax[1], (R)p.bmax[2]};
+    mover = {(R)p.mover[0], (R)p.mover[1], (R)p.mover[2]};
+    iters = p.iters; cap = p.cap; mode = mode_; threads = threads_ < 1 ? 1 : threads_;
+    // kernels.h:62 Poly6 3D, :160 Del_Spikey 3D, :218 C_Spline 3D: norms in double, then cast
+    poly6_norm = static_cast<R>(315.0 / (64.0 * M_PI * std::pow((double)h, 9.0)));
+    spiky_norm = static_cast<R>(-45.0 / (M_PI * std::pow((double)h, 6.0)));
+    cspline_norm = static_cast<R>(32.0 / (M_PI * std::pow((double)h, 9.0)));
+    r_eps = std::numeric_limits<R>::epsilon();  // kernels.h:161
+    // neighbors.h:69-71: ceil(extent / spacing) + 2, extent = max - min (aabb.h:205-207)
+    V3<R> ext = sub(bmax, bmin);
+    dims[0] = (uint64_t)(std::ceil(ext.x / s) + (R)2);
+    dims[1] = (uint64_t)(std::ceil(ext.y / s) + (R)2);
+    dims[2] = (uint64_t)(std::ceil(ext.z / s) + (R)2);
+    ncells = dims[0] * dims[1] * dims[2];
+  }
+
+  // ---- kernels.h ---------------------------------------------------------
+  // Poly6<Real,3>::operator() (kernels.h:64-70)
+  R poly6(R r) const {
+    if (r > h) return (R)0.0;
+    return poly6_norm * (h * h - r * r) * (h * h - r * r) * (h * h - r * r);
+  }
+  // Del_Spikey<Real,3>::operator() (kernels.h:163-172)
+  V3<R> spiky(const V3<R>& p, const V3<R>& q) const {
+    const V3<R> r = sub(p, q);
+    R rm = mag(r);
+    if (rm > h) return {(R)0, (R)0, (R)0};
+    return divs(smul(spiky_norm * (h - rm) * (h - rm), r), rm + r_eps);
+  }
+  // C_Spline<Real,3>::operator() (kernels.h:220-228); note the double literal 2.0
+  R cspline(R r) const {
+    if (r > h) return (R)0.0;
+    else if (r <= h * static_cast<R>(0.5))
+      return cspline_norm * ((2.0 * (h - r) * (h - r) * (h - r) * r * r * r) -
+                             (h * h * h * h * h * h / static_cast<R>(64.0)));
+    else
+      return cspline_norm * (h - r) * (h - r) * (h - r) * r * r * r;
+  }
+
+  // ---- particles.h:510-526 apply_boundary_conditions ----------------------
+  void boundary(V3<R>& pos) const {
+    const float mover_radius = 0.2;  // float even when Real is double (particles.h:515)
+    R dr_squared = magsq(sub(pos, mover));
+    if (dr_squared < mover_radius * mover_radius) {
+      R dr = std::sqrt(dr_squared);
+      acc(pos, divs(smul((R)(mover_radius - dr), sub(pos, mover)), dr));
+    }
+    clamp1(pos.x, bmin.x, bmax.x);
+    clamp1(pos.y, bmin.y, bmax.y);
+    clamp1(pos.z, bmin.z, bmax.z);
+  }
+
+  // ---- neighbors.h -------------------------------------------------------
+  // calculate_bin_id 3D (neighbors.h:108-114). Coordinates below zero have no
+  // defined cell in the reference (size_t cast of a negative float); they are
+  // mapped past the grid here and counted in `outside`.
+  uint64_t bin_id(const V3<R>& p) const {
+    const V3<R> ps = adds(p, s);
+    const R fx = std::floor(ps.x / s), fy = std::floor(ps.y / s), fz = std::floor(ps.z / s);
+    if (!(fx >= 0) || !(fy >= 0) || !(fz >= 0)) return ~(uint64_t)0;
+    const uint64_t bx = (uint64_t)fx, by = (uint64_t)fy, bz = (uint64_t)fz;
+    return (dims[0] * dims[1] * bz) + (by * dims[0] + bx);
+  }
+
+  template <class F>
+  void for_each(size_t b, size_t e, F body) {
+    if (threads > 1) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+      for (long long p = (long long)b; p < (long long)e; ++p) body((size_t)p);
+    } else {
+      for (size_t p = b; p < e; ++p) body(p);
+    }
+  }
+
+  // calculate_bins (neighbors.h:120-125) + sort_bins (:130-132, Thrust
+  // sort_by_key == stable_sort_by_key) + find_bin_bounds (:137-148).
+  // The stable sort is done as a stable counting sort over cell ids, which
+  // yields exactly the stable_sort_by_key permutation; bounds are then the
+  // lower_bound / upper_bound of every cell id over the sorted keys.
+  void bin_and_sort(size_t count) {
+    keys.assign(count, 0);
+    ids.assign(count, 0);
+    for_each(0, count, [&](size_t i) { keys[i] = bin_id(xs[i]); });
+    begin.assign(ncells, 0);
+    end.assign(ncells, 0);
+    // keys outside [0, ncells) sort after every cell (they are never found
+    // by a bound search over [0, ncells)).
+    std::vector<uint64_t> hist(ncells + 1, 0);
+    for (size_t i = 0; i < count; ++i) hist[keys[i] < ncells ? keys[i] : ncells]++;
+    std::vector<uint64_t> off(ncells + 2, 0);
+    for (uint64_t c = 0; c <= ncells; ++c) off[c + 1] = off[c] + hist[c];
+    std::vector<uint64_t> fill(off.begin(), off.end() - 1);
+    std::vector<uint64_t> skeys(count);
+    std::vector<std::pair<uint64_t, uint64_t>> outside_keys;
+    for (size_t i = 0; i < count; ++i) {
+      const uint64_t c = keys[i] < ncells ? keys[i] : ncells;
+      const uint64_t pos = fill[c]++;
+      ids[pos] = i;
+      skeys[pos] = keys[i];
+    }
+    // outside keys: stable order by their actual key value
+    if (hist[ncells] > 1) {
+      std::stable_sort(ids.begin() + off[ncells], ids.end(),
+                       [&](uint64_t a, uint64_t b) { return keys[a] < keys[b]; });
+      for (uint64_t pos = off[ncells]; pos < count; ++pos) skeys[pos] = keys[ids[pos]];
+    }
+    outside = (long long)hist[ncells];
+    for (uint64_t c = 0; c < ncells; ++c) {
+      begin[c] = off[c];
+      end[c] = off[c + 1];
+    }
+    keys.swap(skeys);  // the reference sorts bin_ids_ in place
+  }
+
+  // calculate_neighbor_indices 3D (neighbors.h:171-185): cell ids from FP
+  // coordinates x + i*s, walk order i (x) outer, j (y), k (z) inner.
+  void neighbor_cells(const V3<R>& c, uint64_t* out) const {
+    int idx = 0;
+    for (int i = -1; i < 2; i++)
+      for (int j = -1; j < 2; j++)
+        for (int k = -1; k < 2; k++) {
+          const V3<R> nc = {c.x + i * s, c.y + j * s, c.z + k * s};
+          out[idx++] = bin_id(nc);
+        }
+  }
+
+  // fill_neighbors (neighbors.h:197-230): skip self, accept d^2 < s^2 while
+  // count < cap.  Candidates rejected by the cap are counted in `overflow`.
+  void fill_neighbors(size_t b, size_t e) {
+    lists.assign(n * (size_t)cap, 0);
+    counts.assign(n, 0);
+    const R valid = s * s;
+    long long ovf = 0;
+    auto body = [&](size_t p) {
+      const V3<R> ps = xs[p];
+      int count = 0;
+      long long o = 0;
+      uint64_t cells[27];
+      neighbor_cells(ps, cells);
+      for (int c = 0; c < 27; ++c) {
+        const uint64_t cell = cells[c];
+        if (cell >= ncells) continue;  // reference: out-of-range read (undefined)
+        for (uint64_t jj = begin[cell]; jj < end[cell]; ++jj) {
+          const uint64_t q = ids[jj];
+          if (p == q) continue;
+          const R d2 = magsq(sub(ps, xs[q]));
+          if (d2 < valid && count < cap) {
+            lists[p * (size_t)cap + count] = q;
+            ++count;
+          } else if (d2 < valid) {
+            ++o;
+          }
+        }
+      }
+      counts[p] = count;
+      return o;
+    };
+    if (threads > 1) {
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : ovf)
+      for (long long p = (long long)b; p < (long long)e; ++p) ovf += body((size_t)p);
+    } else {
+      for (size_t p = b; p < e; ++p) ovf += body(p);
+    }
+    overflow = ovf;
+  }
+
+  // Neighbors::find (neighbors.h:241-248); bin span must start at 0 because
+  // sort_bins sorts [0, count) (neighbors.h:130-132).
+  int find_neighbors(size_t bb, size_t be, size_t fb, size_t fe) {
+    if (bb != 0 || be > n || fe > n || fb > fe) return -1;
+    bin_and_sort(be - bb);
+    fill_neighbors(fb, fe);
+    return 0;
+  }
+
+  // ---- particles.h passes -----------------------------------------------
+  // apply_external_forces (particles.h:263-270)
+  void apply_external_forces(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) { v[p].y += g * dt; });
+  }
+  // predict_positions (particles.h:275-287)
+  void predict_positions(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) {
+      V3<R> ps = add(x[p], muls(v[p], dt));
+      boundary(ps);
+      xs[p] = ps;
+    });
+  }
+  // compute_densities (particles.h:292-313), including the in-place "hack"
+  // (:303-305).  ORC_JACOBI reads x* from a copy taken when the pass starts.
+  void compute_densities(size_t b, size_t e) {
+    const R W0 = poly6((R)0.0);
+    std::vector<V3<R>> src;
+    if (mode == ORC_JACOBI) src = xs;
+    const std::vector<V3<R>>& rd = (mode == ORC_JACOBI) ? src : xs;
+    long long hk = 0;
+    auto body = [&](size_t p) {
+      R density = mass * W0;
+      V3<R> xp = rd[p];
+      long long lh = 0;
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        if (mag(sub(xp, rd[q])) < 0.00000001) {
+          dec(xp, divs(muls(v[p], dt), (R)50.0));
+          ++lh;
+          if (mode == ORC_SERIAL) xs[p] = xp;
+        }
+        if (mode == ORC_SERIAL) xp = xs[p];
+        const R r = mag(sub(xp, rd[q]));
+        density += mass * poly6(r);
+      }
+      rho[p] = density;
+      if (mode == ORC_JACOBI) xs[p] = xp;
+      return lh;
+    };
+    if (threads > 1) {
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : hk)
+      for (long long p = (long long)b; p < (long long)e; ++p) hk += body((size_t)p);
+    } else {
+      for (size_t p = b; p < e; ++p) hk += body(p);
+    }
+    hacks += hk;
+  }
+  // compute_pressure_lambdas (particles.h:318-342)
+  void compute_pressure_lambdas(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) {
+      const R constraint = rho[p] / rho0 - static_cast<R>(1.0);
+      const R C_p = (constraint < 0.0 ? 0.0 : constraint);
+      R sum_C = 0.0;
+      V3<R> sum_gradient = {(R)0, (R)0, (R)0};
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        const R density_0 = rho0;
+        const V3<R> gradient = smul((R)-1.0 / density_0, spiky(xs[p], xs[q]));
+        dec(sum_gradient, gradient);
+        sum_C += magsq(gradient);
+      }
+      sum_C += magsq(sum_gradient);
+      lam[p] = -C_p / (sum_C + eps);
+    });
+  }
+  // compute_pressure_dps (particles.h:348-358): dp -> scratch_
+  void compute_pressure_dps(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) {
+      V3<R> dp = {(R)0, (R)0, (R)0};
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        acc(dp, smul(lam[p] + lam[q], spiky(xs[p], xs[q])));
+      }
+      scratch[p] = smul((R)1.0 / rho0, dp);
+    });
+  }
+  // update_position_stars (particles.h:363-370)
+  void update_position_stars(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) {
+      V3<R> nw = add(xs[p], scratch[p]);
+      boundary(nw);
+      xs[p] = nw;
+    });
+  }
+  // update_velocities (particles.h:375-385); threshold compared in double
+  void update_velocities(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) {
+      V3<R> vel = divs(sub(xs[p], x[p]), dt);
+      if (magsq(vel) < 0.000001 * max_speed) vel = {(R)0, (R)0, (R)0};
+      v[p] = vel;
+    });
+  }
+  // update_positions (particles.h:390-394)
+  void update_positions(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) { x[p] = xs[p]; });
+  }
+  // apply_viscosity (particles.h:436-447): in place (ORC_SERIAL) or from a
+  // copy of the velocities taken when the pass starts (ORC_JACOBI)
+  void apply_viscosity(size_t b, size_t e) {
+    std::vector<V3<R>> src;
+    if (mode == ORC_JACOBI) src = v;
+    const std::vector<V3<R>>& rv = (mode == ORC_JACOBI) ? src : v;
+    for_each(b, e, [&](size_t p) {
+      V3<R> dv = {(R)0, (R)0, (R)0};
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        const R r = mag(sub(xs[p], xs[q]));
+        acc(dv, divs(muls(sub(rv[q], rv[p]), poly6(r)), rho[q]));
+      }
+      acc(v[p], smul(visc, dv));
+    });
+  }
+  // compute_vorticity (particles.h:452-466): omega -> scratch_
+  void compute_vorticity(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) {
+      V3<R> w = {(R)0, (R)0, (R)0};
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        const V3<R> del = spiky(xs[p], xs[q]);
+        const V3<R> vd = sub(v[q], v[p]);
+        acc(w, cross(vd, del));
+      }
+      scratch[p] = w;
+    });
+  }
+  // apply_vorticity (particles.h:471-487)
+  void apply_vorticity(size_t b, size_t e) {
+    for_each(b, e, [&](size_t p) {
+      V3<R> eta = {(R)0, (R)0, (R)0};
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        const V3<R> del = spiky(xs[p], xs[q]);
+        const R wm = mag(scratch[q]);
+        acc(eta, smul(wm, del));
+      }
+      const V3<R> N = divs(eta, mag(eta) + std::numeric_limits<R>::epsilon());
+      acc(v[p], muls(muls(cross(N, scratch[p]), vort), dt));
+    });
+  }
+  // apply_surface_tension (particles.h:400-431): colour field -> scratch_,
+  // then cohesion + curvature force into v
+  void apply_surface_tension(size_t cb, size_t ce, size_t b, size_t e) {
+    for_each(cb, ce, [&](size_t p) {
+      V3<R> color = {(R)0, (R)0, (R)0};
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        acc(color, divs(spiky(xs[p], xs[q]), rho[q]));
+      }
+      scratch[p] = smul(h, color);
+    });
+    for_each(b, e, [&](size_t p) {
+      V3<R> f = {(R)0, (R)0, (R)0};
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        const V3<R> r = sub(xs[p], xs[q]);
+        R r_mag = mag(r);
+        if (r_mag < h * 0.000001) r_mag = h * 0.000001;
+        const V3<R> cohesion = divs(smul(-gamma * cspline(r_mag), r), r_mag);
+        const V3<R> curvature = smul(-gamma, sub(scratch[p], scratch[q]));
+        const R K = 2.0 * rho0 / (rho[p] + rho[q]);
+        acc(f, smul(K, add(cohesion, curvature)));
+      }
+      acc(v[p], muls(divs(f, rho[p]), dt));
+    });
+  }
+
+  // max density error used by iterate-to-tolerance solves: max over p of
+  // max(rho/rho0 - 1, 0), the same clamp as particles.h:323-325
+  double emax(size_t b, size_t e) const {
+    double m = 0.0;
+    for (size_t p = b; p < e; ++p) {
+      const R c = rho[p] / rho0 - static_cast<R>(1.0);
+      if ((double)c > m) m = (double)c;
+    }
+    return m;
+  }
+
+  // One step of Simulation/Source/main.cpp:71-124 on a single domain
+  // (resident = local = [0, n)).  Surface tension runs only for gamma != 0.
+  void step() {
+    apply_external_forces(0, n);
+    predict_positions(0, n);
+    find_neighbors(0, n, 0, n);
+    for (int sub = 0; sub < iters; sub++) {
+      compute_densities(0, n);
+      compute_pressure_lambdas(0, n);
+      compute_pressure_dps(0, n);
+      update_position_stars(0, n);
+    }
+    update_velocities(0, n);
+    if (gamma != (R)0) apply_surface_tension(0, n, 0, n);
+    apply_viscosity(0, n);
+    compute_vorticity(0, n);
+    apply_vorticity(0, n);
+    apply_viscosity(0, n);
+    update_positions(0, n);
+  }
+};
+
+struct Handle {
+  int dbl;
+  Sim<float> f;
+  Sim<double> d;
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+template <class R, class T>
+static void load(Sim<R>& S, size_t n, const T* x, const T* xs, const T* v) {
+  S.n = n;
+  S.x.resize(n); S.xs.resize(n); S.v.resize(n);
+  S.scratch.assign(n, {(R)0, (R)0, (R)0});
+  S.rho.assign(n, (R)0); S.lam.assign(n, (R)0);
+  for (size_t i = 0; i < n; ++i) {
+    S.x[i] = {(R)x[3 * i], (R)x[3 * i + 1], (R)x[3 * i + 2]};
+    S.xs[i] = {(R)xs[3 * i], (R)xs[3 * i + 1], (R)xs[3 * i + 2]};
+    S.v[i] = {(R)v[3 * i], (R)v[3 * i + 1], (R)v[3 * i + 2]};
+  }
+  S.lists.assign(n * (size_t)S.cap, 0);
+  S.counts.assign(n, 0);
+}
+
+template <class R>
+static void dump_vec(const std::vector<V3<R>>& a, double* out) {
+  for (size_t i = 0; i < a.size(); ++i) {
+    out[3 * i] = (double)a[i].x; out[3 * i + 1] = (double)a[i].y; out[3 * i + 2] = (double)a[i].z;
+  }
+}
+
+template <class R>
+static int do_pass(Sim<R>& S, int id, size_t b, size_t e, size_t b2, size_t e2) {
+  if (b > e || e > S.n) return -1;
+  switch (id) {
+    case 0: S.apply_external_forces(b, e); break;
+    case 1: S.predict_positions(b, e); break;
+    case 2: return S.find_neighbors(b, e, b2, e2);
+    case 3: S.compute_densities(b, e); break;
+    case 4: S.compute_pressure_lambdas(b, e); break;
+    case 5: S.compute_pressure_dps(b, e); break;
+    case 6: S.update_position_stars(b, e); break;
+    case 7: S.update_velocities(b, e); break;
+    case 8: S.apply_surface_tension(b, e, b2, e2); break;
+    case 9: S.apply_viscosity(b, e); break;
+    case 10: S.compute_vorticity(b, e); break;
+    case 11: S.apply_vorticity(b, e); break;
+    case 12: S.update_positions(b, e); break;
+    default: return -2;
+  }
+  return 0;
+}
+
+extern "C" {
+
+void* orc_create(const Params* p, int use_double, int mode, int threads) {
+  Handle* H = new Handle();
+  H->dbl = use_double;
+  if (use_double) H->d.init(*p, mode, threads);
+  else H->f.init(*p, mode, threads);
+  return H;
+}
+
+void orc_destroy(void* h) { delete (Handle*)h; }
+
+void orc_set_threads(void* h, int threads) {
+  Handle* H = (Handle*)h;
+  H->f.threads = H->d.threads = threads < 1 ? 1 : threads;
+}
+
+void orc_set_mode(void* h, int mode) {
+  Handle* H = (Handle*)h;
+  H->f.mode = H->d.mode = mode;
+}
+
+int orc_set_state(void* h, size_t n, const float* x, const float* xs, const float* v) {
+  Handle* H = (Handle*)h;
+  if (H->dbl) load(H->d, n, x, xs, v);
+  else load(H->f, n, x, xs, v);
+  return 0;
+}
+
+int orc_set_state_f64(void* h, size_t n, const double* x, const double* xs, const double* v) {
+  Handle* H = (Handle*)h;
+  if (H->dbl) load(H->d, n, x, xs, v);
+  else load(H->f, n, x, xs, v);
+  return 0;
+}
+
+// x, xs, v, scratch as 3n doubles; rho, lam as n doubles (any may be NULL)
+int orc_get_state(void* h, double* x, double* xs, double* v, double* rho, double* lam, double* scratch) {
+  Handle* H = (Handle*)h;
+  auto get = [&](auto& S) {
+    if (x) dump_vec(S.x, x);
+    if (xs) dump_vec(S.xs, xs);
+    if (v) dump_vec(S.v, v);
+    if (scratch) dump_vec(S.scratch, scratch);
+    for (size_t i = 0; i < S.n; ++i) {
+      if (rho) rho[i] = (double)S.rho[i];
+      if (lam) lam[i] = (double)S.lam[i];
+    }
+  };
+  if (H->dbl) get(H->d); else get(H->f);
+  return 0;
+}
+
+// overwrite rho / lam / scratch (test set-up for individual passes)
+int orc_set_aux(void* h, const double* rho, const double* lam, const double* scratch) {
+  Handle* H = (Handle*)h;
+  auto set = [&](auto& S) {
+    using R = typename std::remove_reference<decltype(S.rho[0])>::type;
+    for (size_t i = 0; i < S.n; ++i) {
+      if (rho) S.rho[i] = (R)rho[i];
+      if (lam) S.lam[i] = (R)lam[i];
+      if (scratch) S.scratch[i] = {(R)scratch[3 * i], (R)scratch[3 * i + 1], (R)scratch[3 * i + 2]};
+    }
+  };
+  if (H->dbl) set(H->d); else set(H->f);
+  return 0;
+}
+
+int orc_pass(void* h, int id, size_t b, size_t e, size_t b2, size_t e2) {
+  Handle* H = (Handle*)h;
+  return H->dbl ? do_pass(H->d, id, b, e, b2, e2) : do_pass(H->f, id, b, e, b2, e2);
+}
+
+int orc_step(void* h, int nsteps) {
+  Handle* H = (Handle*)h;
+  for (int i = 0; i < nsteps; ++i) {
+    if (H->dbl) H->d.step(); else H->f.step();
+  }
+  return 0;
+}
+
+void orc_dims(void* h, uint64_t* dims) {
+  Handle* H = (Handle*)h;
+  for (int i = 0; i < 3; ++i) dims[i] = H->dbl ? H->d.dims[i] : H->f.dims[i];
+}
+
+// sorted keys / ids (n each), begin / end (ncells each), lists (n*cap) and
+// counts (n), in the reference's original particle indices
+int orc_get_neighbors(void* h, uint64_t* keys, uint64_t* ids, uint64_t* begin, uint64_t* end,
+                      uint64_t* lists, int32_t* counts) {
+  Handle* H = (Handle*)h;
+  auto get = [&](auto& S) {
+    if (keys) std::memcpy(keys, S.keys.data(), S.keys.size() * 8);
+    if (ids) std::memcpy(ids, S.ids.data(), S.ids.size() * 8);
+    if (begin) std::memcpy(begin, S.begin.data(), S.begin.size() * 8);
+    if (end) std::memcpy(end, S.end.data(), S.end.size() * 8);
+    if (lists) std::memcpy(lists, S.lists.data(), S.lists.size() * 8);
+    if (counts) std::memcpy(counts, S.counts.data(), S.counts.size() * 4);
+  };
+  if (H->dbl) get(H->d); else get(H->f);
+  return 0;
+}
+
+// overflow (cap rejections), outside-grid keys, density-hack triggers
+void orc_counters(void* h, long long* out) {
+  Handle* H = (Handle*)h;
+  if (H->dbl) { out[0] = H->d.overflow; out[1] = H->d.outside; out[2] = H->d.hacks; }
+  else { out[0] = H->f.overflow; out[1] = H->f.outside; out[2] = H->f.hacks; }
+}
+
+double orc_emax(void* h) {
+  Handle* H = (Handle*)h;
+  return H->dbl ? H->d.emax(0, H->d.n) : H->f.emax(0, H->f.n);
+}
+
+// --- single-function probes for the kernel / binning pins -----------------
+double orc_poly6(void* h, double r) {
+  Handle* H = (Handle*)h;
+  return H->dbl ? H->d.poly6(r) : (double)H->f.poly6((float)r);
+}
+void orc_spiky(void* h, const double* p, const double* q, double* out) {
+  Handle* H = (Handle*)h;
+  if (H->dbl) {
+    V3<double> r = H->d.spiky({p[0], p[1], p[2]}, {q[0], q[1], q[2]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+  } else {
+    V3<float> r = H->f.spiky({(float)p[0], (float)p[1], (float)p[2]}, {(float)q[0], (float)q[1], (float)q[2]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+  }
+}
+double orc_cspline(void* h, double r) {
+  Handle* H = (Handle*)h;
+  return H->dbl ? H->d.cspline(r) : (double)H->f.cspline((float)r);
+}
+uint64_t orc_bin_id(void* h, const double* p) {
+  Handle* H = (Handle*)h;
+  if (H->dbl) return H->d.bin_id({p[0], p[1], p[2]});
+  return H->f.bin_id({(float)p[0], (float)p[1], (float)p[2]});
+}
+void orc_boundary(void* h, double* p) {
+  Handle* H = (Handle*)h;
+  if (H->dbl) {
+    V3<double> a = {p[0], p[1], p[2]};
+    H->d.boundary(a);
+    p[0] = a.x; p[1] = a.y; p[2] = a.z;
+  } else {
+    V3<float> a = {(float)p[0], (float)p[1], (float)p[2]};
+    H->f.boundary(a);
+    p[0] = a.x; p[1] = a.y; p[2] = a.z;
+  }
+}
+
+}  // extern "C"

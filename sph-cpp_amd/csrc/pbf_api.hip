@@ -1,0 +1,739 @@
+// pbf_api.hip -- C ABI of the PBF library: contexts, buffers, passes, step.
+//
+// Reference call surface: Common/sim_algorithms.h (primitives),
+// Simulation/Source/particles.h and neighbors.h (passes), Common/array.h and
+// Common/managed_allocation.h (storage, replaced by device buffers),
+// Common/parameters.h:123-148 (derived constants), Simulation/Source/main.cpp:71-124
+// (step order).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "pbf_internal.h"
+
+namespace pbf {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return PBF_OK;
+  return set_error(PBF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+KernelTimer* timer_begin(pbf_context* c, const char* name) {
+  for (auto& t : c->timers) {
+    if (t.name == name) {
+      hipEvent_t a, b;
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      hipEventRecord(a, c->stream);
+      t.pending.push_back({a, b});
+      return &t;
+    }
+  }
+  return nullptr;
+}
+
+void timer_end(pbf_context* c, KernelTimer* t) {
+  if (t) hipEventRecord(t->pending.back().second, c->stream);
+}
+
+static void timers_resolve(pbf_context* c) {
+  for (auto& t : c->timers) {
+    for (auto& pr : t.pending) {
+      hipEventSynchronize(pr.second);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, pr.first, pr.second);
+      t.total_ms += ms;
+      t.launches++;
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+    t.pending.clear();
+  }
+}
+
+// smallest float t with sqrtf(t) > h (sqrt with round-to-nearest is monotone)
+static float threshold_sqrt_gt(float h) {
+  float t = h * h;
+  while (t > 0.f && std::sqrt(std::nextafter(t, 0.f)) > h) t = std::nextafter(t, 0.f);
+  while (!(std::sqrt(t) > h)) t = std::nextafter(t, std::numeric_limits<float>::infinity());
+  while (t > 0.f && std::sqrt(std::nextafter(t, 0.f)) > h) t = std::nextafter(t, 0.f);
+  return t;
+}
+
+// smallest float t with (double)sqrtf(t) >= 1e-8 (particles.h:304 compares in double)
+static float threshold_hack() {
+  float t = (float)(1e-8 * 1e-8);
+  while (t > 0.f && (double)std::sqrt(std::nextafter(t, 0.f)) >= 0.00000001) t = std::nextafter(t, 0.f);
+  while (!((double)std::sqrt(t) >= 0.00000001)) t = std::nextafter(t, std::numeric_limits<float>::infinity());
+  while (t > 0.f && (double)std::sqrt(std::nextafter(t, 0.f)) >= 0.00000001) t = std::nextafter(t, 0.f);
+  return t;
+}
+
+static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
+  const float h = P.smoothing_radius, s = P.neighbor_bin_spacing;
+  if (!(h > 0.f) || !(s > 0.f) || !std::isfinite(h) || !std::isfinite(s))
+    return set_error(PBF_ERR_ARG, "smoothing_radius and neighbor_bin_spacing must be positive");
+  if (P.boundary_min[0] != 0.f || P.boundary_min[1] != 0.f || P.boundary_min[2] != 0.f)
+    return set_error(PBF_ERR_UNSUPPORTED,
+                     "boundary_min must be the origin: bin ids add the spacing instead of subtracting the minimum "
+                     "(neighbors.h:110-113)");
+  if (P.max_neighbors < 1 || P.max_neighbors > 128)
+    return set_error(PBF_ERR_ARG, "max_neighbors must be in [1, 128]");
+  if (!(P.time_step != 0.f)) return set_error(PBF_ERR_ARG, "time_step must be non-zero");
+  std::memset(&K, 0, sizeof(K));
+  K.h = h;
+  K.hh = h * h;
+  K.s = s;
+  K.ss = s * s;
+  K.T_h = threshold_sqrt_gt(h);
+  K.T_hack = threshold_hack();
+  // kernels.h:62, :160-161, :218 -- norms computed in double, then cast
+  K.poly6_norm = static_cast<float>(315.0 / (64.0 * M_PI * std::pow((double)h, 9.0)));
+  K.spiky_norm = static_cast<float>(-45.0 / (M_PI * std::pow((double)h, 6.0)));
+  K.cspline_norm = static_cast<float>(32.0 / (M_PI * std::pow((double)h, 9.0)));
+  K.r_eps = std::numeric_limits<float>::epsilon();
+  K.mass = P.rest_mass;
+  {
+    const float r = 0.0f;  // W(0), particles.h:294
+    const float a = h * h - r * r;
+    K.mass_W0 = P.rest_mass * (K.poly6_norm * a * a * a);
+  }
+  K.rho0 = P.rest_density;
+  K.neg_inv_rho0 = (float)-1.0 / P.rest_density;  // particles.h:332
+  K.inv_rho0 = (float)1.0 / P.rest_density;       // particles.h:356
+  K.g = P.gravity;
+  K.dt = P.time_step;
+  K.g_dt = P.gravity * P.time_step;
+  K.eps = P.lambda_epsilon;
+  K.visc = P.visc_c;
+  K.vort = P.vorticity_coef;
+  K.gamma = P.gamma;
+  K.vthr = 0.000001 * (double)P.max_speed;         // particles.h:380
+  K.st_rmin = (double)P.smoothing_radius * 0.000001;  // particles.h:420
+  for (int i = 0; i < 3; ++i) {
+    K.bmin[i] = P.boundary_min[i];
+    K.bmax[i] = P.boundary_max[i];
+    K.mover[i] = P.mover_center[i];
+  }
+  // neighbors.h:69-71: ceil(extent / spacing) + 2 in float, then size_t
+  for (int i = 0; i < 3; ++i) {
+    const float ext = P.boundary_max[i] - P.boundary_min[i];
+    const float d = std::ceil(ext / s) + 2.0f;
+    if (!(d >= 1.f) || !(d < 4.0e9f)) return set_error(PBF_ERR_ARG, "boundary / spacing gives no valid grid");
+    dims[i] = (uint64_t)d;
+  }
+  const double cells = (double)dims[0] * (double)dims[1] * (double)dims[2];
+  if (cells > 4.0e9) return set_error(PBF_ERR_UNSUPPORTED, "grid has more than 4e9 cells");
+  K.dimx = (uint32_t)dims[0];
+  K.dimy = (uint32_t)dims[1];
+  K.dimz = (uint32_t)dims[2];
+  K.dxy = dims[0] * dims[1];
+  K.ncells = (uint32_t)(dims[0] * dims[1] * dims[2]);
+  K.cap = (int)P.max_neighbors;
+  return PBF_OK;
+}
+
+template <class T>
+static int dalloc(T** p, uint64_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return check_hip(hipMalloc((void**)p, count * sizeof(T)), "hipMalloc");
+}
+
+static void dfree(void* p) {
+  if (p) hipFree(p);
+}
+
+static int alloc_grid(pbf_context* c) {
+  const uint64_t need = c->ncells + 2;
+  if (need <= c->grid_alloc) return PBF_OK;
+  dfree(c->cell_count);
+  dfree(c->cell_start);
+  dfree(c->scan_partials);
+  c->cell_count = c->cell_start = c->scan_partials = nullptr;
+  int rc;
+  if ((rc = dalloc(&c->cell_count, need)) || (rc = dalloc(&c->cell_start, need)) ||
+      (rc = dalloc(&c->scan_partials, scan_partials_size(need))))
+    return rc;
+  c->grid_alloc = need;
+  return PBF_OK;
+}
+
+static void free_all(pbf_context* c) {
+  void* ptrs[] = {c->x,    c->xs,   c->v,    c->xa,   c->xsa,  c->va,   c->xt,         c->scr,
+                  c->scra, c->rho,  c->lam,  c->rhoa, c->lama, c->oid,  c->oida,       c->key,
+                  c->keya, c->slot, c->tmp,  c->tmpo, c->nbr,  c->nbr_count, c->counters, c->stage,
+                  c->stage64, c->cell_count, c->cell_start, c->scan_partials};
+  for (void* p : ptrs) dfree(p);
+}
+
+static bool span_ok(const pbf_context* c, pbf_index_span s) { return s.begin <= s.end && s.end <= c->n; }
+
+static int need_lists(pbf_context* c) {
+  if (!c->lists_valid) return set_error(PBF_ERR_STATE, "neighbour pass before find_neighbors (or after add/remove)");
+  return PBF_OK;
+}
+
+// iota of original ids for freshly added slots
+__global__ void k_iota(uint32_t* p, uint64_t first, uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[first + i] = (uint32_t)(first + i);
+}
+__global__ void k_zero_f4(float4* p, uint64_t first, uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[first + i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__global__ void k_zero_f1(float* p, uint64_t first, uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[first + i] = 0.f;
+}
+// keep-flags for remove(): 1 when the original index survives
+__global__ void k_keep(uint32_t n, const uint32_t* __restrict__ oid, uint32_t newn, uint32_t* __restrict__ flag) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) flag[s] = oid[s] < newn ? 1u : 0u;
+}
+__global__ void k_compact_perm(uint32_t n, const uint32_t* __restrict__ oid, uint32_t newn,
+                               const uint32_t* __restrict__ pos, uint32_t* __restrict__ perm) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n && oid[s] < newn) perm[pos[s]] = s;
+}
+
+// neighbour search body shared by pbf_find_neighbors and pbf_step
+static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe);
+
+}  // namespace pbf
+
+using namespace pbf;
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* pbf_last_error(void) { return g_last_error.c_str(); }
+
+int pbf_params_derive(pbf_params* p) {
+  if (!p) return set_error(PBF_ERR_ARG, "null params");
+  // parameters.h:123-148
+  if (!(p->particle_rest_spacing > 0.f)) return set_error(PBF_ERR_ARG, "particle_rest_spacing must be positive");
+  p->particle_radius = (float)(p->particle_rest_spacing / 2.0);
+  if (p->smoothing_radius <= 0.0) p->smoothing_radius = (float)(1.8 * p->particle_rest_spacing);
+  if (p->neighbor_bin_spacing <= 0.0) p->neighbor_bin_spacing = (float)(1.2 * p->smoothing_radius);
+  p->rest_mass = 1.0f;
+  const float particle_volume = std::pow(p->particle_rest_spacing, (float)3);
+  p->rest_density = p->rest_mass / particle_volume;
+  p->max_speed = (float)(0.5 * p->smoothing_radius * p->solve_step_count / p->time_step);
+  if (p->max_neighbors == 0) p->max_neighbors = 60;  // MAX_NEIGHBORS (neighbors.h:39)
+  return PBF_OK;
+}
+
+int pbf_create(const pbf_params* params, pbf_context** out) {
+  if (!params || !out) return set_error(PBF_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (params->max_particles_local == 0 || params->max_particles_local >= (1ull << 31))
+    return set_error(PBF_ERR_ARG, "max_particles_local must be in [1, 2^31)");
+  pbf_context* c = new pbf_context();
+  c->P = *params;
+  int rc = compute_kparams(c->P, c->K, c->dims);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  c->ncells = c->K.ncells;
+  c->cap = c->K.cap;
+  c->capacity = params->max_particles_local;
+  if ((rc = check_hip(hipGetDevice(&c->device), "hipGetDevice")) ||
+      (rc = check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate"))) {
+    delete c;
+    return rc;
+  }
+  const uint64_t N = c->capacity;
+  if ((rc = dalloc(&c->x, N)) || (rc = dalloc(&c->xs, N)) || (rc = dalloc(&c->v, N)) || (rc = dalloc(&c->xa, N)) ||
+      (rc = dalloc(&c->xsa, N)) || (rc = dalloc(&c->va, N)) || (rc = dalloc(&c->xt, N)) ||
+      (rc = dalloc(&c->scr, N)) || (rc = dalloc(&c->scra, N)) || (rc = dalloc(&c->rho, N)) ||
+      (rc = dalloc(&c->lam, N)) || (rc = dalloc(&c->rhoa, N)) || (rc = dalloc(&c->lama, N)) ||
+      (rc = dalloc(&c->oid, N)) || (rc = dalloc(&c->oida, N)) || (rc = dalloc(&c->key, N)) ||
+      (rc = dalloc(&c->keya, N)) || (rc = dalloc(&c->slot, N)) || (rc = dalloc(&c->tmp, N + 1)) ||
+      (rc = dalloc(&c->tmpo, N + 1)) || (rc = dalloc(&c->nbr, N * (uint64_t)c->cap)) ||
+      (rc = dalloc(&c->nbr_count, N)) || (rc = dalloc(&c->counters, (uint64_t)C_NUM)) ||
+      (rc = dalloc(&c->stage, 4 * N)) || (rc = dalloc(&c->stage64, N + 1)) || (rc = alloc_grid(c)) ||
+      (rc = check_hip(hipEventCreate(&c->region_a), "hipEventCreate")) ||
+      (rc = check_hip(hipEventCreate(&c->region_b), "hipEventCreate"))) {
+    free_all(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
+  launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
+  launch_fill_u32((uint32_t*)c->nbr_count, 0u, N, c->stream);
+  *out = c;
+  return check_hip(hipStreamSynchronize(c->stream), "pbf_create");
+}
+
+int pbf_destroy(pbf_context* c) {
+  if (!c) return PBF_OK;
+  hipStreamSynchronize(c->stream);
+  timers_resolve(c);
+  free_all(c);
+  if (c->region_a) hipEventDestroy(c->region_a);
+  if (c->region_b) hipEventDestroy(c->region_b);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return PBF_OK;
+}
+
+int pbf_set_params(pbf_context* c, const pbf_params* params) {
+  if (!c || !params) return set_error(PBF_ERR_ARG, "null argument");
+  if (params->max_particles_local != c->capacity || (int)params->max_neighbors != c->cap)
+    return set_error(PBF_ERR_ARG, "capacity and max_neighbors are fixed at pbf_create");
+  KParams K;
+  uint64_t dims[3];
+  int rc = compute_kparams(*params, K, dims);
+  if (rc) return rc;
+  const bool grid_changed = K.ncells != c->K.ncells || K.s != c->K.s || K.dimx != c->K.dimx || K.dimy != c->K.dimy;
+  c->P = *params;
+  c->K = K;
+  for (int i = 0; i < 3; ++i) c->dims[i] = dims[i];
+  c->ncells = K.ncells;
+  if (grid_changed) {
+    c->lists_valid = false;
+    if ((rc = check_hip(hipStreamSynchronize(c->stream), "sync")) || (rc = alloc_grid(c))) return rc;
+  }
+  return PBF_OK;
+}
+
+int pbf_get_params(const pbf_context* c, pbf_params* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  *out = c->P;
+  return PBF_OK;
+}
+
+uint64_t pbf_local_count(const pbf_context* c) { return c ? c->n : 0; }
+uint64_t pbf_available(const pbf_context* c) { return c ? c->capacity - c->n : 0; }
+uint64_t pbf_max_local_count(const pbf_context* c) { return c ? c->capacity : 0; }
+
+int pbf_add(pbf_context* c, const float* pos, const float* pos_star, const float* vel, uint64_t count) {
+  if (!c || (count && (!pos || !pos_star || !vel))) return set_error(PBF_ERR_ARG, "null argument");
+  if (c->n + count > c->capacity) return set_error(PBF_ERR_CAPACITY, "Not enough capacity to push_back");
+  if (!count) return PBF_OK;
+  const uint64_t first = c->n;
+  int rc;
+  const size_t w = 3 * sizeof(float);
+  if ((rc = check_hip(hipMemcpy2DAsync(c->x + first, sizeof(float4), pos, w, w, count, hipMemcpyHostToDevice,
+                                       c->stream), "add positions")) ||
+      (rc = check_hip(hipMemcpy2DAsync(c->xs + first, sizeof(float4), pos_star, w, w, count,
+                                       hipMemcpyHostToDevice, c->stream), "add position stars")) ||
+      (rc = check_hip(hipMemcpy2DAsync(c->v + first, sizeof(float4), vel, w, w, count, hipMemcpyHostToDevice,
+                                       c->stream), "add velocities")))
+    return rc;
+  const uint32_t nb = (uint32_t)((count + 255) / 256);
+  hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, c->stream, c->oid, first, count);
+  hipLaunchKernelGGL(k_zero_f4, dim3(nb), dim3(256), 0, c->stream, c->scr, first, count);
+  hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, c->rho, first, count);
+  hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, c->lam, first, count);
+  hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, (float*)c->nbr_count, first, count);
+  c->n += count;
+  c->lists_valid = false;
+  return check_hip(hipStreamSynchronize(c->stream), "pbf_add");
+}
+
+int pbf_remove(pbf_context* c, uint64_t count) {
+  if (!c) return set_error(PBF_ERR_ARG, "null argument");
+  if (count > c->n) return set_error(PBF_ERR_ARG, "remove: count exceeds local count");
+  if (!count) return PBF_OK;
+  const uint32_t n = (uint32_t)c->n, newn = (uint32_t)(c->n - count);
+  const uint32_t nb = (n + 255) / 256;
+  // flags -> positions (exclusive scan) -> perm -> gather
+  hipLaunchKernelGGL(k_keep, dim3(nb), dim3(256), 0, c->stream, n, (const uint32_t*)c->oid, newn, c->tmp);
+  launch_exclusive_scan(c->tmp, c->tmpo, n, c->scan_partials, c->stream);
+  hipLaunchKernelGGL(k_compact_perm, dim3(nb), dim3(256), 0, c->stream, n, (const uint32_t*)c->oid, newn,
+                     (const uint32_t*)c->tmpo, c->slot);
+  c->n = newn;
+  launch_reorder(c);
+  c->lists_valid = false;
+  return check_hip(hipStreamSynchronize(c->stream), "pbf_remove");
+}
+
+// ---- getters / setters in original order ----
+static int get_vec(pbf_context* c, const float4* src, float* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  if (!c->n) return PBF_OK;
+  launch_gather_out_f4(c, src, c->stage, 3);
+  int rc = check_hip(hipMemcpyAsync(out, c->stage, c->n * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                     "download");
+  if (rc) return rc;
+  return check_hip(hipStreamSynchronize(c->stream), "download");
+}
+static int get_scalar(pbf_context* c, const float* src, float* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  if (!c->n) return PBF_OK;
+  launch_gather_out_f1(c, src, c->stage);
+  int rc = check_hip(hipMemcpyAsync(out, c->stage, c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                     "download");
+  if (rc) return rc;
+  return check_hip(hipStreamSynchronize(c->stream), "download");
+}
+static int set_vec(pbf_context* c, float4* dst, const float* in) {
+  if (!c || !in) return set_error(PBF_ERR_ARG, "null argument");
+  if (!c->n) return PBF_OK;
+  int rc = check_hip(hipMemcpyAsync(c->stage, in, c->n * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream),
+                     "upload");
+  if (rc) return rc;
+  launch_scatter_in_f4(c, c->stage, dst);
+  return check_hip(hipStreamSynchronize(c->stream), "upload");
+}
+static int set_scalar(pbf_context* c, float* dst, const float* in) {
+  if (!c || !in) return set_error(PBF_ERR_ARG, "null argument");
+  if (!c->n) return PBF_OK;
+  int rc = check_hip(hipMemcpyAsync(c->stage, in, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream), "upload");
+  if (rc) return rc;
+  launch_scatter_in_f1(c, c->stage, dst);
+  return check_hip(hipStreamSynchronize(c->stream), "upload");
+}
+
+int pbf_get_positions(pbf_context* c, float* out) { return get_vec(c, c ? c->x : nullptr, out); }
+int pbf_get_position_stars(pbf_context* c, float* out) { return get_vec(c, c ? c->xs : nullptr, out); }
+int pbf_get_velocities(pbf_context* c, float* out) { return get_vec(c, c ? c->v : nullptr, out); }
+int pbf_get_scratch(pbf_context* c, float* out) { return get_vec(c, c ? c->scr : nullptr, out); }
+int pbf_get_densities(pbf_context* c, float* out) { return get_scalar(c, c ? c->rho : nullptr, out); }
+int pbf_get_lambdas(pbf_context* c, float* out) { return get_scalar(c, c ? c->lam : nullptr, out); }
+int pbf_set_positions(pbf_context* c, const float* in) { return set_vec(c, c ? c->x : nullptr, in); }
+int pbf_set_position_stars(pbf_context* c, const float* in) { return set_vec(c, c ? c->xs : nullptr, in); }
+int pbf_set_velocities(pbf_context* c, const float* in) { return set_vec(c, c ? c->v : nullptr, in); }
+int pbf_set_scratch(pbf_context* c, const float* in) { return set_vec(c, c ? c->scr : nullptr, in); }
+int pbf_set_densities(pbf_context* c, const float* in) { return set_scalar(c, c ? c->rho : nullptr, in); }
+int pbf_set_lambdas(pbf_context* c, const float* in) { return set_scalar(c, c ? c->lam : nullptr, in); }
+
+// ---- passes ----
+#define PBF_CHECK_CTX_SPAN(c, s)                                            \
+  do {                                                                      \
+    if (!(c)) return set_error(PBF_ERR_ARG, "null context");                \
+    if (!span_ok((c), (s))) return set_error(PBF_ERR_ARG, "span out of range"); \
+  } while (0)
+
+int pbf_apply_external_forces(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  launch_forces_predict(c, F_FORCES, s.begin, s.end);
+  return check_hip(hipGetLastError(), "apply_external_forces");
+}
+
+int pbf_predict_positions(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  launch_forces_predict(c, F_PREDICT, s.begin, s.end);
+  return check_hip(hipGetLastError(), "predict_positions");
+}
+
+int pbf_find_neighbors(pbf_context* c, pbf_index_span bin_span, pbf_index_span fill_span) {
+  PBF_CHECK_CTX_SPAN(c, bin_span);
+  PBF_CHECK_CTX_SPAN(c, fill_span);
+  if (bin_span.begin != 0)
+    return set_error(PBF_ERR_ARG, "bin span must start at 0: sort_bins sorts [0, count) (neighbors.h:130-132)");
+  launch_fill_u64(c->counters, 0ull, 3, c->stream);  // overflow, outside, hacks
+  find_neighbors_impl(c, bin_span.end, fill_span.begin, fill_span.end);
+  return check_hip(hipGetLastError(), "find_neighbors");
+}
+
+int pbf_compute_densities(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  if (int rc = need_lists(c)) return rc;
+  launch_density_lambda(c, 0, s.begin, s.end);
+  std::swap(c->xs, c->xt);  // the pass's x* (with the near-coincidence fix) becomes current
+  return check_hip(hipGetLastError(), "compute_densities");
+}
+
+int pbf_compute_pressure_lambdas(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  if (int rc = need_lists(c)) return rc;
+  launch_lambda(c, c->xs, 0, s.begin, s.end);
+  return check_hip(hipGetLastError(), "compute_pressure_lambdas");
+}
+
+int pbf_compute_pressure_dps(pbf_context* c, pbf_index_span s, int substep) {
+  (void)substep;  // unused by the reference as well (particles.h:348)
+  PBF_CHECK_CTX_SPAN(c, s);
+  if (int rc = need_lists(c)) return rc;
+  launch_dp(c, s.begin, s.end);
+  return check_hip(hipGetLastError(), "compute_pressure_dps");
+}
+
+int pbf_update_position_stars(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  launch_update_position_stars(c, s.begin, s.end);
+  return check_hip(hipGetLastError(), "update_position_stars");
+}
+
+int pbf_update_velocities(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  launch_update_velocities(c, s.begin, s.end);
+  return check_hip(hipGetLastError(), "update_velocities");
+}
+
+int pbf_apply_surface_tension(pbf_context* c, pbf_index_span cs, pbf_index_span ts) {
+  PBF_CHECK_CTX_SPAN(c, cs);
+  PBF_CHECK_CTX_SPAN(c, ts);
+  if (int rc = need_lists(c)) return rc;
+  launch_surface_tension(c, cs.begin, cs.end, ts.begin, ts.end);
+  return check_hip(hipGetLastError(), "apply_surface_tension");
+}
+
+int pbf_apply_viscosity(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  if (int rc = need_lists(c)) return rc;
+  launch_viscosity(c, c->v, c->va, 0, s.begin, s.end);
+  std::swap(c->v, c->va);
+  return check_hip(hipGetLastError(), "apply_viscosity");
+}
+
+int pbf_compute_vorticity(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  if (int rc = need_lists(c)) return rc;
+  launch_vorticity(c, c->v, s.begin, s.end);
+  return check_hip(hipGetLastError(), "compute_vorticity");
+}
+
+int pbf_apply_vorticity(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  if (int rc = need_lists(c)) return rc;
+  launch_apply_vorticity(c, c->v, s.begin, s.end);
+  return check_hip(hipGetLastError(), "apply_vorticity");
+}
+
+int pbf_update_positions(pbf_context* c, pbf_index_span s) {
+  PBF_CHECK_CTX_SPAN(c, s);
+  launch_update_positions(c, s.begin, s.end);
+  return check_hip(hipGetLastError(), "update_positions");
+}
+
+// Simulation/Source/main.cpp:71-124 on one domain, fused kernels.
+int pbf_step(pbf_context* c, uint32_t nsteps) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  const uint64_t n = c->n;
+  for (uint32_t st = 0; st < nsteps; ++st) {
+    launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
+    launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
+    find_neighbors_impl(c, n, 0, n);
+    const uint32_t I = c->P.solve_step_count;
+    for (uint32_t it = 0; it < I; ++it) {
+      launch_density_lambda(c, 1, 0, n);      // xs -> xt, rho, lam
+      launch_lambda(c, c->xt, 1, 0, n);       // only if the density fix fired
+      launch_dp_update(c, it + 1 == I, 0, n); // xt, lam -> xs (+ v)
+    }
+    if (I == 0) launch_update_velocities(c, 0, n);
+    c->last_iterations = I;
+    if (c->P.gamma != 0.f) launch_surface_tension(c, 0, n, 0, n);
+    launch_viscosity(c, c->v, c->va, 0, 0, n);  // XSPH 1: v -> va
+    launch_vorticity(c, c->va, 0, n);           // omega -> scr
+    launch_apply_vorticity(c, c->va, 0, n);     // va += ...
+    launch_viscosity(c, c->va, c->v, 1, 0, n);  // XSPH 2: va -> v, x = x*
+  }
+  return check_hip(hipGetLastError(), "pbf_step");
+}
+
+int pbf_sync(pbf_context* c) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  return check_hip(hipStreamSynchronize(c->stream), "pbf_sync");
+}
+
+int pbf_get_stats(pbf_context* c, pbf_stats* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  unsigned long long h[C_NUM];
+  int rc = check_hip(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream), "stats");
+  if (rc) return rc;
+  std::vector<float> rho(c->n);
+  if (c->n && (rc = check_hip(hipMemcpyAsync(rho.data(), c->rho, c->n * 4, hipMemcpyDeviceToHost, c->stream),
+                              "stats")))
+    return rc;
+  if ((rc = check_hip(hipStreamSynchronize(c->stream), "stats"))) return rc;
+  out->neighbor_overflow = h[C_OVERFLOW];
+  out->outside_grid = h[C_OUTSIDE];
+  out->density_hacks = h[C_HACKS];
+  out->iterations = c->last_iterations;
+  double m = 0.0;
+  for (float r : rho) {
+    const float e = r / c->P.rest_density - 1.0f;
+    if ((double)e > m) m = (double)e;
+  }
+  out->max_density_error = m;
+  return PBF_OK;
+}
+
+// ---- introspection ----
+int pbf_grid_dims(const pbf_context* c, uint64_t dims[3]) {
+  if (!c || !dims) return set_error(PBF_ERR_ARG, "null argument");
+  for (int i = 0; i < 3; ++i) dims[i] = c->dims[i];
+  return PBF_OK;
+}
+uint64_t pbf_num_cells(const pbf_context* c) { return c ? c->ncells : 0; }
+
+static int download_u32(pbf_context* c, const uint32_t* src, uint64_t count, std::vector<uint32_t>& out) {
+  out.resize(count);
+  if (!count) return PBF_OK;
+  int rc = check_hip(hipMemcpyAsync(out.data(), src, count * 4, hipMemcpyDeviceToHost, c->stream), "download");
+  if (rc) return rc;
+  return check_hip(hipStreamSynchronize(c->stream), "download");
+}
+
+int pbf_get_bin_ids(pbf_context* c, uint64_t* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  if (int rc = need_lists(c)) return rc;
+  std::vector<uint32_t> k;
+  if (int rc = download_u32(c, c->key, c->n, k)) return rc;
+  for (uint64_t i = 0; i < c->n; ++i) out[i] = k[i];
+  return PBF_OK;
+}
+
+int pbf_get_particle_ids(pbf_context* c, uint64_t* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  if (int rc = need_lists(c)) return rc;
+  std::vector<uint32_t> o;
+  if (int rc = download_u32(c, c->oid, c->n, o)) return rc;
+  for (uint64_t i = 0; i < c->n; ++i) out[i] = o[i];
+  return PBF_OK;
+}
+
+int pbf_get_bin_bounds(pbf_context* c, uint64_t* begin, uint64_t* end) {
+  if (!c || !begin || !end) return set_error(PBF_ERR_ARG, "null argument");
+  if (int rc = need_lists(c)) return rc;
+  std::vector<uint32_t> st;
+  if (int rc = download_u32(c, c->cell_start, c->ncells + 1, st)) return rc;
+  for (uint64_t i = 0; i < c->ncells; ++i) {
+    begin[i] = st[i];
+    end[i] = st[i + 1];
+  }
+  return PBF_OK;
+}
+
+int pbf_get_neighbor_lists(pbf_context* c, uint64_t* out, int32_t* counts) {
+  if (!c || !out || !counts) return set_error(PBF_ERR_ARG, "null argument");
+  if (int rc = need_lists(c)) return rc;
+  const uint64_t n = c->n, K = (uint64_t)c->cap;
+  std::vector<uint32_t> o, cnt, lists;
+  if (int rc = download_u32(c, c->oid, n, o)) return rc;
+  if (int rc = download_u32(c, (const uint32_t*)c->nbr_count, n, cnt)) return rc;
+  if (int rc = download_u32(c, (const uint32_t*)c->nbr, n * K, lists)) return rc;
+  std::memset(out, 0, n * K * sizeof(uint64_t));
+  for (uint64_t p = 0; p < n; ++p) {
+    const uint64_t op = o[p];
+    const int32_t cp = (int32_t)cnt[p];
+    counts[op] = cp;
+    for (int32_t k = 0; k < cp; ++k) out[op * K + k] = o[lists[(uint64_t)k * n + p]];
+  }
+  return PBF_OK;
+}
+
+// ---- primitives ----
+int pbf_sort_by_key_u32(uint32_t* d_keys, uint32_t* d_values, uint64_t n, uint32_t key_bits) {
+  if (n && (!d_keys || !d_values)) return set_error(PBF_ERR_ARG, "null argument");
+  int rc = radix_sort_u32(d_keys, d_values, n, key_bits, 0);
+  if (rc) return rc;
+  return check_hip(hipStreamSynchronize(0), "sort_by_key");
+}
+
+int pbf_lower_bound_counting(const uint32_t* d_sorted, uint64_t n, pbf_index_span search, uint32_t* d_out) {
+  if (search.end < search.begin || (search.end > search.begin && (!d_out || (n && !d_sorted))))
+    return set_error(PBF_ERR_ARG, "bad argument");
+  launch_bounds_counting(d_sorted, n, search.begin, search.end, d_out, 0, 0);
+  return check_hip(hipStreamSynchronize(0), "lower_bound");
+}
+
+int pbf_upper_bound_counting(const uint32_t* d_sorted, uint64_t n, pbf_index_span search, uint32_t* d_out) {
+  if (search.end < search.begin || (search.end > search.begin && (!d_out || (n && !d_sorted))))
+    return set_error(PBF_ERR_ARG, "bad argument");
+  launch_bounds_counting(d_sorted, n, search.begin, search.end, d_out, 1, 0);
+  return check_hip(hipStreamSynchronize(0), "upper_bound");
+}
+
+int pbf_device_alloc(uint64_t bytes, void** out) {
+  if (!out) return set_error(PBF_ERR_ARG, "null argument");
+  return check_hip(hipMalloc(out, bytes ? bytes : 1), "hipMalloc");
+}
+int pbf_device_free(void* p) { return p ? check_hip(hipFree(p), "hipFree") : PBF_OK; }
+int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes) {
+  return check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+}
+int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes) {
+  return check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+}
+
+// ---- measurement ----
+int pbf_profile_kernels(pbf_context* c, const char* names) {
+  if (!c || !names) return set_error(PBF_ERR_ARG, "null argument");
+  hipStreamSynchronize(c->stream);
+  timers_resolve(c);
+  c->timers.clear();
+  std::string all(names);
+  size_t pos = 0;
+  while (pos <= all.size()) {
+    size_t comma = all.find(',', pos);
+    if (comma == std::string::npos) comma = all.size();
+    const std::string nm = all.substr(pos, comma - pos);
+    if (!nm.empty()) {
+      KernelTimer t;
+      t.name = nm;
+      c->timers.push_back(t);
+    }
+    pos = comma + 1;
+  }
+  return PBF_OK;
+}
+
+int pbf_kernel_time(pbf_context* c, const char* name, uint64_t* launches, double* total_ms) {
+  if (!c || !name || !launches || !total_ms) return set_error(PBF_ERR_ARG, "null argument");
+  timers_resolve(c);
+  for (auto& t : c->timers) {
+    if (t.name == name) {
+      *launches = t.launches;
+      *total_ms = t.total_ms;
+      return PBF_OK;
+    }
+  }
+  return set_error(PBF_ERR_ARG, std::string("kernel not profiled: ") + name);
+}
+
+int pbf_region_begin(pbf_context* c) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  return check_hip(hipEventRecord(c->region_a, c->stream), "hipEventRecord");
+}
+
+int pbf_region_end(pbf_context* c, double* ms) {
+  if (!c || !ms) return set_error(PBF_ERR_ARG, "null argument");
+  int rc = check_hip(hipEventRecord(c->region_b, c->stream), "hipEventRecord");
+  if (rc) return rc;
+  if ((rc = check_hip(hipEventSynchronize(c->region_b), "hipEventSynchronize"))) return rc;
+  float f = 0.f;
+  rc = check_hip(hipEventElapsedTime(&f, c->region_a, c->region_b), "hipEventElapsedTime");
+  *ms = f;
+  return rc;
+}
+
+}  // extern "C"
+
+namespace pbf {
+
+static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe) {
+  const uint64_t C = c->ncells;
+  launch_fill_u32(c->cell_count, 0u, C + 1, c->stream);
+  launch_bin(c, bin_end);
+  {
+    KernelTimer* t = timer_begin(c, "scan");
+    launch_exclusive_scan(c->cell_count, c->cell_start, C + 1, c->scan_partials, c->stream);
+    timer_end(c, t);
+  }
+  launch_scatter_rank(c);
+  launch_reorder(c);
+  launch_fill_neighbors(c, fb, fe);
+  c->lists_valid = true;
+}
+
+}  // namespace pbf

@@ -1,0 +1,120 @@
+// pbf_device.h -- device arithmetic of the PBF step.
+//
+// Every function keeps the operation order and rounding of the reference's
+// expressions (Common/vec.h, Simulation/Source/kernels.h, particles.h) so
+// that, compiled with -ffp-contract=off and HIP's default correctly rounded
+// single-precision divide and square root, results are bit-identical to the
+// same expressions on the CPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pbf_internal.h"
+
+namespace pbf {
+
+struct f3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ f3 ld3(const float4* a, uint32_t i) {
+  const float4 t = a[i];
+  return {t.x, t.y, t.z};
+}
+__device__ __forceinline__ void st3(float4* a, uint32_t i, f3 v) { a[i] = make_float4(v.x, v.y, v.z, 0.f); }
+
+// vec.h operator-, operator+, vec*scalar, scalar*vec, vec/scalar, +=, -=
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 mul3(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 smul3(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 div3(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ void acc3(f3& a, f3 b) {
+  a.x += b.x;
+  a.y += b.y;
+  a.z += b.z;
+}
+__device__ __forceinline__ void dec3(f3& a, f3 b) {
+  a.x -= b.x;
+  a.y -= b.y;
+  a.z -= b.z;
+}
+// vec.h:500-508 dot from 0: for a square the leading 0 + x*x is exact, so
+// magnitude_squared is (x*x + y*y) + z*z.
+__device__ __forceinline__ float magsq3(f3 a) { return (a.x * a.x + a.y * a.y) + a.z * a.z; }
+// vec.h:620-630
+__device__ __forceinline__ f3 cross3(f3 l, f3 r) {
+  return {l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x};
+}
+
+// "r is within h", i.e. !(sqrtf(d2) > h), decided on d2 alone: sqrt with
+// round-to-nearest is monotone, so sqrtf(d2) > h exactly when d2 >= T_h.
+// NaN compares false and is kept, like the reference's `r_mag > h` test.
+__device__ __forceinline__ bool within_h(float d2, const KParams& K) { return !(d2 >= K.T_h); }
+
+// Poly6<float,3>::operator() for r <= h (kernels.h:69)
+__device__ __forceinline__ float poly6_in(float r, const KParams& K) {
+  const float a = K.hh - r * r;
+  return K.poly6_norm * a * a * a;
+}
+__device__ __forceinline__ float poly6(float r, const KParams& K) {
+  if (r > K.h) return 0.0f;
+  return poly6_in(r, K);
+}
+// Del_Spikey<float,3>::operator() for r <= h, with r = |d| (kernels.h:171)
+__device__ __forceinline__ f3 spiky_in(f3 d, float r, const KParams& K) {
+  const float s = K.spiky_norm * (K.h - r) * (K.h - r);
+  return div3(smul3(s, d), r + K.r_eps);
+}
+// C_Spline<float,3>::operator() (kernels.h:220-228); the first branch is in double
+__device__ __forceinline__ float cspline(float r, const KParams& K) {
+  const float h = K.h;
+  if (r > h) return 0.0f;
+  if (r <= h * 0.5f) {
+    const double t = 2.0 * (double)(h - r) * (double)(h - r) * (double)(h - r) * (double)r * (double)r * (double)r;
+    const float h6 = h * h * h * h * h * h / 64.0f;
+    return (float)((double)K.cspline_norm * (t - (double)h6));
+  }
+  return K.cspline_norm * (h - r) * (h - r) * (h - r) * r * r * r;
+}
+
+// apply_boundary_conditions (particles.h:510-526)
+__device__ __forceinline__ void boundary(f3& p, const KParams& K) {
+  const float mover_radius = 0.2f;
+  const f3 mc = {K.mover[0], K.mover[1], K.mover[2]};
+  const float dr_squared = magsq3(sub3(p, mc));
+  if (dr_squared < mover_radius * mover_radius) {
+    const float dr = sqrtf(dr_squared);
+    acc3(p, div3(smul3(mover_radius - dr, sub3(p, mc)), dr));
+  }
+  if (p.x < K.bmin[0]) p.x = K.bmin[0]; else if (p.x > K.bmax[0]) p.x = K.bmax[0];
+  if (p.y < K.bmin[1]) p.y = K.bmin[1]; else if (p.y > K.bmax[1]) p.y = K.bmax[1];
+  if (p.z < K.bmin[2]) p.z = K.bmin[2]; else if (p.z > K.bmax[2]) p.z = K.bmax[2];
+}
+
+// calculate_bin_id one axis (neighbors.h:110-111): floor((p + s) / s)
+__device__ __forceinline__ float bin_coord(float p, float s) { return floorf((p + s) / s); }
+
+// Key of a cell given per-axis coordinates; anything that is not a cell of
+// the grid (negative, NaN, past the last cell) maps to ncells ("outside").
+__device__ __forceinline__ uint32_t cell_key(float fx, float fy, float fz, const KParams& K) {
+  if (!(fx >= 0.f) || !(fy >= 0.f) || !(fz >= 0.f)) return K.ncells;
+  if (fx > 4.0e9f || fy > 4.0e9f || fz > 4.0e9f) return K.ncells;
+  const uint64_t bx = (uint64_t)fx, by = (uint64_t)fy, bz = (uint64_t)fz;
+  const uint64_t k = K.dxy * bz + (by * (uint64_t)K.dimx + bx);
+  return k < (uint64_t)K.ncells ? (uint32_t)k : K.ncells;
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD works on
+// one contiguous range of particles (neighbouring cells share its L2).
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t per = nb / 8u, rem = nb % 8u;
+  const uint32_t xcd = b % 8u, idx = b / 8u;
+  // XCDs 0..rem-1 get per+1 blocks, the rest get per
+  const uint32_t base = xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per;
+  return base + idx;
+}
+
+}  // namespace pbf

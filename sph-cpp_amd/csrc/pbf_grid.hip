@@ -1,0 +1,441 @@
+// pbf_grid.hip -- cell binning, stable counting sort, cell bounds, reorder,
+// neighbour lists and the sort / bound primitives.
+//
+// Reference: Simulation/Source/neighbors.h:108-248 (calculate_bin_id,
+// calculate_bins, sort_bins, find_bin_bounds, calculate_neighbor_indices,
+// fill_neighbors) and Common/sim_algorithms.h:57-99 (sort_by_key,
+// lower_bound, upper_bound).
+//
+// Sort.  The reference's sort_by_key is Thrust's stable sort of (cell key,
+// particle index).  Particles here live in cell order from the previous
+// step, so a stable sort of the current slots would order by
+// (key, previous key, index).  Instead: one counting pass (radix = number of
+// cells) gives every cell its range, and each particle's place inside its
+// cell is its rank by original index among the cell's members.  The result
+// is exactly the (key, original index) order the reference produces.
+#include "pbf_device.h"
+
+namespace pbf {
+
+constexpr int kBlock = 256;
+
+static inline uint32_t nblocks(uint64_t n, int per = kBlock) { return (uint32_t)((n + per - 1) / per); }
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t value, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long value, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(n)), dim3(kBlock), 0, s, p, value, n);
+}
+void launch_fill_u64(unsigned long long* p, unsigned long long value, uint64_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_u64, dim3(nblocks(n)), dim3(kBlock), 0, s, p, value, n);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (three passes: tile sums, scan of tile sums, tile scans).
+// A tile is 256 threads x 8 items.  64-wide wavefront scans with __shfl_up.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// block-wide exclusive scan of one value per thread; returns the block total in *total
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / 64; ++i) {
+    const uint32_t s = wsum[i];
+    if (i < w) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(const uint32_t* in, uint64_t n, uint32_t* partials) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) s += in[base + i];
+  uint32_t tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// one block scans all tile sums in place (exclusive), looping with a carry
+__global__ void __launch_bounds__(kBlock) k_scan_partials(uint32_t* partials, uint32_t np) {
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < np; base += kBlock) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < np ? partials[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, &tot);
+    if (i < np) partials[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                       const uint32_t* partials) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t vals[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    vals[i] = (base + i < n) ? in[base + i] : 0u;
+    s += vals[i];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan(s, &tot) + partials[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += vals[i];
+    if (base + i + 1 == n) out[n] = run;  // total
+  }
+}
+
+uint64_t scan_partials_size(uint64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partials, hipStream_t s) {
+  if (n == 0) {
+    launch_fill_u32(out, 0u, 1, s);
+    return;
+  }
+  const uint32_t nt = (uint32_t)((n + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, s, in, n, partials);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, partials, nt);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, s, in, out, n, (const uint32_t*)partials);
+}
+
+// ---------------------------------------------------------------------------
+// calculate_bins (neighbors.h:120-125): key of every slot, and its arrival
+// slot inside the cell's bucket.
+__global__ void __launch_bounds__(kBlock) k_bin(const float4* __restrict__ xs, uint32_t n, uint32_t bin_end,
+                                                KParams K, const uint32_t* __restrict__ oid,
+                                                uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                                uint32_t* __restrict__ cell_count,
+                                                unsigned long long* __restrict__ counters) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t k = K.ncells;  // particles past the bin span are placed in no cell
+  if (oid[p] < bin_end) {
+    const f3 ps = ld3(xs, p);
+    k = cell_key(bin_coord(ps.x, K.s), bin_coord(ps.y, K.s), bin_coord(ps.z, K.s), K);
+    if (k == K.ncells) atomicAdd(&counters[C_OUTSIDE], 1ull);
+  }
+  key[p] = k;
+  slot[p] = atomicAdd(&cell_count[k], 1u);
+}
+
+void launch_bin(pbf_context* c, uint64_t bin_end) {
+  const uint32_t n = (uint32_t)c->n;
+  if (!n) return;
+  KernelTimer* t = timer_begin(c, "bin");
+  hipLaunchKernelGGL(k_bin, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, (const float4*)c->xs, n,
+                     (uint32_t)bin_end, c->K, (const uint32_t*)c->oid, c->key, c->slot, c->cell_count, c->counters);
+  timer_end(c, t);
+}
+
+// bucket placement in arrival order: tmp[start + slot] = slot index, tmpo = its original index
+__global__ void __launch_bounds__(kBlock) k_scatter(uint32_t n, const uint32_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ slot,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ oid, uint32_t* __restrict__ tmp,
+                                                    uint32_t* __restrict__ tmpo) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t pos = start[key[p]] + slot[p];
+  tmp[pos] = p;
+  tmpo[pos] = oid[p];
+}
+
+// final place inside the bucket = rank by original index (stable order)
+__global__ void __launch_bounds__(kBlock) k_rank(uint32_t n, const uint32_t* __restrict__ key,
+                                                 const uint32_t* __restrict__ start,
+                                                 const uint32_t* __restrict__ tmp,
+                                                 const uint32_t* __restrict__ tmpo, uint32_t* __restrict__ perm) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t p = tmp[s];
+  const uint32_t me = tmpo[s];
+  const uint32_t k = key[p];
+  const uint32_t b = start[k], e = start[k + 1];
+  uint32_t r = 0;
+  for (uint32_t t = b; t < e; ++t) r += (tmpo[t] < me) ? 1u : 0u;
+  perm[b + r] = p;
+}
+
+void launch_scatter_rank(pbf_context* c) {
+  const uint32_t n = (uint32_t)c->n;
+  if (!n) return;
+  KernelTimer* t = timer_begin(c, "sort");
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->key,
+                     (const uint32_t*)c->slot, (const uint32_t*)c->cell_start, (const uint32_t*)c->oid, c->tmp,
+                     c->tmpo);
+  // perm goes to `slot` (its arrival slots are no longer needed)
+  hipLaunchKernelGGL(k_rank, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->key,
+                     (const uint32_t*)c->cell_start, (const uint32_t*)c->tmp, (const uint32_t*)c->tmpo, c->slot);
+  timer_end(c, t);
+}
+
+// gather the persistent state into cell order
+__global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* __restrict__ perm,
+                                                    const float4* __restrict__ x, const float4* __restrict__ xs,
+                                                    const float4* __restrict__ v, const float* __restrict__ rho,
+                                                    const float* __restrict__ lam, const float4* __restrict__ scr,
+                                                    const uint32_t* __restrict__ oid,
+                                                    const uint32_t* __restrict__ key, float4* __restrict__ xo,
+                                                    float4* __restrict__ xso, float4* __restrict__ vo,
+                                                    float* __restrict__ rhoo, float* __restrict__ lamo,
+                                                    float4* __restrict__ scro, uint32_t* __restrict__ oido,
+                                                    uint32_t* __restrict__ keyo) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t p = perm[s];
+  xo[s] = x[p];
+  xso[s] = xs[p];
+  vo[s] = v[p];
+  rhoo[s] = rho[p];
+  lamo[s] = lam[p];
+  scro[s] = scr[p];
+  oido[s] = oid[p];
+  keyo[s] = key[p];
+}
+
+void launch_reorder(pbf_context* c) {
+  const uint32_t n = (uint32_t)c->n;
+  if (!n) return;
+  KernelTimer* t = timer_begin(c, "reorder");
+  hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->slot,
+                     (const float4*)c->x, (const float4*)c->xs, (const float4*)c->v, (const float*)c->rho,
+                     (const float*)c->lam, (const float4*)c->scr, (const uint32_t*)c->oid,
+                     (const uint32_t*)c->key, c->xa, c->xsa, c->va, c->rhoa, c->lama, c->scra, c->oida, c->keya);
+  timer_end(c, t);
+  std::swap(c->x, c->xa);
+  std::swap(c->xs, c->xsa);
+  std::swap(c->v, c->va);
+  std::swap(c->rho, c->rhoa);
+  std::swap(c->lam, c->lama);
+  std::swap(c->scr, c->scra);
+  std::swap(c->oid, c->oida);
+  std::swap(c->key, c->keya);
+}
+
+// ---------------------------------------------------------------------------
+// fill_neighbors (neighbors.h:197-230).  One lane per particle walks its 27
+// cells in the reference's order (x offset outer, z inner; cell ids from the
+// FP coordinates x + i*s, neighbors.h:171-185), skips itself and accepts
+// d^2 < s^2 while the list has room.  Lists hold sorted slots, column major.
+__global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t fb, uint32_t fe,
+                                                           const float4* __restrict__ xs,
+                                                           const uint32_t* __restrict__ oid,
+                                                           const uint32_t* __restrict__ start, KParams K,
+                                                           int32_t* __restrict__ nbr,
+                                                           int32_t* __restrict__ nbr_count,
+                                                           unsigned long long* __restrict__ counters) {
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t o = oid[p];
+  if (o < fb || o >= fe) {
+    nbr_count[p] = 0;
+    return;
+  }
+  const f3 ps = ld3(xs, p);
+  const float s = K.s;
+  float cx[3], cy[3], cz[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float off = (float)(i - 1) * s;
+    cx[i] = bin_coord(ps.x + off, s);
+    cy[i] = bin_coord(ps.y + off, s);
+    cz[i] = bin_coord(ps.z + off, s);
+  }
+  int count = 0;
+  uint32_t ovf = 0;
+  const int cap = K.cap;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t cell = cell_key(cx[i], cy[j], cz[k], K);
+        if (cell >= K.ncells) continue;
+        const uint32_t b = start[cell], e = start[cell + 1];
+        for (uint32_t q = b; q < e; ++q) {
+          if (q == p) continue;
+          const f3 xq = ld3(xs, q);
+          const float d2 = magsq3(sub3(ps, xq));
+          if (d2 < K.ss) {
+            if (count < cap) {
+              nbr[(uint64_t)count * n + p] = (int32_t)q;
+              ++count;
+            } else {
+              ++ovf;
+            }
+          }
+        }
+      }
+  nbr_count[p] = count;
+  if (ovf) atomicAdd(&counters[C_OVERFLOW], (unsigned long long)ovf);
+}
+
+void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
+  const uint32_t n = (uint32_t)c->n;
+  if (!n) return;
+  KernelTimer* t = timer_begin(c, "fill_neighbors");
+  hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)fb,
+                     (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
+                     c->K, c->nbr, c->nbr_count, c->counters);
+  timer_end(c, t);
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (u32 key, u32 value) for the sort_by_key primitive.
+// Digit = 8 bits.  Per pass: tile histograms (digit-major), scan, stable
+// scatter.  Inside a tile, each element's rank among equal digits keeps
+// input order: waves rank with ballots, waves are combined through LDS.
+constexpr int kRadixItems = 4;                    // items per thread
+constexpr int kRadixTile = kBlock * kRadixItems;  // 1024 keys per tile
+
+__global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restrict__ keys, uint64_t n, int shift,
+                                                       uint32_t ntiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kRadixTile;
+  for (int i = 0; i < kRadixItems; ++i) {
+    const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Scatter one tile.  Elements are processed in rounds of 256 (one per
+// thread, in input order); within a round, rank = number of earlier lanes of
+// the round with the same digit, found with 8 ballots per wave.
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ vals, uint64_t n, int shift,
+                                                          uint32_t ntiles, const uint32_t* __restrict__ offs,
+                                                          uint32_t* __restrict__ okeys, uint32_t* __restrict__ ovals) {
+  __shared__ uint32_t base_d[256];       // running output offset per digit for this tile
+  __shared__ uint32_t wave_cnt[4][256];  // per wave, count of each digit in this round
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  base_d[threadIdx.x] = offs[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+  const uint64_t tbase = (uint64_t)blockIdx.x * kRadixTile;
+  for (int r = 0; r < kRadixItems; ++r) {
+    for (int i = 0; i < 4; ++i) wave_cnt[i][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t idx = tbase + (uint64_t)r * kBlock + threadIdx.x;
+    const bool valid = idx < n;
+    const uint32_t k = valid ? keys[idx] : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+    // lanes of this wave with the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bal = __ballot(((d >> b) & 1u) != 0u);
+      peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t rank_in_wave = (uint32_t)__popcll(peers & lower);
+    const uint32_t wave_total = (uint32_t)__popcll(peers);
+    if (valid && rank_in_wave == 0) wave_cnt[w][d] = wave_total;
+    __syncthreads();
+    if (valid) {
+      uint32_t before = 0;
+      for (int i = 0; i < w; ++i) before += wave_cnt[i][d];
+      const uint32_t pos = base_d[d] + before + rank_in_wave;
+      okeys[pos] = k;
+      ovals[pos] = vals[idx];
+    }
+    __syncthreads();
+    // advance the digit offsets by this round's counts
+    {
+      const uint32_t dd = threadIdx.x;
+      base_d[dd] += wave_cnt[0][dd] + wave_cnt[1][dd] + wave_cnt[2][dd] + wave_cnt[3][dd];
+    }
+    __syncthreads();
+  }
+}
+
+int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits, hipStream_t s) {
+  if (n <= 1) return PBF_OK;
+  if (n > 0xFFFFFFFFull) return set_error(PBF_ERR_UNSUPPORTED, "sort_by_key: n must be < 2^32");
+  if (key_bits == 0 || key_bits > 32) key_bits = 32;
+  const uint32_t ntiles = (uint32_t)((n + kRadixTile - 1) / kRadixTile);
+  const uint64_t hn = 256ull * ntiles;
+  uint32_t *k2 = nullptr, *v2 = nullptr, *hist = nullptr, *offs = nullptr, *part = nullptr;
+  if (check_hip(hipMallocAsync((void**)&k2, n * 4, s), "hipMallocAsync") ||
+      check_hip(hipMallocAsync((void**)&v2, n * 4, s), "hipMallocAsync") ||
+      check_hip(hipMallocAsync((void**)&hist, hn * 4, s), "hipMallocAsync") ||
+      check_hip(hipMallocAsync((void**)&offs, (hn + 1) * 4, s), "hipMallocAsync") ||
+      check_hip(hipMallocAsync((void**)&part, scan_partials_size(hn) * 4, s), "hipMallocAsync"))
+    return PBF_ERR_HIP;
+  uint32_t *ki = keys, *vi = vals, *ko = k2, *vo = v2;
+  int passes = 0;
+  for (uint32_t shift = 0; shift < key_bits; shift += 8, ++passes) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t*)ki, n, (int)shift, ntiles,
+                       hist);
+    launch_exclusive_scan(hist, offs, hn, part, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t*)ki,
+                       (const uint32_t*)vi, n, (int)shift, ntiles, (const uint32_t*)offs, ko, vo);
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+  }
+  if (passes & 1) {
+    hipMemcpyAsync(keys, ki, n * 4, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(vals, vi, n * 4, hipMemcpyDeviceToDevice, s);
+  }
+  hipFreeAsync(k2, s);
+  hipFreeAsync(v2, s);
+  hipFreeAsync(hist, s);
+  hipFreeAsync(offs, s);
+  hipFreeAsync(part, s);
+  return check_hip(hipGetLastError(), "radix sort launch");
+}
+
+// lower_bound / upper_bound of every value of a counting range
+__global__ void k_bounds(const uint32_t* __restrict__ sorted, uint64_t n, uint64_t b, uint64_t cnt,
+                         uint32_t* __restrict__ out, int upper) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint64_t val = b + i;
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    const uint64_t k = sorted[mid];
+    if (upper ? (k <= val) : (k < val)) lo = mid + 1;
+    else hi = mid;
+  }
+  out[i] = (uint32_t)lo;
+}
+
+void launch_bounds_counting(const uint32_t* sorted, uint64_t n, uint64_t b, uint64_t e, uint32_t* out, int upper,
+                            hipStream_t s) {
+  const uint64_t cnt = e - b;
+  if (!cnt) return;
+  hipLaunchKernelGGL(k_bounds, dim3(nblocks(cnt)), dim3(kBlock), 0, s, sorted, n, b, cnt, out, upper);
+}
+
+}  // namespace pbf

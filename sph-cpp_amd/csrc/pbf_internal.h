@@ -1,0 +1,146 @@
+// pbf_internal.h -- shared declarations of the HIP PBF library (not part of the ABI).
+//
+// Device data layout (one context, n particles, C cells, K = list cap):
+//   x, xs, v         float4[n]   position, position star, velocity (w unused),
+//                                stored in cell order (sorted by (cell key, original index))
+//   xa, xsa, va      float4[n]   reorder targets, swapped with the above
+//   xt               float4[n]   position stars after the density pass (hack applied)
+//   scr              float4[n]   scratch vector: dp, vorticity or colour field
+//   rho, lam         float[n]
+//   oid, oida        uint32[n]   original index of each sorted slot
+//   key, keya        uint32[n]   cell key of each slot
+//   slot, tmp, tmpo  uint32[n]   counting-sort work
+//   cell_count       uint32[C+1] particles per cell (+1 bucket for keys outside the grid)
+//   cell_start       uint32[C+2] exclusive scan of cell_count
+//   nbr              int32[K][n] neighbour lists, column major: nbr[k*n + p]
+//   nbr_count        int32[n]
+//   counters         u64[8]      overflow, outside, hacks, hack flag, e_max bits, ...
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pbf.h"
+
+namespace pbf {
+
+// Constants of one step, computed on the host exactly as the reference
+// computes them (kernels.h norms in double, parameters.h derivations).
+struct KParams {
+  float h, hh, s, ss;            // smoothing radius, h*h, bin spacing, s*s
+  float T_h;                     // smallest float d2 with sqrtf(d2) > h
+  float T_hack;                  // smallest float d2 with (double)sqrtf(d2) >= 1e-8
+  float poly6_norm, spiky_norm, cspline_norm, r_eps;
+  float mass, mass_W0;           // rest mass, rest_mass * W(0)
+  float rho0, neg_inv_rho0, inv_rho0;
+  float g, dt, g_dt;
+  float eps, visc, vort, gamma;
+  double vthr;                   // 0.000001 * max_speed (particles.h:380), in double
+  double st_rmin;                // h * 0.000001 (particles.h:420), in double
+  float bmin[3], bmax[3], mover[3];
+  uint32_t dimx, dimy, dimz;
+  uint32_t ncells;
+  uint64_t dxy;
+  int cap;
+};
+
+enum Counter {
+  C_OVERFLOW = 0,
+  C_OUTSIDE = 1,
+  C_HACKS = 2,
+  C_HACK_FLAG = 3,
+  C_EMAX = 4,  // float bits of the max density error (atomicMax on non-negative floats)
+  C_NUM = 8
+};
+
+struct KernelTimer {
+  std::string name;
+  uint64_t launches = 0;
+  double total_ms = 0.0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+}  // namespace pbf
+
+struct pbf_context {
+  pbf_params P;
+  pbf::KParams K;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  uint64_t n = 0, capacity = 0;
+  uint64_t dims[3] = {0, 0, 0};
+  uint64_t ncells = 0;
+  uint64_t grid_alloc = 0;  // cells allocated
+  int cap = 60;
+  bool lists_valid = false;
+  uint32_t last_iterations = 0;
+
+  float4 *x = nullptr, *xs = nullptr, *v = nullptr;
+  float4 *xa = nullptr, *xsa = nullptr, *va = nullptr;
+  float4 *xt = nullptr, *scr = nullptr;
+  float *rho = nullptr, *lam = nullptr, *rhoa = nullptr, *lama = nullptr;
+  float4* scra = nullptr;
+  uint32_t *oid = nullptr, *oida = nullptr, *key = nullptr, *keya = nullptr;
+  uint32_t *slot = nullptr, *tmp = nullptr, *tmpo = nullptr;
+  uint32_t *cell_count = nullptr, *cell_start = nullptr;
+  uint32_t* scan_partials = nullptr;
+  int32_t *nbr = nullptr, *nbr_count = nullptr;
+  unsigned long long* counters = nullptr;
+  float* stage = nullptr;  // device staging for original-order transfers (4 floats / particle)
+  uint64_t* stage64 = nullptr;
+
+  hipEvent_t region_a = nullptr, region_b = nullptr;
+  std::vector<pbf::KernelTimer> timers;
+};
+
+namespace pbf {
+
+// error plumbing (pbf_api.cpp)
+int set_error(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+// timing hooks around launches (pbf_api.cpp)
+KernelTimer* timer_begin(pbf_context* c, const char* name);
+void timer_end(pbf_context* c, KernelTimer* t);
+
+// ---- launchers (pbf_grid.hip) ----
+void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, unsigned long long value, uint64_t n, hipStream_t s);
+// exclusive scan of in[0..n) into out[0..n]; out[n] = total.  partials needs scan_partials_size(n).
+uint64_t scan_partials_size(uint64_t n);
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partials, hipStream_t s);
+void launch_bin(pbf_context* c, uint64_t bin_end);  // key, slot, cell_count (after fill)
+void launch_scatter_rank(pbf_context* c);   // tmp -> perm (written to slot), reorder
+void launch_reorder(pbf_context* c);
+void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe);
+int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits, hipStream_t s);
+void launch_bounds_counting(const uint32_t* sorted, uint64_t n, uint64_t b, uint64_t e, uint32_t* out,
+                            int upper, hipStream_t s);
+
+// ---- launchers (pbf_passes.hip) ----
+// flags for the particle-local kernel
+enum { F_FORCES = 1, F_PREDICT = 2 };
+void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e);
+void launch_density_lambda(pbf_context* c, int with_lambda, uint64_t b, uint64_t e);  // xs -> xt, rho, lam
+void launch_lambda(pbf_context* c, const float4* pos, int only_if_flag, uint64_t b, uint64_t e);
+void launch_dp(pbf_context* c, uint64_t b, uint64_t e);                     // xs, lam -> scr
+void launch_dp_update(pbf_context* c, int last, uint64_t b, uint64_t e);    // xt, lam -> xs (+ v)
+void launch_update_position_stars(pbf_context* c, uint64_t b, uint64_t e);
+void launch_update_velocities(pbf_context* c, uint64_t b, uint64_t e);
+void launch_update_positions(pbf_context* c, uint64_t b, uint64_t e);
+void launch_viscosity(pbf_context* c, const float4* vin, float4* vout, int write_positions, uint64_t b,
+                      uint64_t e);
+void launch_vorticity(pbf_context* c, const float4* vin, uint64_t b, uint64_t e);
+void launch_apply_vorticity(pbf_context* c, float4* vio, uint64_t b, uint64_t e);
+void launch_surface_tension(pbf_context* c, uint64_t cb, uint64_t ce, uint64_t b, uint64_t e);
+void launch_copy_f4(const float4* src, float4* dst, uint64_t n, hipStream_t s);
+// original-order transfer helpers
+void launch_gather_out_f4(pbf_context* c, const float4* src, float* dst3, int comps);
+void launch_gather_out_f1(pbf_context* c, const float* src, float* dst);
+void launch_scatter_in_f4(pbf_context* c, const float* src3, float4* dst);
+void launch_scatter_in_f1(pbf_context* c, const float* src, float* dst);
+
+}  // namespace pbf

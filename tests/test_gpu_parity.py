@@ -1,0 +1,270 @@
+"""GPU parity: the HIP library against the CPU oracle, through the C ABI.
+
+Integer and index work (cell keys, particle order, bounds, neighbour lists)
+must be bit-exact.  The floating-point passes are compiled without fused
+multiply-add and with correctly rounded divide and square root, and walk the
+neighbour lists in the reference's order, so they are required to be
+bit-exact as well against the oracle's Jacobi mode (the device's semantics).
+Against the oracle's serial mode (the reference's CPP_PAR backend, whose XSPH
+updates velocities in place) the tolerance is stated in
+test_serial_mode_difference.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _pair(pbf, params, pos, vel, mode=O.JACOBI, threads=THREADS):
+    g = pbf.Particles(params)
+    g.add(pos, pos, vel)
+    o = O.Oracle(O.params_from_pbf(params), mode=mode, threads=threads)
+    o.set_state(pos, pos, vel)
+    return g, o
+
+
+def _f32(a):
+    return np.asarray(a, dtype=np.float32)
+
+
+def assert_bitwise(gpu, ref, what):
+    gpu = _f32(gpu)
+    ref = _f32(ref)  # oracle values are float32 values held in double: exact
+    assert gpu.shape == ref.shape, what
+    assert np.all(np.isfinite(gpu)), what + ": non-finite values"
+    bad = gpu != ref
+    if bad.any():
+        idx = np.argwhere(bad)[:5]
+        diff = np.abs(gpu.astype(np.float64) - ref.astype(np.float64)).max()
+        pytest.fail(f"{what}: {int(bad.sum())} mismatches (max |diff| {diff:.3e}); first {idx.tolist()}")
+
+
+def _neighbor_params(pbf):
+    # neighbor_test.ini: 5 x 5 x 5 box, bin spacing 1.0 (smoothing radius is not read there)
+    p = pbf.Params()
+    p.max_particles_local = 1000
+    p.solve_step_count = 4
+    p.max_neighbors = 60
+    p.particle_rest_spacing = 0.5
+    p.smoothing_radius = 1.0
+    p.neighbor_bin_spacing = 1.0
+    p.time_step = 1.0
+    for i in range(3):
+        p.boundary_max[i] = 5.0
+        p.mover_center[i] = -10.0
+    pbf.derive(p)
+    return p
+
+
+def _points(nx, ny, nz, spacing):
+    # builders.cpp:49-63: spacing * i + spacing / 2, x fastest
+    s = np.float32(spacing)
+    out = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                out.append([s * np.float32(i) + s / np.float32(2.0), s * np.float32(j) + s / np.float32(2.0),
+                            s * np.float32(k) + s / np.float32(2.0)])
+    return np.array(out, dtype=np.float32)
+
+
+def test_bin_ids_known_answers(pbf):
+    """neighbor_test.cpp:94-121 (3D): bin ids 0, 49+9-1, 49+49+17-1, 49*6-8-1."""
+    p = _neighbor_params(pbf)
+    pts = np.array([[-1.0] * 3, [0.0] * 3, [1.0] * 3, [4.999] * 3], dtype=np.float32)
+    g = pbf.Particles(p)
+    g.add(pts, pts, np.zeros_like(pts))
+    assert g.bin_dimensions() == (7, 7, 7)
+    g.find_neighbors()
+    keys = g.bin_ids()
+    ids = g.particle_ids()
+    by_particle = np.empty(4, dtype=np.uint64)
+    by_particle[ids.astype(np.int64)] = keys
+    assert by_particle.tolist() == [0, 49 + 9 - 1, 49 + 49 + 17 - 1, 49 * 6 - 8 - 1]
+
+
+def test_neighbor_counts_known_answers(pbf):
+    """neighbor_test.cpp:211-298: 10^3 lattice at half the bin spacing."""
+    p = _neighbor_params(pbf)
+    pts = _points(10, 10, 10, 0.5)
+    g, o = _pair(pbf, p, pts, np.zeros_like(pts))
+    g.find_neighbors()
+    lists, counts = g.neighbor_lists()
+    idx = lambda i, j, k: k * 100 + j * 10 + i
+    for k in range(1, 9):
+        for j in range(1, 9):
+            for i in range(1, 9):
+                assert counts[idx(i, j, k)] == 26
+    for j in range(1, 9):
+        for (i, k) in ((0, 0), (9, 0), (0, 9), (9, 9)):
+            assert counts[idx(i, j, k)] == 11
+    for i in range(1, 9):
+        for (j, k) in ((0, 0), (9, 0), (0, 9), (9, 9)):
+            assert counts[idx(i, j, k)] == 11
+    for a in range(1, 9):
+        for b in range(1, 9):
+            for f in (idx(a, b, 0), idx(a, b, 9), idx(a, 9, b), idx(a, 0, b), idx(0, a, b), idx(9, a, b)):
+                assert counts[f] == 17
+    for c in (0, 99, 9, 90, 900, 999, 909, 990):
+        assert counts[c] == 7
+    # and the lists themselves (order included) match the oracle
+    o.pass_(O.FIND)
+    nb = o.neighbors()
+    assert np.array_equal(counts, nb["counts"])
+    for pi in range(pts.shape[0]):
+        c = counts[pi]
+        assert np.array_equal(lists[pi, :c], nb["lists"][pi, :c])
+
+
+def _small_scene(scenes, seed=3, counts=(16, 12, 10), box=(1.6, 0.8, 0.8), **kw):
+    return scenes.dam_break(counts, box, seed=seed, **kw)
+
+
+def test_binning_sort_bounds_lists_bit_exact(pbf, scenes):
+    p, pos, vel = _small_scene(scenes)
+    rng = np.random.default_rng(11)
+    vel = rng.normal(0, 0.5, size=pos.shape).astype(np.float32)
+    g, o = _pair(pbf, p, pos, vel)
+    for gg in (g,):
+        gg.apply_external_forces()
+        gg.predict_positions()
+        gg.find_neighbors()
+    o.pass_(O.FORCES)
+    o.pass_(O.PREDICT)
+    o.pass_(O.FIND)
+    nb = o.neighbors()
+    assert_bitwise(g.position_stars(), o.state()["xs"], "predicted x*")
+    assert np.array_equal(g.bin_ids(), nb["keys"]), "sorted cell keys"
+    assert np.array_equal(g.particle_ids(), nb["ids"]), "sorted particle ids (stable order)"
+    b, e = g.bin_bounds()
+    assert np.array_equal(b, nb["begin"]) and np.array_equal(e, nb["end"]), "cell bounds"
+    lists, counts = g.neighbor_lists()
+    assert np.array_equal(counts, nb["counts"])
+    mask = np.arange(p.max_neighbors)[None, :] < counts[:, None]
+    assert np.array_equal(np.where(mask, lists, 0), np.where(mask, nb["lists"], 0)), "neighbour lists"
+
+
+def _compare_state(g, o, what, fields=("x", "xs", "v", "rho", "lam")):
+    s = o.state()
+    got = {"x": g.positions(), "xs": g.position_stars(), "v": g.velocities(), "rho": g.densities(),
+           "lam": g.lambdas()}
+    for f in fields:
+        assert_bitwise(got[f], s[f], f"{what}: {f}")
+
+
+def test_one_step_bit_exact(pbf, scenes):
+    p, pos, vel = _small_scene(scenes, seed=5)
+    g, o = _pair(pbf, p, pos, vel)
+    g.step(1)
+    o.step(1)
+    _compare_state(g, o, "1 step")
+
+
+def test_per_pass_api_equals_oracle_passes(pbf, scenes):
+    """Each ABI pass against the oracle pass of the same name, in main.cpp order."""
+    p, pos, vel = _small_scene(scenes, seed=7)
+    g, o = _pair(pbf, p, pos, vel)
+    for _ in range(2):
+        g.apply_external_forces(); o.pass_(O.FORCES)
+        g.predict_positions(); o.pass_(O.PREDICT)
+        g.find_neighbors(); o.pass_(O.FIND)
+        for _it in range(p.solve_step_count):
+            g.compute_densities(); o.pass_(O.DENSITY)
+            g.compute_pressure_lambdas(); o.pass_(O.LAMBDA)
+            g.compute_pressure_dps(); o.pass_(O.DP)
+            assert_bitwise(g.scratch(), o.state()["scratch"], "dp")
+            g.update_position_stars(); o.pass_(O.UPDATE_XS)
+        g.update_velocities(); o.pass_(O.UPDATE_V)
+        g.apply_viscosity(); o.pass_(O.VISC)
+        g.compute_vorticity(); o.pass_(O.VORT)
+        assert_bitwise(g.scratch(), o.state()["scratch"], "vorticity")
+        g.apply_vorticity(); o.pass_(O.APPLY_VORT)
+        g.apply_viscosity(); o.pass_(O.VISC)
+        g.update_positions(); o.pass_(O.UPDATE_X)
+        _compare_state(g, o, "per-pass step")
+
+
+def test_fused_step_equals_per_pass(pbf, scenes):
+    p, pos, vel = _small_scene(scenes, seed=9)
+    a = pbf.Particles(p)
+    b = pbf.Particles(p)
+    for q in (a, b):
+        q.add(pos, pos, vel)
+    for _ in range(3):
+        a.step(1)
+        b.apply_external_forces(); b.predict_positions(); b.find_neighbors()
+        for _it in range(p.solve_step_count):
+            b.compute_densities(); b.compute_pressure_lambdas(); b.compute_pressure_dps(); b.update_position_stars()
+        b.update_velocities(); b.apply_viscosity(); b.compute_vorticity(); b.apply_vorticity()
+        b.apply_viscosity(); b.update_positions()
+    for f in ("positions", "position_stars", "velocities", "densities", "lambdas"):
+        assert_bitwise(getattr(a, f)(), getattr(b, f)(), "fused vs per-pass " + f)
+
+
+def test_c1_one_and_hundred_substeps_bit_exact(pbf, scenes):
+    """32,768-particle dam break: 1 and 100 substeps, GPU == oracle (Jacobi) bit for bit."""
+    p, pos, vel = scenes.config_c1(seed=0)
+    g, o = _pair(pbf, p, pos, vel)
+    g.step(1)
+    o.step(1)
+    _compare_state(g, o, "C1 substep 1")
+    g.step(99)
+    o.step(99)
+    _compare_state(g, o, "C1 substep 100")
+    st = g.stats()
+    assert st["neighbor_overflow"] == o.counters()["overflow"]
+
+
+def test_density_fix_and_lambda_repair(pbf, scenes):
+    """Two coincident particles trigger particles.h:303-305; lambdas must still match."""
+    p, pos, vel = _small_scene(scenes, seed=13, counts=(8, 8, 8), box=(0.8, 0.8, 0.8))
+    pos = pos.copy()
+    pos[10] = pos[11]  # exact duplicate position and equal velocity: still coincident after predict
+    vel = np.zeros_like(pos)
+    g, o = _pair(pbf, p, pos, vel)
+    g.step(1)
+    o.step(1)
+    assert g.stats()["density_hacks"] > 0
+    assert o.counters()["hacks"] > 0
+    _compare_state(g, o, "step with density fix")
+
+
+def test_spans_and_remove(pbf, scenes):
+    """Sub-spans leave other particles untouched; remove() drops the last particles."""
+    p, pos, vel = _small_scene(scenes, seed=17, counts=(8, 8, 8), box=(0.8, 0.8, 0.8))
+    vel = np.ones_like(pos)
+    g, o = _pair(pbf, p, pos, vel)
+    half = pos.shape[0] // 2
+    g.apply_external_forces(pbf.span(0, half))
+    o.pass_(O.FORCES, 0, half)
+    assert_bitwise(g.velocities(), o.state()["v"], "forces on half span")
+    g.remove(10)
+    assert g.local_count() == pos.shape[0] - 10
+    assert_bitwise(g.velocities(), o.state()["v"][:-10], "after remove")
+
+
+def test_sort_by_key_primitive(pbf):
+    rng = np.random.default_rng(0)
+    for n, bits in ((1, 8), (1000, 5), (100_003, 20), (300_000, 32)):
+        keys = rng.integers(0, 1 << bits, size=n, dtype=np.uint64).astype(np.uint32)
+        vals = np.arange(n, dtype=np.uint32)
+        sk, sv = pbf.sort_by_key_u32(keys, vals, bits)
+        order = np.argsort(keys, kind="stable")
+        assert np.array_equal(sk, keys[order])
+        assert np.array_equal(sv, vals[order])
+
+
+def test_bound_primitives(pbf):
+    rng = np.random.default_rng(1)
+    keys = np.sort(rng.integers(0, 500, size=10_000)).astype(np.uint32)
+    lo = pbf.bound_counting(keys, 0, 520, upper=False)
+    hi = pbf.bound_counting(keys, 0, 520, upper=True)
+    vals = np.arange(520)
+    assert np.array_equal(lo, np.searchsorted(keys, vals, side="left"))
+    assert np.array_equal(hi, np.searchsorted(keys, vals, side="right"))

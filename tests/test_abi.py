@@ -1,0 +1,91 @@
+"""CPU checks of the C ABI: the library loads, exports every declared entry
+point, and its host-side logic (parameter derivation, argument checks) works
+without a GPU.  No compute call is made here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pbf.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pbf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_reference_surface():
+    names = declared_functions()
+    # one entry per for_each_index body of particles.h / neighbors.h
+    for n in ("pbf_apply_external_forces", "pbf_predict_positions", "pbf_find_neighbors", "pbf_compute_densities",
+              "pbf_compute_pressure_lambdas", "pbf_compute_pressure_dps", "pbf_update_position_stars",
+              "pbf_update_velocities", "pbf_apply_surface_tension", "pbf_apply_viscosity", "pbf_compute_vorticity",
+              "pbf_apply_vorticity", "pbf_update_positions", "pbf_sort_by_key_u32", "pbf_lower_bound_counting",
+              "pbf_upper_bound_counting", "pbf_step"):
+        assert n in names, n
+
+
+def test_library_exports_every_declared_symbol(pbf):
+    L = pbf.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_params_derive_defaults(pbf):
+    p = pbf.Params()
+    p.particle_rest_spacing = 0.05
+    p.solve_step_count = 4
+    p.time_step = 1 / 120
+    pbf.derive(p)
+    # parameters.h:127-145
+    assert p.particle_radius == pytest.approx(0.025)
+    assert p.smoothing_radius == pytest.approx(0.09)
+    assert p.neighbor_bin_spacing == pytest.approx(0.108)
+    assert p.rest_mass == 1.0
+    assert p.rest_density == pytest.approx(8000.0, rel=1e-6)
+    assert p.max_speed == pytest.approx(0.5 * 0.09 * 4 * 120, rel=1e-6)
+    assert p.max_neighbors == 60
+
+
+def test_derive_rejects_bad_spacing(pbf):
+    p = pbf.Params()
+    p.particle_rest_spacing = 0.0
+    with pytest.raises(pbf.PBFError) as e:
+        pbf.derive(p)
+    assert e.value.code == pbf.PBF_ERR_ARG
+    assert b"spacing" in pbf.lib().pbf_last_error()
+
+
+def test_create_rejects_bad_arguments_before_touching_the_device(pbf):
+    L = pbf.lib()
+    ctx = C.c_void_p()
+    p = pbf.Params()
+    p.max_particles_local = 0
+    assert L.pbf_create(C.byref(p), C.byref(ctx)) == pbf.PBF_ERR_ARG
+    p.max_particles_local = 10
+    p.smoothing_radius = 0.1
+    p.neighbor_bin_spacing = 0.12
+    p.max_neighbors = 60
+    p.time_step = 0.01
+    p.boundary_min[0] = 1.0  # the reference bins from the origin (neighbors.h:110-113)
+    p.boundary_max[0] = p.boundary_max[1] = p.boundary_max[2] = 2.0
+    assert L.pbf_create(C.byref(p), C.byref(ctx)) == pbf.PBF_ERR_UNSUPPORTED
+    assert not ctx.value
+
+
+def test_scene_builders(scenes):
+    params, pos, vel = scenes.config_c1(seed=0)
+    assert pos.shape == (32768, 3) and pos.dtype == np.float32
+    assert params.rest_density == 1000.0 and params.rest_mass == np.float32(0.125)
+    assert params.neighbor_bin_spacing == pytest.approx(0.12)
+    assert np.all(pos > 0) and np.all(pos[:, 0] < 1.6) and np.all(pos[:, 1] < 1.6)
+    # lattice at cell centres plus at most 0.01 h of jitter (particles.h:237-248)
+    lat = scenes.lattice((32, 32, 32))
+    assert np.abs(pos - lat).max() <= 0.001 + 1e-7
+    p2, pos2, _ = scenes.config_c2(seed=0)
+    assert pos2.shape[0] == 1_048_576
+    assert tuple(p2.boundary_max) == pytest.approx((12.8, 6.4, 6.4))

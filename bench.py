@@ -94,6 +94,38 @@ def _profile_traffic(kernel: str):
         return None
 
 
+KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "fill_neighbors", "density_lambda",
+           "lambda_repair", "dp_update", "viscosity", "vorticity", "apply_vorticity", "viscosity_positions"]
+
+
+def timed_run(pbf, params, pos, vel, steps, profile=True):
+    """K substeps from the initial scene on a fresh context; HIP events around
+    every kernel launch on the library's stream when profile is set."""
+    n = pos.shape[0]
+    g = pbf.Particles(params)
+    g.add(pos, pos, vel)
+    if profile:
+        g.profile_kernels(KERNELS)
+    g.sync()
+    t0 = time.perf_counter()
+    g.region_begin()
+    g.step(steps)
+    ev_ms = g.region_end()
+    g.sync()
+    wall = time.perf_counter() - t0
+    kern = {}
+    if profile:
+        for k in KERNELS:
+            launches, ms = g.kernel_time(k)
+            kern[k] = {"launches": launches, "total_ms": ms}
+    st = g.stats()
+    res = {"value": n * steps / wall, "seconds": wall, "event_ms": ev_ms, "kernels": kern, "cells": g.num_cells(),
+           "stats": {"neighbor_overflow": st["neighbor_overflow"], "density_hacks": st["density_hacks"],
+                     "max_density_error": round(st["max_density_error"], 5)}}
+    g.close()
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,7 +134,9 @@ def main(argv=None):
     ap.add_argument("--lattice", default="128x64x128", help="block size in particles (x by y by z)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--kernel", default="", help="kernel to time (default: the slowest in a short probe)")
+    ap.add_argument("--kernel", default="", help="kernel for the roofline (default: the slowest in the timed run)")
+    ap.add_argument("--cap", type=int, default=60, help="neighbour list cap (MAX_NEIGHBORS = 60)")
+    ap.add_argument("--no-variants", dest="variants", action="store_false", help="skip the cap-128 variant")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -113,44 +147,23 @@ def main(argv=None):
     pbf = _load("sph-cpp_amd")
     scenes = _load("sph-cpp_amd.scenes")
     counts = tuple(int(c) for c in args.lattice.lower().split("x"))
-    params, pos, vel = scenes.config_c2(seed=args.seed, lattice_counts=counts)
+    params, pos, vel = scenes.config_c2(seed=args.seed, lattice_counts=counts, cap=args.cap)
     n = pos.shape[0]
 
-    g = pbf.Particles(params)
-    g.add(pos, pos, vel)
+    # warm-up on a throwaway context (module load, first launches)
+    w = pbf.Particles(params)
+    w.add(pos, pos, vel)
+    w.step(args.warmup)
+    w.sync()
+    w.close()
 
-    # short probe: which kernel dominates a substep
-    kernels = ["forces_predict", "bin", "scan", "sort", "reorder", "fill_neighbors", "density_lambda",
-               "lambda_repair", "dp_update", "viscosity", "vorticity", "apply_vorticity", "viscosity_positions"]
-    g.profile_kernels(kernels)
-    g.step(3)
-    g.sync()
-    probe = {k: g.kernel_time(k)[1] for k in kernels}
-    dom = args.kernel or max(probe, key=probe.get)
-    g.profile_kernels("")
-
-    g.step(args.warmup)
-    g.sync()
-
-    # timed region; HIP events around the dominant kernel's launches
-    g.profile_kernels(dom)
-    g.sync()
-    t0 = time.perf_counter()
-    g.region_begin()
-    g.step(args.steps)
-    ev_ms = g.region_end()
-    g.sync()
-    wall = time.perf_counter() - t0
-    launches, kms = g.kernel_time(dom)
-    g.profile_kernels("")
-    stats = g.stats()
-
-    seconds = wall
-    value = n * args.steps / seconds
-    avg_launch_ms = kms / max(1, launches)
+    res = timed_run(pbf, params, pos, vel, args.steps)
+    dom = args.kernel or max(res["kernels"], key=lambda k: res["kernels"][k]["total_ms"])
+    value = res["value"]
+    kd = res["kernels"][dom]
+    avg_launch_ms = kd["total_ms"] / max(1, kd["launches"])
     alg = ALG_BYTES.get(dom, 0) * n
     achieved = alg / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
-    traffic = _profile_traffic(dom)
     roof = {
         "bound": "hbm",
         "kernel": dom,
@@ -158,14 +171,14 @@ def main(argv=None):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": traffic,
+        "traffic": _profile_traffic(dom),
         "alg_bytes_per_launch": alg,
         "avg_launch_ms": round(avg_launch_ms, 5),
-        "launches": launches,
+        "launches": kd["launches"],
     }
     # whole-substep bookkeeping: B_alg = 260 + 44 I + 8 C/N bytes per particle-substep
     I = params.solve_step_count
-    b_alg = 260 + 44 * I + 8.0 * g.num_cells() / n
+    b_alg = 260 + 44 * I + 8.0 * res["cells"] / n
     step_roof = {"alg_bytes_per_particle_substep": round(b_alg, 2),
                  "achieved_GBs": round(b_alg * value / 1e9, 2),
                  "frac_of_hbm_peak": round(b_alg * value / 1e9 / HBM_PEAK_GBS, 5)}
@@ -177,7 +190,7 @@ def main(argv=None):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": seconds * 1e3 / args.steps,
+        "ms_per_step": res["seconds"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -185,21 +198,25 @@ def main(argv=None):
         "data": "synthetic",
         "config": {
             "workload": f"plain PBF dam break, {n} particles ({args.lattice} lattice, spacing 0.05, h 0.1) in "
-                        f"12.8x6.4x6.4, dt 1/120, {I} Jacobi iterations, eps 600, XSPH 0.01, vorticity 0.0005",
+                        f"12.8x6.4x6.4, dt 1/120, {I} Jacobi iterations, eps 600, XSPH 0.01, vorticity 0.0005, "
+                        f"substeps 1..{args.steps} from the initial scene",
             "particles": n,
             "solver": "plain, fixed iterations",
             "jacobi_iterations": I,
             "neighbor_cap": int(params.max_neighbors),
-            "cells": g.num_cells(),
+            "cells": res["cells"],
         },
         "roofline": roof,
         "step_roofline": step_roof,
-        "event_ms_total": ev_ms,
-        "probe_ms": {k: round(v, 4) for k, v in probe.items()},
-        "stats": {"neighbor_overflow": stats["neighbor_overflow"], "density_hacks": stats["density_hacks"],
-                  "max_density_error": stats["max_density_error"]},
+        "kernels_ms_per_step": {k: round(v["total_ms"] / args.steps, 4) for k, v in res["kernels"].items()},
+        "stats_last_step": res["stats"],
     }
-    g.close()
+    if args.variants:
+        vparams, _, _ = scenes.config_c2(seed=args.seed, lattice_counts=counts, cap=128)
+        vr = timed_run(pbf, vparams, pos, vel, args.steps, profile=False)
+        out["variants"] = {"cap128": {"value": vr["value"], "ms_per_step": vr["seconds"] * 1e3 / args.steps,
+                                      "stats_last_step": vr["stats"],
+                                      "note": "same scene with a 128-entry neighbour list (no list saturation)"}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(params, pos, vel)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]

@@ -71,7 +71,14 @@ typedef struct pbf_params {
   float boundary_min[3];         /* must be the origin (neighbors.h:110-113) */
   float boundary_max[3];
   float mover_center[3];
+  uint32_t options;              /* PBF_OPT_* bits (no counterpart in the reference) */
 } pbf_params;
+
+/* Count neighbour candidates dropped by the list cap (pbf_stats.neighbor_overflow).
+ * The reference truncates silently (neighbors.h:223); counting needs the full
+ * 27-cell walk, so without this option a full list ends the walk early.  Lists
+ * are identical either way. */
+#define PBF_OPT_COUNT_OVERFLOW 1u
 
 typedef struct pbf_context pbf_context;
 

@@ -27,6 +27,8 @@ PBF_ERR_HIP = 3
 PBF_ERR_STATE = 4
 PBF_ERR_UNSUPPORTED = 5
 
+PBF_OPT_COUNT_OVERFLOW = 1
+
 
 class PBFError(RuntimeError):
     def __init__(self, code: int, msg: str):
@@ -58,6 +60,7 @@ class Params(C.Structure):
         ("boundary_min", C.c_float * 3),
         ("boundary_max", C.c_float * 3),
         ("mover_center", C.c_float * 3),
+        ("options", C.c_uint32),
     ]
 
     def copy(self) -> "Params":

@@ -143,6 +143,8 @@ static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
   K.dxy = dims[0] * dims[1];
   K.ncells = (uint32_t)(dims[0] * dims[1] * dims[2]);
   K.cap = (int)P.max_neighbors;
+  K.stride = (K.cap + 7) / 8 * 8;
+  K.count_overflow = (P.options & PBF_OPT_COUNT_OVERFLOW) ? 1 : 0;
   return PBF_OK;
 }
 
@@ -160,14 +162,10 @@ static void dfree(void* p) {
 static int alloc_grid(pbf_context* c) {
   const uint64_t need = c->ncells + 2;
   if (need <= c->grid_alloc) return PBF_OK;
-  dfree(c->cell_count);
   dfree(c->cell_start);
-  dfree(c->scan_partials);
-  c->cell_count = c->cell_start = c->scan_partials = nullptr;
+  c->cell_start = nullptr;
   int rc;
-  if ((rc = dalloc(&c->cell_count, need)) || (rc = dalloc(&c->cell_start, need)) ||
-      (rc = dalloc(&c->scan_partials, scan_partials_size(need))))
-    return rc;
+  if ((rc = dalloc(&c->cell_start, need))) return rc;
   c->grid_alloc = need;
   return PBF_OK;
 }
@@ -176,7 +174,8 @@ static void free_all(pbf_context* c) {
   void* ptrs[] = {c->x,    c->xs,   c->v,    c->xa,   c->xsa,  c->va,   c->xt,         c->scr,
                   c->scra, c->rho,  c->lam,  c->rhoa, c->lama, c->oid,  c->oida,       c->key,
                   c->keya, c->slot, c->tmp,  c->tmpo, c->nbr,  c->nbr_count, c->counters, c->stage,
-                  c->stage64, c->cell_count, c->cell_start, c->scan_partials};
+                  c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
+                  c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -265,9 +264,15 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->lam, N)) || (rc = dalloc(&c->rhoa, N)) || (rc = dalloc(&c->lama, N)) ||
       (rc = dalloc(&c->oid, N)) || (rc = dalloc(&c->oida, N)) || (rc = dalloc(&c->key, N)) ||
       (rc = dalloc(&c->keya, N)) || (rc = dalloc(&c->slot, N)) || (rc = dalloc(&c->tmp, N + 1)) ||
-      (rc = dalloc(&c->tmpo, N + 1)) || (rc = dalloc(&c->nbr, N * (uint64_t)c->cap)) ||
+      (rc = dalloc(&c->tmpo, N + 1)) || (rc = dalloc(&c->nbr, N * (uint64_t)c->K.stride)) ||
       (rc = dalloc(&c->nbr_count, N)) || (rc = dalloc(&c->counters, (uint64_t)C_NUM)) ||
       (rc = dalloc(&c->stage, 4 * N)) || (rc = dalloc(&c->stage64, N + 1)) || (rc = alloc_grid(c)) ||
+      (rc = dalloc(&c->sk[0], N + 1)) || (rc = dalloc(&c->sk[1], N + 1)) || (rc = dalloc(&c->sv[0], N + 1)) ||
+      (rc = dalloc(&c->sv[1], N + 1)) || (rc = dalloc(&c->rhist, radix_hist_size(N))) ||
+      (rc = dalloc(&c->roffs, radix_hist_size(N) + 1)) ||
+      (rc = dalloc(&c->rpart, scan_partials_size(radix_hist_size(N) + 1))) || (rc = dalloc(&c->heavy, N)) ||
+      (rc = dalloc(&c->cnb, N * (uint64_t)c->K.stride)) || (rc = dalloc(&c->cnb_count, N)) ||
+      (rc = dalloc(&c->scan_partials, scan_partials_size(N + 1))) ||
       (rc = check_hip(hipEventCreate(&c->region_a), "hipEventCreate")) ||
       (rc = check_hip(hipEventCreate(&c->region_b), "hipEventCreate"))) {
     free_all(c);
@@ -359,6 +364,7 @@ int pbf_remove(pbf_context* c, uint64_t count) {
   launch_exclusive_scan(c->tmp, c->tmpo, n, c->scan_partials, c->stream);
   hipLaunchKernelGGL(k_compact_perm, dim3(nb), dim3(256), 0, c->stream, n, (const uint32_t*)c->oid, newn,
                      (const uint32_t*)c->tmpo, c->slot);
+  c->perm = c->slot;
   c->n = newn;
   launch_reorder(c);
   c->lists_valid = false;
@@ -525,17 +531,17 @@ int pbf_step(pbf_context* c, uint32_t nsteps) {
     find_neighbors_impl(c, n, 0, n);
     const uint32_t I = c->P.solve_step_count;
     for (uint32_t it = 0; it < I; ++it) {
-      launch_density_lambda(c, 1, 0, n);      // xs -> xt, rho, lam
-      launch_lambda(c, c->xt, 1, 0, n);       // only if the density fix fired
-      launch_dp_update(c, it + 1 == I, 0, n); // xt, lam -> xs (+ v)
+      launch_density_lambda_masked(c, (int)it);          // xs -> xt, rho, lam, mask
+      launch_lambda_repair(c, (int)it);                  // only if the density fix fired
+      launch_dp_update_masked(c, (int)it, it + 1 == I);  // xt, lam -> xs (+ v)
     }
     if (I == 0) launch_update_velocities(c, 0, n);
     c->last_iterations = I;
     if (c->P.gamma != 0.f) launch_surface_tension(c, 0, n, 0, n);
-    launch_viscosity(c, c->v, c->va, 0, 0, n);  // XSPH 1: v -> va
-    launch_vorticity(c, c->va, 0, n);           // omega -> scr
-    launch_apply_vorticity(c, c->va, 0, n);     // va += ...
-    launch_viscosity(c, c->va, c->v, 1, 0, n);  // XSPH 2: va -> v, x = x*
+    launch_viscosity_first(c);         // XSPH 1: v -> va, mask of final positions
+    launch_vorticity_masked(c);        // omega -> scr
+    launch_apply_vorticity_masked(c);  // va += ...
+    launch_viscosity_last(c);          // XSPH 2: va -> v, x = x*
   }
   return check_hip(hipGetLastError(), "pbf_step");
 }
@@ -722,15 +728,9 @@ int pbf_region_end(pbf_context* c, double* ms) {
 namespace pbf {
 
 static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe) {
-  const uint64_t C = c->ncells;
-  launch_fill_u32(c->cell_count, 0u, C + 1, c->stream);
+  launch_fill_u64(c->counters + C_HEAVY, 0ull, 1, c->stream);
   launch_bin(c, bin_end);
-  {
-    KernelTimer* t = timer_begin(c, "scan");
-    launch_exclusive_scan(c->cell_count, c->cell_start, C + 1, c->scan_partials, c->stream);
-    timer_end(c, t);
-  }
-  launch_scatter_rank(c);
+  launch_sort(c);
   launch_reorder(c);
   launch_fill_neighbors(c, fb, fe);
   c->lists_valid = true;

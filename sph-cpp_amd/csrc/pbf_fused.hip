@@ -1,0 +1,414 @@
+// pbf_fused.hip -- neighbour passes of the fused step (pbf_step).
+//
+// Same arithmetic, order and results as the per-pass kernels in
+// pbf_passes.hip (particles.h:292-487), organised for a 64-wide wavefront:
+//
+//  * Latency: a neighbour walk is a chain of dependent loads (list entry, then
+//    the neighbour's data).  Neighbours are taken in batches of kBatch: the
+//    list entries of a batch come in two 16-byte loads, then all gathers are
+//    issued, then the arithmetic runs in list order.
+//  * Only about half of a particle's listed neighbours lie within h (the list
+//    radius is the 1.2 h bin spacing, neighbors.h:198); the others contribute
+//    exact zeros.  The density pass and the first XSPH pass walk the whole
+//    list and append the entries within h to a compact row (cnb).  The passes
+//    that see the same positions afterwards -- the Jacobi iteration's dp
+//    pass, and vorticity, vorticity-apply and the second XSPH pass -- walk
+//    only the compact row, in the same order.
+//  * The near-coincidence fix of compute_densities (particles.h:303-305) moves
+//    x*_p in the middle of the walk; when it fires, the density pass finishes
+//    that particle with a plain walk and raises the iteration's flag, and the
+//    lambda and dp passes of that iteration fall back to plain walks.
+#include "pbf_device.h"
+
+namespace pbf {
+
+constexpr int kBlock = 256;
+constexpr int kBatch = 8;
+static inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+__device__ __forceinline__ f3 xyz(float4 t) { return {t.x, t.y, t.z}; }
+
+// Entries k0 .. k0+7 of particle p's column-major list (entry k at list[k*n + p]):
+// for each entry, the 64 lanes of a wavefront read one contiguous segment.
+// Entries at or past cnt are replaced by p, which keeps every gather in bounds.
+__device__ __forceinline__ void load_batch(const int32_t* __restrict__ list, uint32_t n, int k0, int cnt,
+                                           uint32_t p, uint32_t (&q)[kBatch]) {
+#pragma unroll
+  for (int u = 0; u < kBatch; ++u) q[u] = (k0 + u < cnt) ? (uint32_t)list[(uint64_t)(k0 + u) * n + p] : p;
+}
+
+// ---------------------------------------------------------------------------
+// compute_densities (particles.h:292-313) + compute_pressure_lambdas (:318-342)
+// xs -> xt, rho, lam; compact row of the entries within h.
+__global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it, KParams K,
+                                                             const float4* __restrict__ xs,
+                                                             const float4* __restrict__ v, float4* __restrict__ xt,
+                                                             float* __restrict__ rho, float* __restrict__ lam,
+                                                             const int32_t* __restrict__ nbr,
+                                                             const int32_t* __restrict__ nbr_count,
+                                                             int32_t* __restrict__ cnb,
+                                                             int32_t* __restrict__ cnb_count,
+                                                             unsigned long long* __restrict__ counters) {
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  f3 xp = ld3(xs, p);
+  const int cnt = nbr_count[p];
+  // phase 1: compact row of the entries within h; stop at a near-coincident one
+  int nc = 0;
+  int hack_at = -1;
+  for (int k0 = 0; k0 < cnt && hack_at < 0; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(nbr, n, k0, cnt, p, q);
+    float4 t[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const float d2 = magsq3(sub3(xp, xyz(t[u])));
+      if (k0 + u >= cnt || hack_at >= 0 || !within_h(d2, K)) continue;
+      if (d2 < K.T_hack) {  // particles.h:304: handled by the plain walk below
+        hack_at = k0 + u;
+        continue;
+      }
+      cnb[(uint64_t)(nc++) * n + p] = (int32_t)q[u];
+    }
+  }
+  cnb_count[p] = nc;
+  // phase 2: kernel arithmetic over the compact row, in list order
+  float density = K.mass_W0;
+  float sum_C = 0.0f;
+  f3 sg = {0.f, 0.f, 0.f};
+  uint32_t hacks = 0;
+  for (int k0 = 0; k0 < nc; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(cnb, n, k0, nc, p, q);
+    float4 t[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (k0 + u >= nc) continue;
+      const f3 d = sub3(xp, xyz(t[u]));
+      const float r = sqrtf(magsq3(d));
+      density += K.mass * poly6_in(r, K);
+      const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
+      dec3(sg, grad);
+      sum_C += magsq3(grad);
+    }
+  }
+  if (hack_at >= 0) {
+    // particles.h:304-305: move x*_p, then finish this particle with a plain walk
+    for (int k = hack_at; k < cnt; ++k) {
+      const f3 xq = ld3(xs, (uint32_t)nbr[(uint64_t)k * n + p]);
+      f3 d = sub3(xp, xq);
+      float d2 = magsq3(d);
+      if (!within_h(d2, K)) continue;
+      if (d2 < K.T_hack) {
+        dec3(xp, div3(mul3(ld3(v, p), K.dt), 50.0f));
+        ++hacks;
+        d = sub3(xp, xq);
+        d2 = magsq3(d);
+        if (!within_h(d2, K)) continue;
+      }
+      const float r = sqrtf(d2);
+      density += K.mass * poly6_in(r, K);
+      const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
+      dec3(sg, grad);
+      sum_C += magsq3(grad);
+    }
+  }
+  rho[p] = density;
+  st3(xt, p, xp);
+  sum_C += magsq3(sg);
+  const float constraint = density / K.rho0 - 1.0f;
+  const float C_p = ((double)constraint < 0.0) ? 0.0f : constraint;
+  lam[p] = -C_p / (sum_C + K.eps);
+  if (hacks) {
+    atomicAdd(&counters[C_HACKS], (unsigned long long)hacks);
+    atomicExch(&counters[C_HACK_FLAG + (it & 1)], 1ull);
+  }
+}
+
+// compute_pressure_lambdas (particles.h:318-342) from xt for every particle,
+// only when the iteration's density pass moved a particle (plain walks).
+__global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, KParams K,
+                                                          const float4* __restrict__ xt,
+                                                          const float* __restrict__ rho, float* __restrict__ lam,
+                                                          const int32_t* __restrict__ nbr,
+                                                          const int32_t* __restrict__ nbr_count,
+                                                          const unsigned long long* __restrict__ counters) {
+  if (counters[C_HACK_FLAG + (it & 1)] == 0ull) return;
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const f3 xp = ld3(xt, p);
+  float sum_C = 0.0f;
+  f3 sg = {0.f, 0.f, 0.f};
+  const int cnt = nbr_count[p];
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(nbr, n, k0, cnt, p, q);
+    float4 t[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) t[u] = xt[q[u]];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const f3 d = sub3(xp, xyz(t[u]));
+      const float d2 = magsq3(d);
+      if (k0 + u >= cnt || !within_h(d2, K)) continue;
+      const float r = sqrtf(d2);
+      const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
+      dec3(sg, grad);
+      sum_C += magsq3(grad);
+    }
+  }
+  sum_C += magsq3(sg);
+  const float constraint = rho[p] / K.rho0 - 1.0f;
+  const float C_p = ((double)constraint < 0.0) ? 0.0f : constraint;
+  lam[p] = -C_p / (sum_C + K.eps);
+}
+
+// compute_pressure_dps (particles.h:348-358) + update_position_stars
+// (:363-370); on the last iteration also update_velocities (:375-385).
+__global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int last, KParams K,
+                                                        const float4* __restrict__ xt,
+                                                        const float* __restrict__ lam,
+                                                        const float4* __restrict__ x, float4* __restrict__ xs,
+                                                        float4* __restrict__ v, const int32_t* __restrict__ nbr,
+                                                        const int32_t* __restrict__ nbr_count,
+                                                        const int32_t* __restrict__ cnb,
+                                                        const int32_t* __restrict__ cnb_count,
+                                                        unsigned long long* __restrict__ counters) {
+  // clear the next iteration's flag (nobody reads it until the next density pass)
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_HACK_FLAG + ((it + 1) & 1)] = 0ull;
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  // after the density fix, positions moved: walk the whole list with the test
+  const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull;
+  const f3 xp = ld3(xt, p);
+  const float lp = lam[p];
+  f3 dp = {0.f, 0.f, 0.f};
+  const int cnt = plain ? nbr_count[p] : cnb_count[p];
+  const int32_t* list = plain ? nbr : cnb;
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(list, n, k0, cnt, p, q);
+    float4 t[kBatch];
+    float lq[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      t[u] = xt[q[u]];
+      lq[u] = lam[q[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const f3 d = sub3(xp, xyz(t[u]));
+      const float d2 = magsq3(d);
+      if (k0 + u >= cnt || (plain && !within_h(d2, K))) continue;
+      const float r = sqrtf(d2);
+      acc3(dp, smul3(lp + lq[u], spiky_in(d, r, K)));
+    }
+  }
+  f3 nw = add3(xp, smul3(K.inv_rho0, dp));
+  boundary(nw, K);
+  st3(xs, p, nw);
+  if (last) {
+    f3 vel = div3(sub3(nw, ld3(x, p)), K.dt);
+    if ((double)magsq3(vel) < K.vthr) vel = {0.f, 0.f, 0.f};
+    st3(v, p, vel);
+  }
+}
+
+// XSPH sum (particles.h:441-444) over a compact row
+__device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ cnb, uint32_t n, int nc, uint32_t p, f3 xp,
+                                       f3 vp, const KParams& K, const float4* __restrict__ xs,
+                                       const float* __restrict__ rho, const float4* __restrict__ vin) {
+  f3 dv = {0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < nc; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(cnb, n, k0, nc, p, q);
+    float4 t[kBatch], vq[kBatch];
+    float rq[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      t[u] = xs[q[u]];
+      vq[u] = vin[q[u]];
+      rq[u] = rho[q[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (k0 + u >= nc) continue;
+      const float r = sqrtf(magsq3(sub3(xp, xyz(t[u]))));
+      acc3(dv, div3(mul3(sub3(xyz(vq[u]), vp), poly6_in(r, K)), rq[u]));
+    }
+  }
+  return dv;
+}
+
+// ---------------------------------------------------------------------------
+// apply_viscosity (particles.h:436-447), first call: walks the whole list and
+// leaves the compact rows of the final positions for the passes that follow.
+__global__ void __launch_bounds__(kBlock) k_viscosity_first(uint32_t n, KParams K, const float4* __restrict__ xs,
+                                                            const float* __restrict__ rho,
+                                                            const float4* __restrict__ vin,
+                                                            float4* __restrict__ vout,
+                                                            const int32_t* __restrict__ nbr,
+                                                            const int32_t* __restrict__ nbr_count,
+                                                            int32_t* __restrict__ cnb,
+                                                            int32_t* __restrict__ cnb_count) {
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const f3 xp = ld3(xs, p);
+  const f3 vp = ld3(vin, p);
+  const int cnt = nbr_count[p];
+  int nc = 0;
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(nbr, n, k0, cnt, p, q);
+    float4 t[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const float d2 = magsq3(sub3(xp, xyz(t[u])));
+      if (k0 + u < cnt && within_h(d2, K)) cnb[(uint64_t)(nc++) * n + p] = (int32_t)q[u];
+    }
+  }
+  cnb_count[p] = nc;
+  const f3 dv = xsph_sum(cnb, n, nc, p, xp, vp, K, xs, rho, vin);
+  f3 nv = vp;
+  acc3(nv, smul3(K.visc, dv));
+  st3(vout, p, nv);
+}
+
+// apply_viscosity, second call, fused with update_positions (particles.h:390-394)
+__global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K, const float4* __restrict__ xs,
+                                                           const float* __restrict__ rho,
+                                                           const float4* __restrict__ vin, float4* __restrict__ vout,
+                                                           float4* __restrict__ x, const int32_t* __restrict__ cnb,
+                                                           const int32_t* __restrict__ cnb_count) {
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const f3 xp = ld3(xs, p);
+  const f3 vp = ld3(vin, p);
+  const f3 dv = xsph_sum(cnb, n, cnb_count[p], p, xp, vp, K, xs, rho, vin);
+  f3 nv = vp;
+  acc3(nv, smul3(K.visc, dv));
+  st3(vout, p, nv);
+  x[p] = make_float4(xp.x, xp.y, xp.z, 0.f);
+}
+
+// compute_vorticity (particles.h:452-466) -> scr
+__global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
+                                                        const float4* __restrict__ vin, float4* __restrict__ scr,
+                                                        const int32_t* __restrict__ cnb,
+                                                        const int32_t* __restrict__ cnb_count) {
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const f3 xp = ld3(xs, p);
+  const f3 vp = ld3(vin, p);
+  const int cnt = cnb_count[p];
+  f3 w = {0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(cnb, n, k0, cnt, p, q);
+    float4 t[kBatch], vq[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      t[u] = xs[q[u]];
+      vq[u] = vin[q[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (k0 + u >= cnt) continue;
+      const f3 d = sub3(xp, xyz(t[u]));
+      const float r = sqrtf(magsq3(d));
+      acc3(w, cross3(sub3(xyz(vq[u]), vp), spiky_in(d, r, K)));
+    }
+  }
+  st3(scr, p, w);
+}
+
+// apply_vorticity (particles.h:471-487), v in place (own particle only)
+__global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
+                                                              const float4* __restrict__ scr,
+                                                              float4* __restrict__ v,
+                                                              const int32_t* __restrict__ cnb,
+                                                              const int32_t* __restrict__ cnb_count) {
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const f3 xp = ld3(xs, p);
+  const int cnt = cnb_count[p];
+  f3 eta = {0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(cnb, n, k0, cnt, p, q);
+    float4 t[kBatch], wq[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      t[u] = xs[q[u]];
+      wq[u] = scr[q[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (k0 + u >= cnt) continue;
+      const f3 d = sub3(xp, xyz(t[u]));
+      const float r = sqrtf(magsq3(d));
+      const float wm = sqrtf(magsq3(xyz(wq[u])));
+      acc3(eta, smul3(wm, spiky_in(d, r, K)));
+    }
+  }
+  const f3 N = div3(eta, sqrtf(magsq3(eta)) + K.r_eps);
+  f3 vp = ld3(v, p);
+  acc3(vp, mul3(mul3(cross3(N, ld3(scr, p)), K.vort), K.dt));
+  st3(v, p, vp);
+}
+
+// ---- launchers ----
+#define PBF_LAUNCH(c, name, kern, ...)                                                                    \
+  do {                                                                                                    \
+    const uint32_t n_ = (uint32_t)(c)->n;                                                                 \
+    if (!n_) return;                                                                                      \
+    KernelTimer* t_ = timer_begin((c), name);                                                             \
+    hipLaunchKernelGGL(kern, dim3(nblocks(n_)), dim3(kBlock), 0, (c)->stream, n_, __VA_ARGS__);           \
+    timer_end((c), t_);                                                                                   \
+  } while (0)
+
+void launch_density_lambda_masked(pbf_context* c, int it) {
+  PBF_LAUNCH(c, "density_lambda", k_density_lambda_c, it, c->K, (const float4*)c->xs, (const float4*)c->v, c->xt,
+             c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb, c->cnb_count,
+             c->counters);
+}
+
+void launch_lambda_repair(pbf_context* c, int it) {
+  PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->K, (const float4*)c->xt, (const float*)c->rho, c->lam,
+             (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, (const unsigned long long*)c->counters);
+}
+
+void launch_dp_update_masked(pbf_context* c, int it, int last) {
+  PBF_LAUNCH(c, "dp_update", k_dp_update_c, it, last, c->K, (const float4*)c->xt, (const float*)c->lam,
+             (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
+             (const int32_t*)c->cnb, (const int32_t*)c->cnb_count, c->counters);
+}
+
+void launch_viscosity_first(pbf_context* c) {
+  PBF_LAUNCH(c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
+             (const float4*)c->v, c->va, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb,
+             c->cnb_count);
+}
+
+void launch_vorticity_masked(pbf_context* c) {
+  PBF_LAUNCH(c, "vorticity", k_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->va, c->scr,
+             (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
+}
+
+void launch_apply_vorticity_masked(pbf_context* c) {
+  PBF_LAUNCH(c, "apply_vorticity", k_apply_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->scr, c->va,
+             (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
+}
+
+void launch_viscosity_last(pbf_context* c) {
+  PBF_LAUNCH(c, "viscosity_positions", k_viscosity_last, c->K, (const float4*)c->xs, (const float*)c->rho,
+             (const float4*)c->va, c->v, c->x, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
+}
+
+}  // namespace pbf

@@ -130,23 +130,36 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32
 }
 
 // ---------------------------------------------------------------------------
-// calculate_bins (neighbors.h:120-125): key of every slot, and its arrival
-// slot inside the cell's bucket.
+// calculate_bins (neighbors.h:120-125): cell key of every slot, and the
+// inverse permutation (original index -> slot) so that the keys can be
+// listed in original-index order for the stable sort.
 __global__ void __launch_bounds__(kBlock) k_bin(const float4* __restrict__ xs, uint32_t n, uint32_t bin_end,
                                                 KParams K, const uint32_t* __restrict__ oid,
-                                                uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
-                                                uint32_t* __restrict__ cell_count,
+                                                uint32_t* __restrict__ key, uint32_t* __restrict__ inv,
                                                 unsigned long long* __restrict__ counters) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const uint32_t o = oid[p];
   uint32_t k = K.ncells;  // particles past the bin span are placed in no cell
-  if (oid[p] < bin_end) {
+  if (o < bin_end) {
     const f3 ps = ld3(xs, p);
     k = cell_key(bin_coord(ps.x, K.s), bin_coord(ps.y, K.s), bin_coord(ps.z, K.s), K);
     if (k == K.ncells) atomicAdd(&counters[C_OUTSIDE], 1ull);
   }
   key[p] = k;
-  slot[p] = atomicAdd(&cell_count[k], 1u);
+  inv[o] = p;
+}
+
+// (key, slot) pairs listed by original index: a stable sort of this sequence
+// by key is the reference's stable sort of (key, index) pairs.
+__global__ void __launch_bounds__(kBlock) k_keys_by_oid(uint32_t n, const uint32_t* __restrict__ key,
+                                                        const uint32_t* __restrict__ inv,
+                                                        uint32_t* __restrict__ sk, uint32_t* __restrict__ sv) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = inv[i];
+  sk[i] = key[s];
+  sv[i] = s;
 }
 
 void launch_bin(pbf_context* c, uint64_t bin_end) {
@@ -154,49 +167,38 @@ void launch_bin(pbf_context* c, uint64_t bin_end) {
   if (!n) return;
   KernelTimer* t = timer_begin(c, "bin");
   hipLaunchKernelGGL(k_bin, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, (const float4*)c->xs, n,
-                     (uint32_t)bin_end, c->K, (const uint32_t*)c->oid, c->key, c->slot, c->cell_count, c->counters);
+                     (uint32_t)bin_end, c->K, (const uint32_t*)c->oid, c->key, c->slot, c->counters);
+  hipLaunchKernelGGL(k_keys_by_oid, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->key,
+                     (const uint32_t*)c->slot, c->sk[0], c->sv[0]);
   timer_end(c, t);
 }
 
-// bucket placement in arrival order: tmp[start + slot] = slot index, tmpo = its original index
-__global__ void __launch_bounds__(kBlock) k_scatter(uint32_t n, const uint32_t* __restrict__ key,
-                                                    const uint32_t* __restrict__ slot,
-                                                    const uint32_t* __restrict__ start,
-                                                    const uint32_t* __restrict__ oid, uint32_t* __restrict__ tmp,
-                                                    uint32_t* __restrict__ tmpo) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t pos = start[key[p]] + slot[p];
-  tmp[pos] = p;
-  tmpo[pos] = oid[p];
+// find_bin_bounds (neighbors.h:137-148): begin[c] = lower_bound(keys, c);
+// end[c] = upper_bound(keys, c) = lower_bound(keys, c + 1) = begin[c + 1].
+__global__ void k_cell_start(const uint32_t* __restrict__ sorted, uint32_t n, uint32_t ncells,
+                             uint32_t* __restrict__ start) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncells + 1) return;
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (sorted[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  start[c] = lo;
 }
 
-// final place inside the bucket = rank by original index (stable order)
-__global__ void __launch_bounds__(kBlock) k_rank(uint32_t n, const uint32_t* __restrict__ key,
-                                                 const uint32_t* __restrict__ start,
-                                                 const uint32_t* __restrict__ tmp,
-                                                 const uint32_t* __restrict__ tmpo, uint32_t* __restrict__ perm) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t p = tmp[s];
-  const uint32_t me = tmpo[s];
-  const uint32_t k = key[p];
-  const uint32_t b = start[k], e = start[k + 1];
-  uint32_t r = 0;
-  for (uint32_t t = b; t < e; ++t) r += (tmpo[t] < me) ? 1u : 0u;
-  perm[b + r] = p;
-}
-
-void launch_scatter_rank(pbf_context* c) {
+void launch_sort(pbf_context* c) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
   KernelTimer* t = timer_begin(c, "sort");
-  hipLaunchKernelGGL(k_scatter, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->key,
-                     (const uint32_t*)c->slot, (const uint32_t*)c->cell_start, (const uint32_t*)c->oid, c->tmp,
-                     c->tmpo);
-  // perm goes to `slot` (its arrival slots are no longer needed)
-  hipLaunchKernelGGL(k_rank, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->key,
-                     (const uint32_t*)c->cell_start, (const uint32_t*)c->tmp, (const uint32_t*)c->tmpo, c->slot);
+  uint32_t bits = 1;
+  while (bits < 32 && (c->K.ncells >> bits) != 0) ++bits;  // keys go up to ncells (the outside bucket)
+  const int which = radix_sort_pairs(c->sk[0], c->sv[0], c->sk[1], c->sv[1], n, bits, c->rhist, c->roffs,
+                                     c->rpart, c->stream);
+  c->perm = c->sv[which];
+  hipLaunchKernelGGL(k_cell_start, dim3(nblocks((uint64_t)c->K.ncells + 2)), dim3(kBlock), 0, c->stream,
+                     (const uint32_t*)c->sk[which], n, c->K.ncells, c->cell_start);
   timer_end(c, t);
 }
 
@@ -228,7 +230,7 @@ void launch_reorder(pbf_context* c) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
   KernelTimer* t = timer_begin(c, "reorder");
-  hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->slot,
+  hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->perm,
                      (const float4*)c->x, (const float4*)c->xs, (const float4*)c->v, (const float*)c->rho,
                      (const float*)c->lam, (const float4*)c->scr, (const uint32_t*)c->oid,
                      (const uint32_t*)c->key, c->xa, c->xsa, c->va, c->rhoa, c->lama, c->scra, c->oida, c->keya);
@@ -244,16 +246,40 @@ void launch_reorder(pbf_context* c) {
 }
 
 // ---------------------------------------------------------------------------
-// fill_neighbors (neighbors.h:197-230).  One lane per particle walks its 27
-// cells in the reference's order (x offset outer, z inner; cell ids from the
-// FP coordinates x + i*s, neighbors.h:171-185), skips itself and accepts
+// fill_neighbors (neighbors.h:197-230).  A particle walks its 27 cells in the
+// reference's order (x offset outer, z inner; cell ids from the FP
+// coordinates x + i*s, neighbors.h:171-185), skips itself and accepts
 // d^2 < s^2 while the list has room.  Lists hold sorted slots, column major.
+//
+// Two tiers with identical results: one lane per particle when its 27 cells
+// hold at most kHeavyCandidates particles; otherwise the particle goes to a
+// list that k_fill_heavy walks with a whole wavefront (64 candidates per
+// step, accepted ones appended in order through a ballot).
+constexpr uint32_t kHeavyCandidates = 1536;
+
+struct WalkCells {
+  float cx[3], cy[3], cz[3];
+};
+
+__device__ __forceinline__ WalkCells walk_cells(f3 ps, float s) {
+  WalkCells w;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float off = (float)(i - 1) * s;
+    w.cx[i] = bin_coord(ps.x + off, s);
+    w.cy[i] = bin_coord(ps.y + off, s);
+    w.cz[i] = bin_coord(ps.z + off, s);
+  }
+  return w;
+}
+
 __global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t fb, uint32_t fe,
                                                            const float4* __restrict__ xs,
                                                            const uint32_t* __restrict__ oid,
                                                            const uint32_t* __restrict__ start, KParams K,
                                                            int32_t* __restrict__ nbr,
                                                            int32_t* __restrict__ nbr_count,
+                                                           uint32_t* __restrict__ heavy,
                                                            unsigned long long* __restrict__ counters) {
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -263,14 +289,17 @@ __global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t 
     return;
   }
   const f3 ps = ld3(xs, p);
-  const float s = K.s;
-  float cx[3], cy[3], cz[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float off = (float)(i - 1) * s;
-    cx[i] = bin_coord(ps.x + off, s);
-    cy[i] = bin_coord(ps.y + off, s);
-    cz[i] = bin_coord(ps.z + off, s);
+  const WalkCells w = walk_cells(ps, K.s);
+  uint32_t total = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], K);
+        if (cell < K.ncells) total += start[cell + 1] - start[cell];
+      }
+  if (total > kHeavyCandidates) {
+    heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
+    return;
   }
   int count = 0;
   uint32_t ovf = 0;
@@ -278,25 +307,80 @@ __global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t 
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 3; ++k) {
-        const uint32_t cell = cell_key(cx[i], cy[j], cz[k], K);
+        const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], K);
         if (cell >= K.ncells) continue;
         const uint32_t b = start[cell], e = start[cell + 1];
-        for (uint32_t q = b; q < e; ++q) {
-          if (q == p) continue;
-          const f3 xq = ld3(xs, q);
-          const float d2 = magsq3(sub3(ps, xq));
-          if (d2 < K.ss) {
-            if (count < cap) {
-              nbr[(uint64_t)count * n + p] = (int32_t)q;
-              ++count;
-            } else {
-              ++ovf;
+        // candidates of a cell are contiguous: load 8 at a time, test in order
+        for (uint32_t q0 = b; q0 < e; q0 += 8) {
+          float4 t[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) t[u] = xs[min(q0 + (uint32_t)u, e - 1u)];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const uint32_t q = q0 + (uint32_t)u;
+            if (q >= e || q == p) continue;
+            const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
+            if (d2 < K.ss) {
+              if (count < cap) {
+                nbr[(uint64_t)count * n + p] = (int32_t)q;
+                ++count;
+                // a full list cannot change any more; only the overflow count could
+                if (count == cap && !K.count_overflow) goto done;
+              } else {
+                ++ovf;
+              }
             }
           }
         }
       }
+done:
   nbr_count[p] = count;
   if (ovf) atomicAdd(&counters[C_OVERFLOW], (unsigned long long)ovf);
+}
+
+// One wavefront per heavy particle; waves stride over the heavy list, whose
+// length is read once per wave (every wave exits after the last entry).
+__global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4* __restrict__ xs,
+                                                       const uint32_t* __restrict__ start, KParams K,
+                                                       int32_t* __restrict__ nbr, int32_t* __restrict__ nbr_count,
+                                                       const uint32_t* __restrict__ heavy,
+                                                       unsigned long long* __restrict__ counters) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t nheavy = (uint32_t)counters[C_HEAVY];
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t cap = (uint32_t)K.cap;
+  for (uint32_t h = wave; h < nheavy; h += nwaves) {
+    const uint32_t p = heavy[h];
+    const f3 ps = ld3(xs, p);
+    const WalkCells w = walk_cells(ps, K.s);
+    uint32_t count = 0;  // accepted so far, uncapped (uniform across the wave)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], K);
+          if (cell >= K.ncells) continue;
+          const uint32_t b = start[cell], e = start[cell + 1];
+          for (uint32_t base = b; base < e; base += 64) {
+            const uint32_t q = base + lane;
+            bool acc = false;
+            if (q < e && q != p) acc = magsq3(sub3(ps, ld3(xs, q))) < K.ss;
+            const uint64_t m = __ballot(acc);
+            if (acc) {
+              const uint32_t pos = count + (uint32_t)__popcll(m & below);
+              if (pos < cap) nbr[(uint64_t)pos * n + p] = (int32_t)q;
+            }
+            count += (uint32_t)__popcll(m);
+            if (count >= cap && !K.count_overflow) goto done;
+          }
+        }
+  done:
+    if (lane == 0) {
+      nbr_count[p] = (int32_t)(count < cap ? count : cap);
+      if (count > cap && K.count_overflow) atomicAdd(&counters[C_OVERFLOW], (unsigned long long)(count - cap));
+    }
+  }
 }
 
 void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
@@ -305,7 +389,12 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
   KernelTimer* t = timer_begin(c, "fill_neighbors");
   hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)fb,
                      (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
-                     c->K, c->nbr, c->nbr_count, c->counters);
+                     c->K, c->nbr, c->nbr_count, c->heavy, c->counters);
+  // enough waves to cover the chip; each loops over the heavy list (usually empty)
+  const uint32_t blocks = std::min<uint32_t>(nblocks((uint64_t)n * 64), 2048u);
+  hipLaunchKernelGGL(k_fill_heavy, dim3(blocks), dim3(kBlock), 0, c->stream, n, (const float4*)c->xs,
+                     (const uint32_t*)c->cell_start, c->K, c->nbr, c->nbr_count, (const uint32_t*)c->heavy,
+                     c->counters);
   timer_end(c, t);
 }
 
@@ -379,12 +468,35 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
   }
 }
 
+uint64_t radix_hist_size(uint64_t n) { return 256ull * ((n + kRadixTile - 1) / kRadixTile); }
+
+// Stable LSD sort of (keys, vals) over key_bits bits, ping-ponging between
+// (k0, v0) and (k1, v1).  Returns 0 or 1: the buffer pair holding the result.
+int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint64_t n, uint32_t key_bits,
+                     uint32_t* hist, uint32_t* offs, uint32_t* partials, hipStream_t s) {
+  if (n <= 1) return 0;
+  const uint32_t ntiles = (uint32_t)((n + kRadixTile - 1) / kRadixTile);
+  const uint64_t hn = 256ull * ntiles;
+  uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+  int which = 0;
+  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t*)ki, n, (int)shift, ntiles,
+                       hist);
+    launch_exclusive_scan(hist, offs, hn, partials, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t*)ki,
+                       (const uint32_t*)vi, n, (int)shift, ntiles, (const uint32_t*)offs, ko, vo);
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+    which ^= 1;
+  }
+  return which;
+}
+
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits, hipStream_t s) {
   if (n <= 1) return PBF_OK;
   if (n > 0xFFFFFFFFull) return set_error(PBF_ERR_UNSUPPORTED, "sort_by_key: n must be < 2^32");
   if (key_bits == 0 || key_bits > 32) key_bits = 32;
-  const uint32_t ntiles = (uint32_t)((n + kRadixTile - 1) / kRadixTile);
-  const uint64_t hn = 256ull * ntiles;
+  const uint64_t hn = radix_hist_size(n);
   uint32_t *k2 = nullptr, *v2 = nullptr, *hist = nullptr, *offs = nullptr, *part = nullptr;
   if (check_hip(hipMallocAsync((void**)&k2, n * 4, s), "hipMallocAsync") ||
       check_hip(hipMallocAsync((void**)&v2, n * 4, s), "hipMallocAsync") ||
@@ -392,20 +504,9 @@ int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits
       check_hip(hipMallocAsync((void**)&offs, (hn + 1) * 4, s), "hipMallocAsync") ||
       check_hip(hipMallocAsync((void**)&part, scan_partials_size(hn) * 4, s), "hipMallocAsync"))
     return PBF_ERR_HIP;
-  uint32_t *ki = keys, *vi = vals, *ko = k2, *vo = v2;
-  int passes = 0;
-  for (uint32_t shift = 0; shift < key_bits; shift += 8, ++passes) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t*)ki, n, (int)shift, ntiles,
-                       hist);
-    launch_exclusive_scan(hist, offs, hn, part, s);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, (const uint32_t*)ki,
-                       (const uint32_t*)vi, n, (int)shift, ntiles, (const uint32_t*)offs, ko, vo);
-    std::swap(ki, ko);
-    std::swap(vi, vo);
-  }
-  if (passes & 1) {
-    hipMemcpyAsync(keys, ki, n * 4, hipMemcpyDeviceToDevice, s);
-    hipMemcpyAsync(vals, vi, n * 4, hipMemcpyDeviceToDevice, s);
+  if (radix_sort_pairs(keys, vals, k2, v2, n, key_bits, hist, offs, part, s)) {
+    hipMemcpyAsync(keys, k2, n * 4, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(vals, v2, n * 4, hipMemcpyDeviceToDevice, s);
   }
   hipFreeAsync(k2, s);
   hipFreeAsync(v2, s);

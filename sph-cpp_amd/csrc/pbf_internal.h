@@ -9,11 +9,17 @@
 //   rho, lam         float[n]
 //   oid, oida        uint32[n]   original index of each sorted slot
 //   key, keya        uint32[n]   cell key of each slot
-//   slot, tmp, tmpo  uint32[n]   counting-sort work
-//   cell_count       uint32[C+1] particles per cell (+1 bucket for keys outside the grid)
-//   cell_start       uint32[C+2] exclusive scan of cell_count
-//   nbr              int32[K][n] neighbour lists, column major: nbr[k*n + p]
+//   slot             uint32[n]   inverse permutation (original index -> slot)
+//   sk[2], sv[2]     uint32[n]   radix-sort ping-pong buffers (cell key, slot)
+//   rhist, roffs     uint32[256 * tiles]  radix digit histograms and their scan
+//   cell_start       uint32[C+2] first sorted slot of every cell (C = the outside bucket)
+//   heavy            uint32[n]   particles whose 27 cells hold too many candidates
+//                                for one lane (built by a whole wavefront instead)
+//   nbr              int32[K][n] neighbour lists, column major: nbr[k*n + p] (entry k of
+//                                a wavefront's 64 particles is one contiguous 256-byte read)
 //   nbr_count        int32[n]
+//   cnb, cnb_count   int32[K][n], int32[n]  compact lists of the entries within h, column
+//                                major (built by the density and first XSPH passes)
 //   counters         u64[8]      overflow, outside, hacks, hack flag, e_max bits, ...
 #pragma once
 
@@ -45,14 +51,17 @@ struct KParams {
   uint32_t ncells;
   uint64_t dxy;
   int cap;
+  int stride;                    // list row length: cap rounded up to a multiple of 8
+  int count_overflow;            // PBF_OPT_COUNT_OVERFLOW
 };
 
 enum Counter {
   C_OVERFLOW = 0,
   C_OUTSIDE = 1,
   C_HACKS = 2,
-  C_HACK_FLAG = 3,
-  C_EMAX = 4,  // float bits of the max density error (atomicMax on non-negative floats)
+  C_HACK_FLAG = 3,  // two flags (3, 4) used on alternate Jacobi iterations
+  C_HEAVY = 5,      // number of entries in the heavy list
+  C_EMAX = 6,       // float bits of the max density error (atomicMax on non-negative floats)
   C_NUM = 8
 };
 
@@ -85,9 +94,14 @@ struct pbf_context {
   float4* scra = nullptr;
   uint32_t *oid = nullptr, *oida = nullptr, *key = nullptr, *keya = nullptr;
   uint32_t *slot = nullptr, *tmp = nullptr, *tmpo = nullptr;
-  uint32_t *cell_count = nullptr, *cell_start = nullptr;
+  uint32_t *sk[2] = {nullptr, nullptr}, *sv[2] = {nullptr, nullptr};
+  uint32_t *rhist = nullptr, *roffs = nullptr, *rpart = nullptr;
+  uint32_t* perm = nullptr;  // sorted slot -> previous slot (points into sv or tmpo)
+  uint32_t* heavy = nullptr;
+  uint32_t* cell_start = nullptr;
   uint32_t* scan_partials = nullptr;
   int32_t *nbr = nullptr, *nbr_count = nullptr;
+  int32_t *cnb = nullptr, *cnb_count = nullptr;
   unsigned long long* counters = nullptr;
   float* stage = nullptr;  // device staging for original-order transfers (4 floats / particle)
   uint64_t* stage64 = nullptr;
@@ -112,9 +126,12 @@ void launch_fill_u64(unsigned long long* p, unsigned long long value, uint64_t n
 // exclusive scan of in[0..n) into out[0..n]; out[n] = total.  partials needs scan_partials_size(n).
 uint64_t scan_partials_size(uint64_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partials, hipStream_t s);
-void launch_bin(pbf_context* c, uint64_t bin_end);  // key, slot, cell_count (after fill)
-void launch_scatter_rank(pbf_context* c);   // tmp -> perm (written to slot), reorder
-void launch_reorder(pbf_context* c);
+void launch_bin(pbf_context* c, uint64_t bin_end);  // key, inverse permutation, keys by original index
+void launch_sort(pbf_context* c);            // radix sort -> perm, cell_start
+void launch_reorder(pbf_context* c);         // gather state by perm
+uint64_t radix_hist_size(uint64_t n);
+int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint64_t n, uint32_t key_bits,
+                     uint32_t* hist, uint32_t* offs, uint32_t* partials, hipStream_t s);
 void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe);
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits, hipStream_t s);
 void launch_bounds_counting(const uint32_t* sorted, uint64_t n, uint64_t b, uint64_t e, uint32_t* out,
@@ -125,6 +142,14 @@ void launch_bounds_counting(const uint32_t* sorted, uint64_t n, uint64_t b, uint
 enum { F_FORCES = 1, F_PREDICT = 2 };
 void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e);
 void launch_density_lambda(pbf_context* c, int with_lambda, uint64_t b, uint64_t e);  // xs -> xt, rho, lam
+// fused-step kernels that build and use the within-h masks
+void launch_density_lambda_masked(pbf_context* c, int it);   // xs -> xt, rho, lam, mask
+void launch_lambda_repair(pbf_context* c, int it);            // lam from xt if the iteration's flag is set
+void launch_dp_update_masked(pbf_context* c, int it, int last);
+void launch_viscosity_first(pbf_context* c);                  // v -> va, mask for the final positions
+void launch_vorticity_masked(pbf_context* c);                 // va -> scr
+void launch_apply_vorticity_masked(pbf_context* c);           // va += ...
+void launch_viscosity_last(pbf_context* c);                   // va -> v, x = x*
 void launch_lambda(pbf_context* c, const float4* pos, int only_if_flag, uint64_t b, uint64_t e);
 void launch_dp(pbf_context* c, uint64_t b, uint64_t e);                     // xs, lam -> scr
 void launch_dp_update(pbf_context* c, int last, uint64_t b, uint64_t e);    // xt, lam -> xs (+ v)

@@ -27,7 +27,7 @@ JITTER = 0.01 * H   # uniform +/- 0.01 h
 
 def pbf_params(box, capacity, iters=ITERS, cap=60, h=H, spacing=SPACING, dt=DT, gamma=0.0,
                visc=0.01, vort=0.0005, eps=600.0, rest_density=REST_DENSITY, gravity=-9.8,
-               mover=(-1.5, -2.0, -1.5)) -> "pbf.Params":
+               mover=(-1.5, -2.0, -1.5), count_overflow=False) -> "pbf.Params":
     p = pbf.Params()
     p.max_particles_local = int(capacity)
     p.solve_step_count = int(iters)
@@ -42,6 +42,7 @@ def pbf_params(box, capacity, iters=ITERS, cap=60, h=H, spacing=SPACING, dt=DT, 
     p.visc_c = visc
     p.vorticity_coef = vort
     p.k_stiff = 1.0
+    p.options = pbf.PBF_OPT_COUNT_OVERFLOW if count_overflow else 0
     for i in range(3):
         p.boundary_min[i] = 0.0
         p.boundary_max[i] = float(box[i])

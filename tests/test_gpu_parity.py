@@ -209,7 +209,7 @@ def test_fused_step_equals_per_pass(pbf, scenes):
 
 def test_c1_one_and_hundred_substeps_bit_exact(pbf, scenes):
     """32,768-particle dam break: 1 and 100 substeps, GPU == oracle (Jacobi) bit for bit."""
-    p, pos, vel = scenes.config_c1(seed=0)
+    p, pos, vel = scenes.config_c1(seed=0, count_overflow=True)
     g, o = _pair(pbf, p, pos, vel)
     g.step(1)
     o.step(1)
@@ -268,3 +268,45 @@ def test_bound_primitives(pbf):
     vals = np.arange(520)
     assert np.array_equal(lo, np.searchsorted(keys, vals, side="left"))
     assert np.array_equal(hi, np.searchsorted(keys, vals, side="right"))
+
+
+def test_dense_cluster_heavy_lists_bit_exact(pbf, scenes):
+    """Thousands of particles inside a few cells (the collapsed state of a saturated
+    list): the wavefront list builder must give the reference's lists, order and
+    overflow count, and a whole step must stay bit-exact."""
+    rng = np.random.default_rng(21)
+    dense = rng.uniform(0.30, 0.40, size=(3000, 3)).astype(np.float32)
+    sparse = scenes.lattice((8, 8, 8), origin=(0.5, 0.05, 0.05))
+    pos = np.concatenate([dense, sparse]).astype(np.float32)
+    for count_overflow in (True, False):
+        p = scenes.pbf_params((1.0, 1.0, 1.0), pos.shape[0], count_overflow=count_overflow)
+        g, o = _pair(pbf, p, pos, np.zeros_like(pos))
+        g.find_neighbors()
+        o.pass_(O.FIND)
+        nb = o.neighbors()
+        lists, counts = g.neighbor_lists()
+        assert np.array_equal(counts, nb["counts"])
+        mask = np.arange(p.max_neighbors)[None, :] < counts[:, None]
+        assert np.array_equal(np.where(mask, lists, 0), np.where(mask, nb["lists"], 0))
+        if count_overflow:
+            assert g.stats()["neighbor_overflow"] == o.counters()["overflow"] > 0
+    g.step(1)
+    o.step(1)
+    _compare_state(g, o, "dense cluster step")
+
+
+def test_partial_bin_span(pbf, scenes):
+    """find_neighbors with a bin span shorter than the particle count (halo-free
+    reference usage bins [0, count), neighbors.h:241-248)."""
+    p, pos, vel = _small_scene(scenes, seed=23)
+    g, o = _pair(pbf, p, pos, vel)
+    half = pos.shape[0] // 2
+    g.find_neighbors(pbf.span(0, half), pbf.span(0, pos.shape[0]))
+    o.pass_(O.FIND, 0, half, 0, pos.shape[0])
+    nb = o.neighbors()
+    assert np.array_equal(g.bin_ids()[:half], nb["keys"][:half])
+    assert np.array_equal(g.particle_ids()[:half], nb["ids"][:half])
+    b, e = g.bin_bounds()
+    assert np.array_equal(b, nb["begin"]) and np.array_equal(e, nb["end"])
+    lists, counts = g.neighbor_lists()
+    assert np.array_equal(counts, nb["counts"])

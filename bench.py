@@ -94,7 +94,7 @@ def _profile_traffic(kernel: str):
         return None
 
 
-KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "fill_neighbors", "density_lambda",
+KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "fill_neighbors", "density_lambda", "mark_dirty",
            "lambda_repair", "dp_update", "viscosity", "vorticity", "apply_vorticity", "viscosity_positions"]
 
 

@@ -175,7 +175,7 @@ static void free_all(pbf_context* c) {
                   c->scra, c->rho,  c->lam,  c->rhoa, c->lama, c->oid,  c->oida,       c->key,
                   c->keya, c->slot, c->tmp,  c->tmpo, c->nbr,  c->nbr_count, c->counters, c->stage,
                   c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
-                  c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count};
+                  c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -272,6 +272,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->roffs, radix_hist_size(N) + 1)) ||
       (rc = dalloc(&c->rpart, scan_partials_size(radix_hist_size(N) + 1))) || (rc = dalloc(&c->heavy, N)) ||
       (rc = dalloc(&c->cnb, N * (uint64_t)c->K.stride)) || (rc = dalloc(&c->cnb_count, N)) ||
+      (rc = dalloc(&c->moved, N)) || (rc = dalloc(&c->dirty, N)) ||
       (rc = dalloc(&c->scan_partials, scan_partials_size(N + 1))) ||
       (rc = check_hip(hipEventCreate(&c->region_a), "hipEventCreate")) ||
       (rc = check_hip(hipEventCreate(&c->region_b), "hipEventCreate"))) {
@@ -282,6 +283,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   }
   launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
   launch_fill_u32((uint32_t*)c->nbr_count, 0u, N, c->stream);
+  launch_fill_u32(c->dirty, 0u, N, c->stream);
   *out = c;
   return check_hip(hipStreamSynchronize(c->stream), "pbf_create");
 }
@@ -531,8 +533,10 @@ int pbf_step(pbf_context* c, uint32_t nsteps) {
     find_neighbors_impl(c, n, 0, n);
     const uint32_t I = c->P.solve_step_count;
     for (uint32_t it = 0; it < I; ++it) {
-      launch_density_lambda_masked(c, (int)it);          // xs -> xt, rho, lam, mask
-      launch_lambda_repair(c, (int)it);                  // only if the density fix fired
+      ++c->dirty_stamp;
+      launch_density_lambda_masked(c, (int)it);          // xs -> xt, rho, lam, compact lists
+      launch_mark_dirty(c, (int)it);                     // only if the density fix fired
+      launch_lambda_repair(c, (int)it);
       launch_dp_update_masked(c, (int)it, it + 1 == I);  // xt, lam -> xs (+ v)
     }
     if (I == 0) launch_update_velocities(c, 0, n);

@@ -48,6 +48,7 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
                                                              const int32_t* __restrict__ nbr_count,
                                                              int32_t* __restrict__ cnb,
                                                              int32_t* __restrict__ cnb_count,
+                                                             uint32_t* __restrict__ moved,
                                                              unsigned long long* __restrict__ counters) {
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -125,21 +126,68 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
   lam[p] = -C_p / (sum_C + K.eps);
   if (hacks) {
     atomicAdd(&counters[C_HACKS], (unsigned long long)hacks);
+    moved[atomicAdd(&counters[C_MOVED_N + (it & 1)], 1ull)] = p;
     atomicExch(&counters[C_HACK_FLAG + (it & 1)], 1ull);
   }
 }
 
-// compute_pressure_lambdas (particles.h:318-342) from xt for every particle,
-// only when the iteration's density pass moved a particle (plain walks).
-__global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, KParams K,
+// A particle is dirty when it or a particle in its list was moved by the
+// density fix: its fused lambda and its compact list may be stale.  A list
+// walk visits cells computed from floating-point coordinates (x + i*s,
+// neighbors.h:177-179), which lie within two cells of the particle's own cell
+// on every axis, so every particle that can list q sits within two cells of
+// q's cell: all particles of those 5x5x5 cells are marked.  Many moved
+// particles mark everybody instead.
+constexpr uint32_t kMaxMovedMarked = 4096;
+constexpr uint32_t kMaxMarkedPerMoved = 8192;
+
+__global__ void k_mark_dirty(int it, uint32_t stamp, KParams K, const uint32_t* __restrict__ key,
+                             const uint32_t* __restrict__ start, const uint32_t* __restrict__ moved,
+                             uint32_t* __restrict__ dirty, unsigned long long* __restrict__ counters) {
+  const uint32_t nmoved = (uint32_t)counters[C_MOVED_N + (it & 1)];
+  if (nmoved > kMaxMovedMarked) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_MOVED_ALL + (it & 1)] = 1ull;
+    return;
+  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nmoved; i += gridDim.x * blockDim.x) {
+    const uint32_t kq = key[moved[i]];
+    if (kq >= K.ncells) {  // outside the grid: no cell to start from
+      counters[C_MOVED_ALL + (it & 1)] = 1ull;
+      continue;
+    }
+    const int cz = (int)(kq / K.dxy), cy = (int)((kq % K.dxy) / K.dimx), cx = (int)(kq % K.dimx);
+    uint32_t total = 0;
+    for (int z = max(cz - 2, 0); z <= min(cz + 2, (int)K.dimz - 1); ++z)
+      for (int y = max(cy - 2, 0); y <= min(cy + 2, (int)K.dimy - 1); ++y) {
+        const uint64_t row = (uint64_t)z * K.dxy + (uint64_t)y * K.dimx;
+        total += start[row + min(cx + 2, (int)K.dimx - 1) + 1] - start[row + max(cx - 2, 0)];
+      }
+    if (total > kMaxMarkedPerMoved) {  // crowded neighbourhood: cheaper to treat everybody as dirty
+      counters[C_MOVED_ALL + (it & 1)] = 1ull;
+      continue;
+    }
+    for (int z = max(cz - 2, 0); z <= min(cz + 2, (int)K.dimz - 1); ++z)
+      for (int y = max(cy - 2, 0); y <= min(cy + 2, (int)K.dimy - 1); ++y) {
+        const uint64_t row = (uint64_t)z * K.dxy + (uint64_t)y * K.dimx;
+        const uint32_t b = start[row + max(cx - 2, 0)], e = start[row + min(cx + 2, (int)K.dimx - 1) + 1];
+        for (uint32_t p = b; p < e; ++p) dirty[p] = stamp;
+      }
+  }
+}
+
+// compute_pressure_lambdas (particles.h:318-342) from xt for the dirty
+// particles of this iteration (plain walks).
+__global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, uint32_t stamp, KParams K,
                                                           const float4* __restrict__ xt,
                                                           const float* __restrict__ rho, float* __restrict__ lam,
                                                           const int32_t* __restrict__ nbr,
                                                           const int32_t* __restrict__ nbr_count,
+                                                          const uint32_t* __restrict__ dirty,
                                                           const unsigned long long* __restrict__ counters) {
   if (counters[C_HACK_FLAG + (it & 1)] == 0ull) return;
+  const bool all = counters[C_MOVED_ALL + (it & 1)] != 0ull;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  if (p >= n || (!all && dirty[p] != stamp)) return;
   const f3 xp = ld3(xt, p);
   float sum_C = 0.0f;
   f3 sg = {0.f, 0.f, 0.f};
@@ -169,7 +217,7 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, KP
 
 // compute_pressure_dps (particles.h:348-358) + update_position_stars
 // (:363-370); on the last iteration also update_velocities (:375-385).
-__global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int last, KParams K,
+__global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, KParams K,
                                                         const float4* __restrict__ xt,
                                                         const float* __restrict__ lam,
                                                         const float4* __restrict__ x, float4* __restrict__ xs,
@@ -177,13 +225,19 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
                                                         const int32_t* __restrict__ nbr_count,
                                                         const int32_t* __restrict__ cnb,
                                                         const int32_t* __restrict__ cnb_count,
+                                                        const uint32_t* __restrict__ dirty,
                                                         unsigned long long* __restrict__ counters) {
-  // clear the next iteration's flag (nobody reads it until the next density pass)
-  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_HACK_FLAG + ((it + 1) & 1)] = 0ull;
+  // clear the next iteration's flags (nobody reads them until the next density pass)
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counters[C_HACK_FLAG + ((it + 1) & 1)] = 0ull;
+    counters[C_MOVED_N + ((it + 1) & 1)] = 0ull;
+    counters[C_MOVED_ALL + ((it + 1) & 1)] = 0ull;
+  }
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  // after the density fix, positions moved: walk the whole list with the test
-  const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull;
+  // a dirty particle (moved by the density fix, or next to one) walks its whole list with the test
+  const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull &&
+                     (counters[C_MOVED_ALL + (it & 1)] != 0ull || dirty[p] == stamp);
   const f3 xp = ld3(xt, p);
   const float lp = lam[p];
   f3 dp = {0.f, 0.f, 0.f};
@@ -376,18 +430,30 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
 void launch_density_lambda_masked(pbf_context* c, int it) {
   PBF_LAUNCH(c, "density_lambda", k_density_lambda_c, it, c->K, (const float4*)c->xs, (const float4*)c->v, c->xt,
              c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb, c->cnb_count,
-             c->counters);
+             c->moved, c->counters);
+}
+
+void launch_mark_dirty(pbf_context* c, int it) {
+  const uint32_t n = (uint32_t)c->n;
+  if (!n) return;
+  KernelTimer* t = timer_begin(c, "mark_dirty");
+  hipLaunchKernelGGL(k_mark_dirty, dim3(16), dim3(kBlock), 0, c->stream, it, c->dirty_stamp, c->K,
+                     (const uint32_t*)c->key, (const uint32_t*)c->cell_start, (const uint32_t*)c->moved, c->dirty,
+                     c->counters);
+  timer_end(c, t);
 }
 
 void launch_lambda_repair(pbf_context* c, int it) {
-  PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->K, (const float4*)c->xt, (const float*)c->rho, c->lam,
-             (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, (const unsigned long long*)c->counters);
+  PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->dirty_stamp, c->K, (const float4*)c->xt,
+             (const float*)c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
+             (const uint32_t*)c->dirty, (const unsigned long long*)c->counters);
 }
 
 void launch_dp_update_masked(pbf_context* c, int it, int last) {
-  PBF_LAUNCH(c, "dp_update", k_dp_update_c, it, last, c->K, (const float4*)c->xt, (const float*)c->lam,
-             (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
-             (const int32_t*)c->cnb, (const int32_t*)c->cnb_count, c->counters);
+  PBF_LAUNCH(c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, c->K, (const float4*)c->xt,
+             (const float*)c->lam, (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
+             (const int32_t*)c->nbr_count, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count,
+             (const uint32_t*)c->dirty, c->counters);
 }
 
 void launch_viscosity_first(pbf_context* c) {

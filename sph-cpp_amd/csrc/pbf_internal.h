@@ -62,7 +62,9 @@ enum Counter {
   C_HACK_FLAG = 3,  // two flags (3, 4) used on alternate Jacobi iterations
   C_HEAVY = 5,      // number of entries in the heavy list
   C_EMAX = 6,       // float bits of the max density error (atomicMax on non-negative floats)
-  C_NUM = 8
+  C_MOVED_N = 7,    // two counts (7, 8): particles moved by the density fix, per iteration parity
+  C_MOVED_ALL = 9,  // two flags (9, 10): a moved particle had a full list -> treat all as dirty
+  C_NUM = 16
 };
 
 struct KernelTimer {
@@ -86,6 +88,7 @@ struct pbf_context {
   int cap = 60;
   bool lists_valid = false;
   uint32_t last_iterations = 0;
+  uint32_t dirty_stamp = 0;  // incremented per Jacobi iteration of the fused step
 
   float4 *x = nullptr, *xs = nullptr, *v = nullptr;
   float4 *xa = nullptr, *xsa = nullptr, *va = nullptr;
@@ -98,6 +101,8 @@ struct pbf_context {
   uint32_t *rhist = nullptr, *roffs = nullptr, *rpart = nullptr;
   uint32_t* perm = nullptr;  // sorted slot -> previous slot (points into sv or tmpo)
   uint32_t* heavy = nullptr;
+  uint32_t* moved = nullptr;  // particles moved by the density fix in the current iteration
+  uint32_t* dirty = nullptr;  // == dirty_stamp: the particle or one of its neighbours moved
   uint32_t* cell_start = nullptr;
   uint32_t* scan_partials = nullptr;
   int32_t *nbr = nullptr, *nbr_count = nullptr;
@@ -144,7 +149,8 @@ void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e);
 void launch_density_lambda(pbf_context* c, int with_lambda, uint64_t b, uint64_t e);  // xs -> xt, rho, lam
 // fused-step kernels that build and use the within-h masks
 void launch_density_lambda_masked(pbf_context* c, int it);   // xs -> xt, rho, lam, mask
-void launch_lambda_repair(pbf_context* c, int it);            // lam from xt if the iteration's flag is set
+void launch_mark_dirty(pbf_context* c, int it);               // particles next to a moved one
+void launch_lambda_repair(pbf_context* c, int it);            // lam from xt for dirty particles
 void launch_dp_update_masked(pbf_context* c, int it, int last);
 void launch_viscosity_first(pbf_context* c);                  // v -> va, mask for the final positions
 void launch_vorticity_masked(pbf_context* c);                 // va -> scr

@@ -137,7 +137,18 @@ def main(argv=None):
     ap.add_argument("--kernel", default="", help="kernel for the roofline (default: the slowest in the timed run)")
     ap.add_argument("--cap", type=int, default=60, help="neighbour list cap (MAX_NEIGHBORS = 60)")
     ap.add_argument("--no-variants", dest="variants", action="store_false", help="skip the cap-128 variant")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB mebibytes device to device 5 times (PMC byte-counter calibration)")
     args = ap.parse_args(argv)
+
+    if args.calibrate_copy:
+        pbf = _load("sph-cpp_amd")
+        nbytes = args.calibrate_copy << 20
+        a, b = pbf.DeviceBuffer(nbytes), pbf.DeviceBuffer(nbytes)
+        for _ in range(5):
+            pbf._check(pbf.lib().pbf_memcpy_device(b.ptr, a.ptr, nbytes))
+        print(json.dumps({"copy_bytes": nbytes, "copies": 5}), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the single-domain PBF step has no multi-GPU path in this build "

@@ -53,7 +53,7 @@ typedef struct pbf_index_span {
 typedef struct pbf_params {
   uint64_t max_particles_local;  /* capacity (parameters.h:550) */
   uint32_t solve_step_count;     /* Jacobi iterations per step (parameters.h:552) */
-  uint32_t max_neighbors;        /* list cap, MAX_NEIGHBORS = 60 (neighbors.h:39); 1..64 */
+  uint32_t max_neighbors;        /* list cap, MAX_NEIGHBORS = 60 (neighbors.h:39); 1..128 */
   float particle_rest_spacing;
   float particle_radius;
   float smoothing_radius;        /* h */
@@ -195,6 +195,15 @@ int pbf_device_alloc(uint64_t bytes, void** out);
 int pbf_device_free(void* ptr);
 int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes);
 int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes);
+int pbf_memcpy_device(void* dst, const void* src, uint64_t bytes);  /* device to device */
+
+/* ---- output ----
+ * Append one positions record to `path`, with the content AdiosWriter::write_particles
+ * writes per call (adios_writer.h:78-114, adios_writer.xml:4-14): int64 global_bytes,
+ * int64 local_bytes, int64 offset_bytes, then local_bytes of positions (Vec<float,3>, original
+ * order).  ADIOS itself is not used; the file is these records back to back (one domain:
+ * global = local, offset = 0). */
+int pbf_append_positions_record(pbf_context* ctx, const char* path);
 
 /* ---- measurement ----
  * Bracket every launch of the kernels named in `names` (comma separated,

@@ -499,6 +499,78 @@ struct Sim {
     return m;
   }
 
+  // max density error as a float reduction (the device keeps it in R)
+  R emax_r(size_t b, size_t e) const {
+    R m = (R)0;
+    for (size_t p = b; p < e; ++p) {
+      const R c = rho[p] / rho0 - static_cast<R>(1.0);
+      if (c > m) m = c;
+    }
+    return m;
+  }
+
+  // Iterate-to-tolerance step (no reference counterpart: the reference runs
+  // a fixed solve_step_count of Jacobi iterations, Simulation/Source/main.cpp:82).
+  // Iteration slot k: density pass, e_k = max density error; stop if
+  // e_k <= tol, else lambdas, dps, position stars.  Warm start (warm != NULL,
+  // one multiplier per particle): a density pass first records e_pre, then
+  // one dp pass with lam = warm moves x*; if slot 0 then measures e_0 > e_pre,
+  // x* returns to its value after that first density pass and slot 0 makes no
+  // update (the fallback to a zero warm start).  lam_sum (optional) receives
+  // the sum of the multipliers of every update that was applied, the warm
+  // one included.  Returns the number of slots used (<= max_iter).
+  int solve(R tol, int max_iter, const R* warm, R* hist, R* lam_sum, int* kept) {
+    apply_external_forces(0, n);
+    predict_positions(0, n);
+    find_neighbors(0, n, 0, n);
+    std::vector<R> acc(n, (R)0);
+    std::vector<V3<R>> saved;
+    R e_pre = (R)0;
+    int keep = 0;
+    if (warm) {
+      compute_densities(0, n);
+      e_pre = emax_r(0, n);
+      saved = xs;
+      lam.assign(warm, warm + n);
+      compute_pressure_dps(0, n);
+      update_position_stars(0, n);
+      keep = 1;
+    }
+    int used = 0;
+    bool done = false;
+    for (int it = 0; it < max_iter && !done; ++it) {
+      compute_densities(0, n);
+      const R e = emax_r(0, n);
+      if (hist) hist[it] = e;
+      used = it + 1;
+      if (warm && it == 0 && e > e_pre) {
+        xs = saved;
+        keep = 0;
+        continue;
+      }
+      if (e <= tol) {
+        done = true;
+        continue;
+      }
+      compute_pressure_lambdas(0, n);
+      compute_pressure_dps(0, n);
+      update_position_stars(0, n);
+      for (size_t p = 0; p < n; ++p) acc[p] += lam[p];
+    }
+    if (warm && keep)
+      for (size_t p = 0; p < n; ++p) acc[p] += warm[p];
+    if (lam_sum) std::copy(acc.begin(), acc.end(), lam_sum);
+    if (kept) *kept = keep;
+    update_velocities(0, n);
+    if (gamma != (R)0) apply_surface_tension(0, n, 0, n);
+    apply_viscosity(0, n);
+    compute_vorticity(0, n);
+    apply_vorticity(0, n);
+    apply_viscosity(0, n);
+    update_positions(0, n);
+    return used;
+  }
+
   // One step of Simulation/Source/main.cpp:71-124 on a single domain
   // (resident = local = [0, n)).  Surface tension runs only for gamma != 0.
   void step() {
@@ -654,6 +726,20 @@ int orc_step(void* h, int nsteps) {
     if (H->dbl) H->d.step(); else H->f.step();
   }
   return 0;
+}
+
+// iterate-to-tolerance step; warm / lam_sum are n floats in particle order
+// (either may be NULL), hist receives max_iter floats (may be NULL)
+int orc_solve(void* h, float tol, int max_iter, const float* warm, float* hist, float* lam_sum, int* kept) {
+  Handle* H = (Handle*)h;
+  if (max_iter < 1) return -1;
+  if (!H->dbl) return H->f.solve(tol, max_iter, warm, hist, lam_sum, kept);
+  std::vector<double> w, hs((size_t)max_iter, 0.0), ls(H->d.n, 0.0);
+  if (warm) w.assign(warm, warm + H->d.n);
+  const int used = H->d.solve(tol, max_iter, warm ? w.data() : nullptr, hs.data(), ls.data(), kept);
+  for (int i = 0; hist && i < max_iter; ++i) hist[i] = (float)hs[(size_t)i];
+  for (size_t i = 0; lam_sum && i < H->d.n; ++i) lam_sum[i] = (float)ls[i];
+  return used;
 }
 
 void orc_dims(void* h, uint64_t* dims) {

@@ -149,6 +149,8 @@ def lib():
     _sig(L, "pbf_device_free", [_P])
     _sig(L, "pbf_memcpy_to_device", [_P, _P, C.c_uint64])
     _sig(L, "pbf_memcpy_to_host", [_P, _P, C.c_uint64])
+    _sig(L, "pbf_memcpy_device", [_P, _P, C.c_uint64])
+    _sig(L, "pbf_append_positions_record", [_P, C.c_char_p])
     _sig(L, "pbf_profile_kernels", [_P, C.c_char_p])
     _sig(L, "pbf_kernel_time", [_P, C.c_char_p, _U64, C.POINTER(C.c_double)])
     _sig(L, "pbf_region_begin", [_P])
@@ -362,6 +364,10 @@ class Particles:
         counts = np.zeros(n, dtype=np.int32)
         _check(lib().pbf_get_neighbor_lists(self._ctx, lists.ctypes.data_as(_U64), counts.ctypes.data_as(_I32)))
         return lists, counts
+
+    def append_positions_record(self, path: str) -> None:
+        """AdiosWriter::write_particles record layout (adios_writer.h:78-114), raw."""
+        _check(lib().pbf_append_positions_record(self._ctx, path.encode()))
 
     # ---- measurement ----
     def profile_kernels(self, names) -> None:

@@ -455,7 +455,7 @@ int pbf_find_neighbors(pbf_context* c, pbf_index_span bin_span, pbf_index_span f
 int pbf_compute_densities(pbf_context* c, pbf_index_span s) {
   PBF_CHECK_CTX_SPAN(c, s);
   if (int rc = need_lists(c)) return rc;
-  launch_density_lambda(c, 0, s.begin, s.end);
+  launch_density(c, s.begin, s.end);
   std::swap(c->xs, c->xt);  // the pass's x* (with the near-coincidence fix) becomes current
   return check_hip(hipGetLastError(), "compute_densities");
 }
@@ -463,7 +463,7 @@ int pbf_compute_densities(pbf_context* c, pbf_index_span s) {
 int pbf_compute_pressure_lambdas(pbf_context* c, pbf_index_span s) {
   PBF_CHECK_CTX_SPAN(c, s);
   if (int rc = need_lists(c)) return rc;
-  launch_lambda(c, c->xs, 0, s.begin, s.end);
+  launch_lambda(c, c->xs, s.begin, s.end);
   return check_hip(hipGetLastError(), "compute_pressure_lambdas");
 }
 
@@ -675,6 +675,9 @@ int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes) {
 int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes) {
   return check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
 }
+int pbf_memcpy_device(void* dst, const void* src, uint64_t bytes) {
+  return check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");
+}
 
 // ---- measurement ----
 int pbf_profile_kernels(pbf_context* c, const char* names) {
@@ -741,3 +744,17 @@ static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, u
 }
 
 }  // namespace pbf
+
+extern "C" int pbf_append_positions_record(pbf_context* c, const char* path) {
+  if (!c || !path) return set_error(PBF_ERR_ARG, "null argument");
+  std::vector<float> pos(c->n * 3);
+  if (int rc = pbf_get_positions(c, pos.data())) return rc;
+  FILE* f = std::fopen(path, "ab");
+  if (!f) return set_error(PBF_ERR_ARG, std::string("cannot open ") + path);
+  const int64_t local_bytes = (int64_t)(c->n * 3 * sizeof(float));
+  const int64_t header[3] = {local_bytes, local_bytes, 0};  // global, local, offset (one domain)
+  bool ok = std::fwrite(header, sizeof(header), 1, f) == 1;
+  if (ok && local_bytes) ok = std::fwrite(pos.data(), (size_t)local_bytes, 1, f) == 1;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? PBF_OK : set_error(PBF_ERR_ARG, std::string("write failed: ") + path);
+}

@@ -37,10 +37,27 @@ __device__ __forceinline__ void load_batch(const int32_t* __restrict__ list, uin
   for (int u = 0; u < kBatch; ++u) q[u] = (k0 + u < cnt) ? (uint32_t)list[(uint64_t)(k0 + u) * n + p] : p;
 }
 
+// The passes that build compact lists run one wavefront per block and stage
+// each lane's entries in its own LDS column (entry k of the 64 lanes
+// contiguous): phase 2 reads them from LDS, and the global copy is written at
+// the end with entry k of the 64 lanes in one contiguous store.
+constexpr int kStageBlock = 64;
+
+__device__ __forceinline__ void load_batch_lds(const int32_t* __restrict__ s_list, uint32_t lane, int k0, int cnt,
+                                               uint32_t p, uint32_t (&q)[kBatch]) {
+#pragma unroll
+  for (int u = 0; u < kBatch; ++u) q[u] = (k0 + u < cnt) ? (uint32_t)s_list[(k0 + u) * kStageBlock + lane] : p;
+}
+
+__device__ __forceinline__ void flush_compact(const int32_t* __restrict__ s_list, uint32_t lane, int nc,
+                                              int32_t* __restrict__ cnb, uint32_t n, uint32_t p) {
+  for (int k = 0; k < nc; ++k) cnb[(uint64_t)k * n + p] = s_list[k * kStageBlock + lane];
+}
+
 // ---------------------------------------------------------------------------
 // compute_densities (particles.h:292-313) + compute_pressure_lambdas (:318-342)
 // xs -> xt, rho, lam; compact row of the entries within h.
-__global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it, KParams K,
+__global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, int it, KParams K,
                                                              const float4* __restrict__ xs,
                                                              const float4* __restrict__ v, float4* __restrict__ xt,
                                                              float* __restrict__ rho, float* __restrict__ lam,
@@ -50,6 +67,8 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
                                                              int32_t* __restrict__ cnb_count,
                                                              uint32_t* __restrict__ moved,
                                                              unsigned long long* __restrict__ counters) {
+  extern __shared__ int32_t s_list[];  // [cap][kStageBlock]
+  const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   f3 xp = ld3(xs, p);
@@ -71,10 +90,11 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
         hack_at = k0 + u;
         continue;
       }
-      cnb[(uint64_t)(nc++) * n + p] = (int32_t)q[u];
+      s_list[(nc++) * kStageBlock + lane] = (int32_t)q[u];
     }
   }
   cnb_count[p] = nc;
+  flush_compact(s_list, lane, nc, cnb, n, p);
   // phase 2: kernel arithmetic over the compact row, in list order
   float density = K.mass_W0;
   float sum_C = 0.0f;
@@ -82,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
   uint32_t hacks = 0;
   for (int k0 = 0; k0 < nc; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(cnb, n, k0, nc, p, q);
+    load_batch_lds(s_list, lane, k0, nc, p, q);
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
@@ -272,14 +292,19 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
   }
 }
 
-// XSPH sum (particles.h:441-444) over a compact row
-__device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ cnb, uint32_t n, int nc, uint32_t p, f3 xp,
-                                       f3 vp, const KParams& K, const float4* __restrict__ xs,
+// XSPH sum (particles.h:441-444) over a compact row, from global memory
+// (FROM_LDS = false: cnb column-major) or from the lane's LDS column
+template <bool FROM_LDS>
+__device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_t n, uint32_t lane, int nc,
+                                       uint32_t p, f3 xp, f3 vp, const KParams& K, const float4* __restrict__ xs,
                                        const float* __restrict__ rho, const float4* __restrict__ vin) {
   f3 dv = {0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < nc; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(cnb, n, k0, nc, p, q);
+    if (FROM_LDS)
+      load_batch_lds(list, lane, k0, nc, p, q);
+    else
+      load_batch(list, n, k0, nc, p, q);
     float4 t[kBatch], vq[kBatch];
     float rq[kBatch];
 #pragma unroll
@@ -301,7 +326,7 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ cnb, uint32_t
 // ---------------------------------------------------------------------------
 // apply_viscosity (particles.h:436-447), first call: walks the whole list and
 // leaves the compact rows of the final positions for the passes that follow.
-__global__ void __launch_bounds__(kBlock) k_viscosity_first(uint32_t n, KParams K, const float4* __restrict__ xs,
+__global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                             const float* __restrict__ rho,
                                                             const float4* __restrict__ vin,
                                                             float4* __restrict__ vout,
@@ -309,6 +334,8 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_first(uint32_t n, KParams 
                                                             const int32_t* __restrict__ nbr_count,
                                                             int32_t* __restrict__ cnb,
                                                             int32_t* __restrict__ cnb_count) {
+  extern __shared__ int32_t s_list[];  // [cap][kStageBlock]
+  const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const f3 xp = ld3(xs, p);
@@ -324,11 +351,12 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_first(uint32_t n, KParams 
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const float d2 = magsq3(sub3(xp, xyz(t[u])));
-      if (k0 + u < cnt && within_h(d2, K)) cnb[(uint64_t)(nc++) * n + p] = (int32_t)q[u];
+      if (k0 + u < cnt && within_h(d2, K)) s_list[(nc++) * kStageBlock + lane] = (int32_t)q[u];
     }
   }
   cnb_count[p] = nc;
-  const f3 dv = xsph_sum(cnb, n, nc, p, xp, vp, K, xs, rho, vin);
+  flush_compact(s_list, lane, nc, cnb, n, p);
+  const f3 dv = xsph_sum<true>(s_list, n, lane, nc, p, xp, vp, K, xs, rho, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
   st3(vout, p, nv);
@@ -344,7 +372,7 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K
   if (p >= n) return;
   const f3 xp = ld3(xs, p);
   const f3 vp = ld3(vin, p);
-  const f3 dv = xsph_sum(cnb, n, cnb_count[p], p, xp, vp, K, xs, rho, vin);
+  const f3 dv = xsph_sum<false>(cnb, n, 0, cnb_count[p], p, xp, vp, K, xs, rho, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
   st3(vout, p, nv);
@@ -427,10 +455,20 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     timer_end((c), t_);                                                                                   \
   } while (0)
 
+#define PBF_LAUNCH_STAGED(c, name, kern, ...)                                                              \
+  do {                                                                                                      \
+    const uint32_t n_ = (uint32_t)(c)->n;                                                                   \
+    if (!n_) return;                                                                                        \
+    KernelTimer* t_ = timer_begin((c), name);                                                               \
+    hipLaunchKernelGGL(kern, dim3((n_ + kStageBlock - 1) / kStageBlock), dim3(kStageBlock),                 \
+                       (uint32_t)(kStageBlock * (c)->K.cap * sizeof(int32_t)), (c)->stream, n_, __VA_ARGS__); \
+    timer_end((c), t_);                                                                                     \
+  } while (0)
+
 void launch_density_lambda_masked(pbf_context* c, int it) {
-  PBF_LAUNCH(c, "density_lambda", k_density_lambda_c, it, c->K, (const float4*)c->xs, (const float4*)c->v, c->xt,
-             c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb, c->cnb_count,
-             c->moved, c->counters);
+  PBF_LAUNCH_STAGED(c, "density_lambda", k_density_lambda_c, it, c->K, (const float4*)c->xs, (const float4*)c->v,
+                    c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb,
+                    c->cnb_count, c->moved, c->counters);
 }
 
 void launch_mark_dirty(pbf_context* c, int it) {
@@ -457,7 +495,7 @@ void launch_dp_update_masked(pbf_context* c, int it, int last) {
 }
 
 void launch_viscosity_first(pbf_context* c) {
-  PBF_LAUNCH(c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
+  PBF_LAUNCH_STAGED(c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
              (const float4*)c->v, c->va, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb,
              c->cnb_count);
 }

@@ -1,4 +1,4 @@
-// pbf_grid.hip -- cell binning, stable counting sort, cell bounds, reorder,
+// pbf_grid.hip -- cell binning, stable radix sort, cell bounds, reorder,
 // neighbour lists and the sort / bound primitives.
 //
 // Reference: Simulation/Source/neighbors.h:108-248 (calculate_bin_id,
@@ -9,10 +9,18 @@
 // Sort.  The reference's sort_by_key is Thrust's stable sort of (cell key,
 // particle index).  Particles here live in cell order from the previous
 // step, so a stable sort of the current slots would order by
-// (key, previous key, index).  Instead: one counting pass (radix = number of
-// cells) gives every cell its range, and each particle's place inside its
-// cell is its rank by original index among the cell's members.  The result
-// is exactly the (key, original index) order the reference produces.
+// (key, previous key, index).  Instead the (key, slot) pairs are listed in
+// original-index order (through the inverse permutation written by the
+// binning pass) and sorted by a stable LSD radix sort on the key bits (8-bit
+// digits, per-tile histograms, wavefront ballot ranking in LDS): the result is
+// exactly the (key, original index) order the reference produces.
+//
+// Lists.  One lane per particle walks the 27 cells in the reference's order
+// and stages its entries in LDS, then writes them column-major (entry k of
+// every particle contiguous).  A particle whose candidate count passes a
+// budget is handed to a list of heavy particles that one wavefront each
+// finishes with ballot-compacted appends, so that collapsed cells do not
+// serialise a wavefront on one lane.
 #include "pbf_device.h"
 
 namespace pbf {
@@ -273,14 +281,22 @@ __device__ __forceinline__ WalkCells walk_cells(f3 ps, float s) {
   return w;
 }
 
-__global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t fb, uint32_t fe,
-                                                           const float4* __restrict__ xs,
-                                                           const uint32_t* __restrict__ oid,
-                                                           const uint32_t* __restrict__ start, KParams K,
-                                                           int32_t* __restrict__ nbr,
-                                                           int32_t* __restrict__ nbr_count,
-                                                           uint32_t* __restrict__ heavy,
-                                                           unsigned long long* __restrict__ counters) {
+// One wavefront per block; each lane stages its entries in its own LDS
+// column (entry k of the 64 lanes is contiguous) and writes them out at the
+// end, so that every global store of entry k covers the wavefront's 64
+// consecutive slots instead of 64 scattered 4-byte writes.
+constexpr int kFillBlock = 64;
+
+__global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint32_t fb, uint32_t fe,
+                                                               const float4* __restrict__ xs,
+                                                               const uint32_t* __restrict__ oid,
+                                                               const uint32_t* __restrict__ start, KParams K,
+                                                               int32_t* __restrict__ nbr,
+                                                               int32_t* __restrict__ nbr_count,
+                                                               uint32_t* __restrict__ heavy,
+                                                               unsigned long long* __restrict__ counters) {
+  extern __shared__ int32_t s_list[];  // [cap][kFillBlock]
+  const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint32_t o = oid[p];
@@ -290,17 +306,24 @@ __global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t 
   }
   const f3 ps = ld3(xs, p);
   const WalkCells w = walk_cells(ps, K.s);
-  uint32_t total = 0;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      for (int k = 0; k < 3; ++k) {
-        const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], K);
-        if (cell < K.ncells) total += start[cell + 1] - start[cell];
-      }
-  if (total > kHeavyCandidates) {
-    heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
-    return;
+  if (K.count_overflow) {
+    // every candidate must be seen: route by the total number of candidates
+    uint32_t total = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], K);
+          if (cell < K.ncells) total += start[cell + 1] - start[cell];
+        }
+    if (total > kHeavyCandidates) {
+      heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
+      return;
+    }
   }
+  // otherwise the walk ends when the list is full; a lane that has scanned
+  // kHeavyCandidates without filling its list hands the particle over (the
+  // wavefront builder starts again and writes the same entries)
+  uint32_t scanned = 0;
   int count = 0;
   uint32_t ovf = 0;
   const int cap = K.cap;
@@ -312,6 +335,11 @@ __global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t 
         const uint32_t b = start[cell], e = start[cell + 1];
         // candidates of a cell are contiguous: load 8 at a time, test in order
         for (uint32_t q0 = b; q0 < e; q0 += 8) {
+          scanned += 8;
+          if (scanned > kHeavyCandidates && !K.count_overflow) {
+            heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
+            return;
+          }
           float4 t[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) t[u] = xs[min(q0 + (uint32_t)u, e - 1u)];
@@ -322,7 +350,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t 
             const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
             if (d2 < K.ss) {
               if (count < cap) {
-                nbr[(uint64_t)count * n + p] = (int32_t)q;
+                s_list[count * kFillBlock + lane] = (int32_t)q;
                 ++count;
                 // a full list cannot change any more; only the overflow count could
                 if (count == cap && !K.count_overflow) goto done;
@@ -335,6 +363,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_neighbors(uint32_t n, uint32_t 
       }
 done:
   nbr_count[p] = count;
+  for (int k = 0; k < count; ++k) nbr[(uint64_t)k * n + p] = s_list[k * kFillBlock + lane];
   if (ovf) atomicAdd(&counters[C_OVERFLOW], (unsigned long long)ovf);
 }
 
@@ -387,7 +416,8 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
   KernelTimer* t = timer_begin(c, "fill_neighbors");
-  hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)fb,
+  hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n, kFillBlock)), dim3(kFillBlock),
+                     (uint32_t)(kFillBlock * c->K.cap * sizeof(int32_t)), c->stream, n, (uint32_t)fb,
                      (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
                      c->K, c->nbr, c->nbr_count, c->heavy, c->counters);
   // enough waves to cover the chip; each loops over the heavy list (usually empty)

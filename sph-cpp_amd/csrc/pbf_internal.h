@@ -63,7 +63,7 @@ enum Counter {
   C_HEAVY = 5,      // number of entries in the heavy list
   C_EMAX = 6,       // float bits of the max density error (atomicMax on non-negative floats)
   C_MOVED_N = 7,    // two counts (7, 8): particles moved by the density fix, per iteration parity
-  C_MOVED_ALL = 9,  // two flags (9, 10): a moved particle had a full list -> treat all as dirty
+  C_MOVED_ALL = 9,  // two flags (9, 10): too many moved particles or marked cells -> treat all as dirty
   C_NUM = 16
 };
 
@@ -146,19 +146,18 @@ void launch_bounds_counting(const uint32_t* sorted, uint64_t n, uint64_t b, uint
 // flags for the particle-local kernel
 enum { F_FORCES = 1, F_PREDICT = 2 };
 void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e);
-void launch_density_lambda(pbf_context* c, int with_lambda, uint64_t b, uint64_t e);  // xs -> xt, rho, lam
-// fused-step kernels that build and use the within-h masks
-void launch_density_lambda_masked(pbf_context* c, int it);   // xs -> xt, rho, lam, mask
+void launch_density(pbf_context* c, uint64_t b, uint64_t e);  // xs -> xt, rho
+// ---- fused-step launchers (pbf_fused.hip): they build and use the compact lists ----
+void launch_density_lambda_masked(pbf_context* c, int it);   // xs -> xt, rho, lam, compact lists
 void launch_mark_dirty(pbf_context* c, int it);               // particles next to a moved one
 void launch_lambda_repair(pbf_context* c, int it);            // lam from xt for dirty particles
 void launch_dp_update_masked(pbf_context* c, int it, int last);
-void launch_viscosity_first(pbf_context* c);                  // v -> va, mask for the final positions
+void launch_viscosity_first(pbf_context* c);                  // v -> va, compact lists of the final positions
 void launch_vorticity_masked(pbf_context* c);                 // va -> scr
 void launch_apply_vorticity_masked(pbf_context* c);           // va += ...
 void launch_viscosity_last(pbf_context* c);                   // va -> v, x = x*
-void launch_lambda(pbf_context* c, const float4* pos, int only_if_flag, uint64_t b, uint64_t e);
+void launch_lambda(pbf_context* c, const float4* pos, uint64_t b, uint64_t e);
 void launch_dp(pbf_context* c, uint64_t b, uint64_t e);                     // xs, lam -> scr
-void launch_dp_update(pbf_context* c, int last, uint64_t b, uint64_t e);    // xt, lam -> xs (+ v)
 void launch_update_position_stars(pbf_context* c, uint64_t b, uint64_t e);
 void launch_update_velocities(pbf_context* c, uint64_t b, uint64_t e);
 void launch_update_positions(pbf_context* c, uint64_t b, uint64_t e);

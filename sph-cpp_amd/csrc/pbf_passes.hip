@@ -51,20 +51,15 @@ void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e) {
   timer_end(c, t);
 }
 
-// compute_densities (particles.h:292-313) fused with compute_pressure_lambdas
-// (:318-342).  Reads xs, writes xt (position star with the near-coincidence
-// fix applied), rho and lam.  The fused lambda is only valid when no particle
-// triggered the fix; otherwise the hack flag is raised and k_lambda recomputes
-// every lambda from xt.
-template <bool LAMBDA>
-__global__ void __launch_bounds__(kBlock) k_density_lambda(uint32_t n, uint32_t b, uint32_t e, KParams K,
-                                                           const uint32_t* __restrict__ oid,
-                                                           const float4* __restrict__ xs,
-                                                           const float4* __restrict__ v, float4* __restrict__ xt,
-                                                           float* __restrict__ rho, float* __restrict__ lam,
-                                                           const int32_t* __restrict__ nbr,
-                                                           const int32_t* __restrict__ nbr_count,
-                                                           unsigned long long* __restrict__ counters) {
+// compute_densities (particles.h:292-313).  Reads xs, writes xt (the position
+// star with the near-coincidence fix applied, particles.h:303-305) and rho;
+// the API swaps xs and xt afterwards.
+__global__ void __launch_bounds__(kBlock) k_density(uint32_t n, uint32_t b, uint32_t e, KParams K,
+                                                    const uint32_t* __restrict__ oid, const float4* __restrict__ xs,
+                                                    const float4* __restrict__ v, float4* __restrict__ xt,
+                                                    float* __restrict__ rho, const int32_t* __restrict__ nbr,
+                                                    const int32_t* __restrict__ nbr_count,
+                                                    unsigned long long* __restrict__ counters) {
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   f3 xp = ld3(xs, p);
@@ -73,8 +68,6 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda(uint32_t n, uint32_t 
     return;
   }
   float density = K.mass_W0;
-  float sum_C = 0.0f;
-  f3 sg = {0.f, 0.f, 0.f};
   uint32_t hacks = 0;
   const int cnt = nbr_count[p];
   for (int k = 0; k < cnt; ++k) {
@@ -91,55 +84,29 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda(uint32_t n, uint32_t 
       d2 = magsq3(d);
       if (!within_h(d2, K)) continue;
     }
-    const float r = sqrtf(d2);
-    density += K.mass * poly6_in(r, K);
-    if (LAMBDA) {
-      const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
-      dec3(sg, grad);
-      sum_C += magsq3(grad);
-    }
+    density += K.mass * poly6_in(sqrtf(d2), K);
   }
   rho[p] = density;
   st3(xt, p, xp);
-  if (LAMBDA) {
-    sum_C += magsq3(sg);
-    const float constraint = density / K.rho0 - 1.0f;
-    const float C_p = ((double)constraint < 0.0) ? 0.0f : constraint;
-    lam[p] = -C_p / (sum_C + K.eps);
-  }
-  if (hacks) {
-    atomicAdd(&counters[C_HACKS], (unsigned long long)hacks);
-    atomicExch(&counters[C_HACK_FLAG], 1ull);
-  }
+  if (hacks) atomicAdd(&counters[C_HACKS], (unsigned long long)hacks);
 }
 
-void launch_density_lambda(pbf_context* c, int with_lambda, uint64_t b, uint64_t e) {
+void launch_density(pbf_context* c, uint64_t b, uint64_t e) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
-  if (with_lambda) {
-    KernelTimer* t = timer_begin(c, "density_lambda");
-    hipLaunchKernelGGL(k_density_lambda<true>, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)b,
-                       (uint32_t)e, c->K, (const uint32_t*)c->oid, (const float4*)c->xs, (const float4*)c->v, c->xt,
-                       c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->counters);
-    timer_end(c, t);
-  } else {
-    KernelTimer* t = timer_begin(c, "density");
-    hipLaunchKernelGGL(k_density_lambda<false>, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)b,
-                       (uint32_t)e, c->K, (const uint32_t*)c->oid, (const float4*)c->xs, (const float4*)c->v, c->xt,
-                       c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->counters);
-    timer_end(c, t);
-  }
+  KernelTimer* t = timer_begin(c, "density");
+  hipLaunchKernelGGL(k_density, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)b, (uint32_t)e, c->K,
+                     (const uint32_t*)c->oid, (const float4*)c->xs, (const float4*)c->v, c->xt, c->rho,
+                     (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->counters);
+  timer_end(c, t);
 }
 
-// compute_pressure_lambdas (particles.h:318-342) from positions `pos`.  With
-// only_if_flag the kernel does nothing unless a density pass raised the flag.
-__global__ void __launch_bounds__(kBlock) k_lambda(uint32_t n, uint32_t b, uint32_t e, KParams K, int only_if_flag,
+// compute_pressure_lambdas (particles.h:318-342) from positions `pos`
+__global__ void __launch_bounds__(kBlock) k_lambda(uint32_t n, uint32_t b, uint32_t e, KParams K,
                                                    const uint32_t* __restrict__ oid, const float4* __restrict__ pos,
                                                    const float* __restrict__ rho, float* __restrict__ lam,
                                                    const int32_t* __restrict__ nbr,
-                                                   const int32_t* __restrict__ nbr_count,
-                                                   const unsigned long long* __restrict__ counters) {
-  if (only_if_flag && counters[C_HACK_FLAG] == 0ull) return;
+                                                   const int32_t* __restrict__ nbr_count) {
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n || !in_span(oid[p], b, e)) return;
   const f3 xp = ld3(pos, p);
@@ -162,13 +129,13 @@ __global__ void __launch_bounds__(kBlock) k_lambda(uint32_t n, uint32_t b, uint3
   lam[p] = -C_p / (sum_C + K.eps);
 }
 
-void launch_lambda(pbf_context* c, const float4* pos, int only_if_flag, uint64_t b, uint64_t e) {
+void launch_lambda(pbf_context* c, const float4* pos, uint64_t b, uint64_t e) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
-  KernelTimer* t = timer_begin(c, only_if_flag ? "lambda_repair" : "lambda");
+  KernelTimer* t = timer_begin(c, "lambda");
   hipLaunchKernelGGL(k_lambda, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)b, (uint32_t)e, c->K,
-                     only_if_flag, (const uint32_t*)c->oid, pos, (const float*)c->rho, c->lam,
-                     (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, (const unsigned long long*)c->counters);
+                     (const uint32_t*)c->oid, pos, (const float*)c->rho, c->lam, (const int32_t*)c->nbr,
+                     (const int32_t*)c->nbr_count);
   timer_end(c, t);
 }
 
@@ -205,42 +172,6 @@ void launch_dp(pbf_context* c, uint64_t b, uint64_t e) {
   hipLaunchKernelGGL(k_dp, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)b, (uint32_t)e, c->K,
                      (const uint32_t*)c->oid, (const float4*)c->xs, (const float*)c->lam, c->scr,
                      (const int32_t*)c->nbr, (const int32_t*)c->nbr_count);
-  timer_end(c, t);
-}
-
-// compute_pressure_dps + update_position_stars (particles.h:363-370) fused;
-// in the last iteration also update_velocities (:375-385).  Reads xt, lam;
-// writes xs (and v).  Resets the hack flag for the next density pass.
-__global__ void __launch_bounds__(kBlock) k_dp_update(uint32_t n, int last, KParams K,
-                                                      const float4* __restrict__ xt, const float* __restrict__ lam,
-                                                      const float4* __restrict__ x, float4* __restrict__ xs,
-                                                      float4* __restrict__ v, const int32_t* __restrict__ nbr,
-                                                      const int32_t* __restrict__ nbr_count,
-                                                      unsigned long long* __restrict__ counters) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_HACK_FLAG] = 0ull;
-  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const f3 xp = ld3(xt, p);
-  const f3 dp = pressure_dp(p, n, xp, lam[p], K, xt, lam, nbr, nbr_count[p]);
-  f3 nw = add3(xp, dp);
-  boundary(nw, K);
-  st3(xs, p, nw);
-  if (last) {
-    f3 vel = div3(sub3(nw, ld3(x, p)), K.dt);
-    if ((double)magsq3(vel) < K.vthr) vel = {0.f, 0.f, 0.f};
-    st3(v, p, vel);
-  }
-}
-
-void launch_dp_update(pbf_context* c, int last, uint64_t b, uint64_t e) {
-  (void)b;
-  (void)e;
-  const uint32_t n = (uint32_t)c->n;
-  if (!n) return;
-  KernelTimer* t = timer_begin(c, "dp_update");
-  hipLaunchKernelGGL(k_dp_update, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, last, c->K,
-                     (const float4*)c->xt, (const float*)c->lam, (const float4*)c->x, c->xs, c->v,
-                     (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->counters);
   timer_end(c, t);
 }
 

@@ -67,6 +67,8 @@ def lib():
         L.orc_bin_id.argtypes = [P, D]
         L.orc_bin_id.restype = C.c_uint64
         L.orc_boundary.argtypes = [P, D]
+        L.orc_solve.argtypes = [P, C.c_float, C.c_int, F, F, F, C.POINTER(C.c_int)]
+        L.orc_solve.restype = C.c_int
         _lib = L
     return _lib
 
@@ -189,6 +191,20 @@ class Oracle:
 
     def emax(self):
         return float(self.L.orc_emax(self.h))
+
+    def solve(self, tol, max_iter, warm=None):
+        """Iterate-to-tolerance step; returns (slots used, e_max history,
+        cumulative multipliers, warm start kept)."""
+        F = C.POINTER(C.c_float)
+        hist = np.zeros(max_iter, np.float32)
+        lsum = np.zeros(self.n, np.float32)
+        kept = C.c_int(0)
+        w = None if warm is None else np.ascontiguousarray(warm, dtype=np.float32)
+        used = self.L.orc_solve(self.h, float(tol), int(max_iter), None if w is None else w.ctypes.data_as(F),
+                                hist.ctypes.data_as(F), lsum.ctypes.data_as(F), C.byref(kept))
+        if used < 0:
+            raise RuntimeError("oracle solve failed")
+        return used, hist[:used], lsum, bool(kept.value)
 
     def poly6(self, r):
         return float(self.L.orc_poly6(self.h, float(r)))

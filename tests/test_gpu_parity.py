@@ -310,3 +310,50 @@ def test_partial_bin_span(pbf, scenes):
     assert np.array_equal(b, nb["begin"]) and np.array_equal(e, nb["end"])
     lists, counts = g.neighbor_lists()
     assert np.array_equal(counts, nb["counts"])
+
+
+def test_serial_mode_difference(pbf, scenes):
+    """Against the reference's serial CPP_PAR semantics (in-place XSPH and density fix).
+
+    Tolerances (measured with the oracle, stated here):
+      * C1 after 1 substep: positions bit-exact (XSPH changes only velocities,
+        which reach positions one substep later); velocities within twice the
+        float-vs-double spread of the serial run itself.
+      * 8,192-particle dam break after 10 and 100 substeps: RMS |dx|/h between
+        the GPU and the serial float run at most 3x the RMS between the serial
+        float and serial double runs (measured ratios 2.2 and 1.1).
+    """
+    p, pos, vel = scenes.config_c1(seed=0)
+    g = pbf.Particles(p)
+    g.add(pos, pos, vel)
+    g.step(1)
+    op = O.params_from_pbf(p)
+    s32 = O.Oracle(op, mode=O.SERIAL, threads=1)
+    s64 = O.Oracle(op, double=True, mode=O.SERIAL, threads=1)
+    for o in (s32, s64):
+        o.set_state(pos, pos, vel)
+        o.step(1)
+    assert_bitwise(g.positions(), s32.state()["x"], "C1 1 substep vs serial: x")
+    dv_gpu = np.abs(g.velocities() - s32.state()["v"]).max()
+    dv_fp = np.abs(s32.state()["v"] - s64.state()["v"]).max()
+    assert dv_gpu <= 2.0 * dv_fp, (dv_gpu, dv_fp)
+
+    p, pos, vel = scenes.dam_break((16, 16, 32), (1.6, 0.8, 1.6), seed=2)
+    g = pbf.Particles(p)
+    g.add(pos, pos, vel)
+    op = O.params_from_pbf(p)
+    s32 = O.Oracle(op, mode=O.SERIAL, threads=1)
+    s64 = O.Oracle(op, double=True, mode=O.SERIAL, threads=1)
+    for o in (s32, s64):
+        o.set_state(pos, pos, vel)
+    done = 0
+    for k in (10, 100):
+        g.step(k - done)
+        for o in (s32, s64):
+            o.step(k - done)
+        done = k
+        a = np.linalg.norm(g.positions() - s32.state()["x"], axis=1) / 0.1
+        b = np.linalg.norm(s32.state()["x"] - s64.state()["x"], axis=1) / 0.1
+        rms_gpu, rms_fp = np.sqrt((a ** 2).mean()), np.sqrt((b ** 2).mean())
+        print(f"substep {k}: GPU vs serial RMS {rms_gpu:.3e} h, float vs double RMS {rms_fp:.3e} h", flush=True)
+        assert rms_gpu <= 3.0 * rms_fp, (k, rms_gpu, rms_fp)

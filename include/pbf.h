@@ -82,6 +82,9 @@ typedef struct pbf_params {
 
 typedef struct pbf_context pbf_context;
 
+/* Most iteration slots of one pbf_solve call. */
+#define PBF_MAX_SOLVE_ITERATIONS 256
+
 /* Counters for the last step / pass. */
 typedef struct pbf_stats {
   uint64_t neighbor_overflow;  /* candidates within the bin spacing dropped by the cap
@@ -159,6 +162,36 @@ int pbf_update_positions(pbf_context* ctx, pbf_index_span span);        /* parti
 /* The whole step body of Simulation/Source/main.cpp:71-124 on one domain
  * (resident = local = all particles), nsteps times, with fused kernels. */
 int pbf_step(pbf_context* ctx, uint32_t nsteps);
+
+/* ---- iterate-to-tolerance solve (no counterpart in the reference, which runs a
+ * fixed solve_step_count of Jacobi iterations, Simulation/Source/main.cpp:82) ----
+ * One step like pbf_step, except the pressure loop: iteration slot k runs the density
+ * pass and measures e_k = max_p max(rho_p/rho0 - 1, 0) (the clamp of particles.h:325);
+ * if e_k <= tolerance the loop ends (x* keeps that pass's near-coincidence fix),
+ * otherwise lambdas, dps and position stars follow.  At most max_iterations slots.
+ * Velocities then come from update_velocities (particles.h:375-385).
+ *
+ * Warm start: before slot 0, a density pass measures e_pre and one dp pass with the
+ * warm multipliers Lambda_p in place of lambda moves x*.  If slot 0 then measures
+ * e_0 > e_pre, x* returns to its value before the warm update and slot 0 applies
+ * nothing (the fallback to a zero warm start).  Sources of Lambda:
+ *   PBF_WARM_GIVEN     pbf_set_warm_lambdas (original particle order)
+ *   PBF_WARM_PREVIOUS  the multipliers summed over the previous solve or step
+ * Every solve keeps the sum of the multipliers it applied (the warm ones included);
+ * pbf_get_lambda_sums returns it. */
+enum { PBF_WARM_NONE = 0, PBF_WARM_GIVEN = 1, PBF_WARM_PREVIOUS = 2 };
+
+typedef struct pbf_solve_info {
+  uint32_t iterations;              /* slots used (<= max_iterations) */
+  int32_t warm_start_kept;          /* 1: warm-start positions kept; 0: rejected or none */
+  float error_before_warm_start;    /* e_pre (warm start only) */
+  float error_history[PBF_MAX_SOLVE_ITERATIONS];  /* e_k of every used slot */
+} pbf_solve_info;
+
+int pbf_solve(pbf_context* ctx, float tolerance, uint32_t max_iterations, int warm_start);
+int pbf_set_warm_lambdas(pbf_context* ctx, const float* lambdas);
+int pbf_get_lambda_sums(pbf_context* ctx, float* out);
+int pbf_get_solve_info(pbf_context* ctx, pbf_solve_info* out);
 
 /* Block until the context's queued work is done. */
 int pbf_sync(pbf_context* ctx);

@@ -92,6 +92,19 @@ class Stats(C.Structure):
     ]
 
 
+PBF_MAX_SOLVE_ITERATIONS = 256
+PBF_WARM_NONE, PBF_WARM_GIVEN, PBF_WARM_PREVIOUS = 0, 1, 2
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_uint32),
+        ("warm_start_kept", C.c_int32),
+        ("error_before_warm_start", C.c_float),
+        ("error_history", C.c_float * PBF_MAX_SOLVE_ITERATIONS),
+    ]
+
+
 _lib = None
 
 _P = C.c_void_p
@@ -135,6 +148,10 @@ def lib():
     _sig(L, "pbf_find_neighbors", [_P, IndexSpan, IndexSpan])
     _sig(L, "pbf_apply_surface_tension", [_P, IndexSpan, IndexSpan])
     _sig(L, "pbf_step", [_P, C.c_uint32])
+    _sig(L, "pbf_solve", [_P, C.c_float, C.c_uint32, C.c_int])
+    _sig(L, "pbf_set_warm_lambdas", [_P, _F])
+    _sig(L, "pbf_get_lambda_sums", [_P, _F])
+    _sig(L, "pbf_get_solve_info", [_P, C.POINTER(SolveInfo)])
     _sig(L, "pbf_sync", [_P])
     _sig(L, "pbf_get_stats", [_P, C.POINTER(Stats)])
     _sig(L, "pbf_grid_dims", [_P, _U64])
@@ -322,6 +339,29 @@ class Particles:
     def step(self, nsteps: int = 1) -> None:
         """Simulation/Source/main.cpp:71-124 on one domain, nsteps times (fused kernels)."""
         _check(lib().pbf_step(self._ctx, int(nsteps)))
+
+    def solve(self, tolerance: float, max_iterations: int, warm_start: int = PBF_WARM_NONE) -> None:
+        """One step with an iterate-to-tolerance pressure solve (pbf_solve)."""
+        _check(lib().pbf_solve(self._ctx, float(tolerance), int(max_iterations), int(warm_start)))
+
+    def set_warm_lambdas(self, lambdas) -> None:
+        a = np.ascontiguousarray(lambdas, dtype=np.float32).reshape(-1)
+        if a.shape[0] != self.local_count():
+            raise ValueError("one multiplier per local particle")
+        _check(lib().pbf_set_warm_lambdas(self._ctx, a.ctypes.data_as(_F)))
+
+    def lambda_sums(self) -> np.ndarray:
+        out = np.empty(self.local_count(), dtype=np.float32)
+        _check(lib().pbf_get_lambda_sums(self._ctx, out.ctypes.data_as(_F)))
+        return out
+
+    def solve_info(self) -> dict:
+        si = SolveInfo()
+        _check(lib().pbf_get_solve_info(self._ctx, C.byref(si)))
+        k = si.iterations
+        return {"iterations": k, "warm_start_kept": bool(si.warm_start_kept),
+                "error_before_warm_start": si.error_before_warm_start,
+                "error_history": np.array(si.error_history[:k], dtype=np.float32)}
 
     def sync(self) -> None:
         _check(lib().pbf_sync(self._ctx))

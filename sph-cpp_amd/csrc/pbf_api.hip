@@ -175,7 +175,8 @@ static void free_all(pbf_context* c) {
                   c->scra, c->rho,  c->lam,  c->rhoa, c->lama, c->oid,  c->oida,       c->key,
                   c->keya, c->slot, c->tmp,  c->tmpo, c->nbr,  c->nbr_count, c->counters, c->stage,
                   c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
-                  c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty};
+                  c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty,
+                  c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -273,6 +274,9 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->rpart, scan_partials_size(radix_hist_size(N) + 1))) || (rc = dalloc(&c->heavy, N)) ||
       (rc = dalloc(&c->cnb, N * (uint64_t)c->K.stride)) || (rc = dalloc(&c->cnb_count, N)) ||
       (rc = dalloc(&c->moved, N)) || (rc = dalloc(&c->dirty, N)) ||
+      (rc = dalloc(&c->ehist, (uint64_t)kMaxSolveSlots + 1)) || (rc = dalloc(&c->lsum, N)) ||
+      (rc = dalloc(&c->lsuma, N)) || (rc = dalloc(&c->wlam, N)) || (rc = dalloc(&c->wgiven, N)) ||
+      (rc = dalloc(&c->xsave, N)) ||
       (rc = dalloc(&c->scan_partials, scan_partials_size(N + 1))) ||
       (rc = check_hip(hipEventCreate(&c->region_a), "hipEventCreate")) ||
       (rc = check_hip(hipEventCreate(&c->region_b), "hipEventCreate"))) {
@@ -284,6 +288,8 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
   launch_fill_u32((uint32_t*)c->nbr_count, 0u, N, c->stream);
   launch_fill_u32(c->dirty, 0u, N, c->stream);
+  launch_fill_u32((uint32_t*)c->lsum, 0u, N, c->stream);
+  launch_fill_u32(c->ehist, 0u, (uint64_t)kMaxSolveSlots + 1, c->stream);
   *out = c;
   return check_hip(hipStreamSynchronize(c->stream), "pbf_create");
 }
@@ -349,6 +355,7 @@ int pbf_add(pbf_context* c, const float* pos, const float* pos_star, const float
   hipLaunchKernelGGL(k_zero_f4, dim3(nb), dim3(256), 0, c->stream, c->scr, first, count);
   hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, c->rho, first, count);
   hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, c->lam, first, count);
+  hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, c->lsum, first, count);
   hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, (float*)c->nbr_count, first, count);
   c->n += count;
   c->lists_valid = false;
@@ -523,31 +530,133 @@ int pbf_update_positions(pbf_context* c, pbf_index_span s) {
   return check_hip(hipGetLastError(), "update_positions");
 }
 
+// The part of the step after the pressure solve (Simulation/Source/main.cpp:108-124).
+static void step_tail(pbf_context* c) {
+  const uint64_t n = c->n;
+  if (c->P.gamma != 0.f) launch_surface_tension(c, 0, n, 0, n);
+  launch_viscosity_first(c);         // XSPH 1: v -> va, compact lists of the final positions
+  launch_vorticity_masked(c);        // omega -> scr
+  launch_apply_vorticity_masked(c);  // va += ...
+  launch_viscosity_last(c);          // XSPH 2: va -> v, x = x*
+}
+
 // Simulation/Source/main.cpp:71-124 on one domain, fused kernels.
 int pbf_step(pbf_context* c, uint32_t nsteps) {
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   const uint64_t n = c->n;
+  const SolveCtl S{-1.0f, 0, nullptr, c->track_lsum ? c->lsum : nullptr, nullptr};
   for (uint32_t st = 0; st < nsteps; ++st) {
     launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
     launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
     find_neighbors_impl(c, n, 0, n);
+    if (c->track_lsum) launch_fill_u32((uint32_t*)c->lsum, 0u, n, c->stream);
     const uint32_t I = c->P.solve_step_count;
     for (uint32_t it = 0; it < I; ++it) {
       ++c->dirty_stamp;
-      launch_density_lambda_masked(c, (int)it);          // xs -> xt, rho, lam, compact lists
-      launch_mark_dirty(c, (int)it);                     // only if the density fix fired
-      launch_lambda_repair(c, (int)it);
-      launch_dp_update_masked(c, (int)it, it + 1 == I);  // xt, lam -> xs (+ v)
+      launch_density_lambda_masked(c, (int)it, 0, S);       // xs -> xt, rho, lam, compact lists
+      launch_mark_dirty(c, (int)it, S);                     // only if the density fix fired
+      launch_lambda_repair(c, (int)it, S);
+      launch_dp_update_masked(c, (int)it, it + 1 == I, S);  // xt, lam -> xs (+ v)
     }
     if (I == 0) launch_update_velocities(c, 0, n);
     c->last_iterations = I;
-    if (c->P.gamma != 0.f) launch_surface_tension(c, 0, n, 0, n);
-    launch_viscosity_first(c);         // XSPH 1: v -> va, mask of final positions
-    launch_vorticity_masked(c);        // omega -> scr
-    launch_apply_vorticity_masked(c);  // va += ...
-    launch_viscosity_last(c);          // XSPH 2: va -> v, x = x*
+    c->last_was_solve = false;
+    step_tail(c);
   }
   return check_hip(hipGetLastError(), "pbf_step");
+}
+
+// One step with an iterate-to-tolerance pressure solve.  The loop runs on the
+// device without host round trips: every slot's kernels are queued, and once
+// a slot measures an error <= tol the remaining ones return at once.
+int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm_start) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  if (!(tolerance >= 0.0f)) return set_error(PBF_ERR_ARG, "tolerance must be >= 0");
+  if (max_iterations < 1 || max_iterations > kMaxSolveSlots)
+    return set_error(PBF_ERR_ARG, "max_iterations must be in [1, PBF_MAX_SOLVE_ITERATIONS]");
+  if (warm_start < PBF_WARM_NONE || warm_start > PBF_WARM_PREVIOUS)
+    return set_error(PBF_ERR_ARG, "unknown warm start");
+  if (warm_start == PBF_WARM_GIVEN && !c->has_given)
+    return set_error(PBF_ERR_STATE, "PBF_WARM_GIVEN without pbf_set_warm_lambdas");
+  const uint64_t n = c->n;
+  const int warm = warm_start != PBF_WARM_NONE;
+  const bool had_lsum = c->track_lsum;
+  c->track_lsum = true;
+  launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
+  launch_fill_u64(c->counters + C_ITERS, (unsigned long long)max_iterations, 1, c->stream);
+  launch_fill_u64(c->counters + C_WARM_KEPT, (unsigned long long)warm, 1, c->stream);
+  launch_fill_u32(c->ehist, 0u, (uint64_t)max_iterations + 1, c->stream);
+  launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
+  find_neighbors_impl(c, n, 0, n);
+  if (warm_start == PBF_WARM_PREVIOUS) {
+    if (had_lsum) launch_copy_f1(c->lsum, c->wlam, n, c->stream);
+    else launch_fill_u32((uint32_t*)c->wlam, 0u, n, c->stream);
+  } else if (warm_start == PBF_WARM_GIVEN) {
+    launch_scatter_in_f1(c, c->wgiven, c->wlam);
+  }
+  launch_fill_u32((uint32_t*)c->lsum, 0u, n, c->stream);
+  const SolveCtl S{tolerance, warm, c->ehist, c->lsum, c->xsave};
+  if (warm) {
+    // first density pass (its error is the reference for the fallback), then the warm update
+    ++c->dirty_stamp;
+    const SolveCtl S0{-1.0f, 0, c->ehist, nullptr, nullptr};
+    launch_density_lambda_masked(c, 1, 0, S0);
+    launch_warm_dp(c);
+  }
+  for (uint32_t it = 0; it < max_iterations; ++it) {
+    ++c->dirty_stamp;
+    launch_density_lambda_masked(c, (int)it, (int)it + 1, S);
+    launch_mark_dirty(c, (int)it, S);
+    launch_lambda_repair(c, (int)it, S);
+    launch_dp_update_masked(c, (int)it, 0, S);
+  }
+  if (warm) launch_add_warm(c);
+  launch_update_velocities(c, 0, n);
+  c->last_iterations = max_iterations;
+  c->last_was_solve = true;
+  step_tail(c);
+  return check_hip(hipGetLastError(), "pbf_solve");
+}
+
+int pbf_set_warm_lambdas(pbf_context* c, const float* lambdas) {
+  if (!c || !lambdas) return set_error(PBF_ERR_ARG, "null argument");
+  if (c->n) {
+    int rc = check_hip(hipMemcpyAsync(c->wgiven, lambdas, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream),
+                       "upload");
+    if (rc) return rc;
+    if ((rc = check_hip(hipStreamSynchronize(c->stream), "upload"))) return rc;
+  }
+  c->has_given = true;
+  return PBF_OK;
+}
+
+int pbf_get_lambda_sums(pbf_context* c, float* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  if (!c->track_lsum) return set_error(PBF_ERR_STATE, "no pbf_solve has run");
+  if (!c->n) return PBF_OK;
+  launch_gather_out_f1(c, c->lsum, c->stage);
+  int rc = check_hip(hipMemcpyAsync(out, c->stage, c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                     "download");
+  if (rc) return rc;
+  return check_hip(hipStreamSynchronize(c->stream), "download");
+}
+
+int pbf_get_solve_info(pbf_context* c, pbf_solve_info* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  if (!c->last_was_solve) return set_error(PBF_ERR_STATE, "the last step was not a pbf_solve");
+  unsigned long long h[C_NUM];
+  uint32_t e[kMaxSolveSlots + 1];
+  int rc = check_hip(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream), "solve info");
+  if (rc) return rc;
+  if ((rc = check_hip(hipMemcpyAsync(e, c->ehist, sizeof(e), hipMemcpyDeviceToHost, c->stream), "solve info")))
+    return rc;
+  if ((rc = check_hip(hipStreamSynchronize(c->stream), "solve info"))) return rc;
+  std::memset(out, 0, sizeof(*out));
+  out->iterations = (uint32_t)h[C_ITERS];
+  out->warm_start_kept = (int32_t)h[C_WARM_KEPT];
+  std::memcpy(&out->error_before_warm_start, &e[0], 4);
+  for (uint32_t k = 0; k < out->iterations && k < kMaxSolveSlots; ++k) std::memcpy(&out->error_history[k], &e[k + 1], 4);
+  return PBF_OK;
 }
 
 int pbf_sync(pbf_context* c) {
@@ -568,7 +677,7 @@ int pbf_get_stats(pbf_context* c, pbf_stats* out) {
   out->neighbor_overflow = h[C_OVERFLOW];
   out->outside_grid = h[C_OUTSIDE];
   out->density_hacks = h[C_HACKS];
-  out->iterations = c->last_iterations;
+  out->iterations = c->last_was_solve ? h[C_ITERS] : c->last_iterations;
   double m = 0.0;
   for (float r : rho) {
     const float e = r / c->P.rest_density - 1.0f;

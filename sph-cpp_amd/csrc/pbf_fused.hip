@@ -57,20 +57,16 @@ __device__ __forceinline__ void flush_compact(const int32_t* __restrict__ s_list
 // ---------------------------------------------------------------------------
 // compute_densities (particles.h:292-313) + compute_pressure_lambdas (:318-342)
 // xs -> xt, rho, lam; compact row of the entries within h.
-__global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, int it, KParams K,
-                                                             const float4* __restrict__ xs,
-                                                             const float4* __restrict__ v, float4* __restrict__ xt,
-                                                             float* __restrict__ rho, float* __restrict__ lam,
-                                                             const int32_t* __restrict__ nbr,
-                                                             const int32_t* __restrict__ nbr_count,
-                                                             int32_t* __restrict__ cnb,
-                                                             int32_t* __restrict__ cnb_count,
-                                                             uint32_t* __restrict__ moved,
-                                                             unsigned long long* __restrict__ counters) {
-  extern __shared__ int32_t s_list[];  // [cap][kStageBlock]
-  const uint32_t lane = threadIdx.x;
-  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+// Returns this particle's density error max(rho/rho0 - 1, 0).
+__device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int it, const KParams& K,
+                                                    int32_t* __restrict__ s_list, uint32_t lane,
+                                                    const float4* __restrict__ xs, const float4* __restrict__ v,
+                                                    float4* __restrict__ xt, float* __restrict__ rho,
+                                                    float* __restrict__ lam, const int32_t* __restrict__ nbr,
+                                                    const int32_t* __restrict__ nbr_count,
+                                                    int32_t* __restrict__ cnb, int32_t* __restrict__ cnb_count,
+                                                    uint32_t* __restrict__ moved,
+                                                    unsigned long long* __restrict__ counters) {
   f3 xp = ld3(xs, p);
   const int cnt = nbr_count[p];
   // phase 1: compact row of the entries within h; stop at a near-coincident one
@@ -149,6 +145,42 @@ __global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, in
     moved[atomicAdd(&counters[C_MOVED_N + (it & 1)], 1ull)] = p;
     atomicExch(&counters[C_HACK_FLAG + (it & 1)], 1ull);
   }
+  return constraint > 0.0f ? constraint : 0.0f;
+}
+
+// Max over the wavefront; every lane of the wavefront must call it.
+__device__ __forceinline__ float wave_max(float e) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) e = fmaxf(e, __shfl_xor(e, d, 64));
+  return e;
+}
+
+// One block = one wavefront.  With S.ehist, the block's largest density
+// error goes into slot eslot (atomicMax on the bits of a non-negative float
+// orders like the float).  With S.tol >= 0 nothing runs once the solve is done.
+__global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S,
+                                                                  KParams K, const float4* __restrict__ xs,
+                                                                  const float4* __restrict__ v,
+                                                                  float4* __restrict__ xt, float* __restrict__ rho,
+                                                                  float* __restrict__ lam,
+                                                                  const int32_t* __restrict__ nbr,
+                                                                  const int32_t* __restrict__ nbr_count,
+                                                                  int32_t* __restrict__ cnb,
+                                                                  int32_t* __restrict__ cnb_count,
+                                                                  uint32_t* __restrict__ moved,
+                                                                  unsigned long long* __restrict__ counters) {
+  if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
+  extern __shared__ int32_t s_list[];  // [cap][kStageBlock]
+  const uint32_t lane = threadIdx.x;
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  float e = 0.0f;
+  if (p < n)
+    e = density_lambda_one(p, n, it, K, s_list, lane, xs, v, xt, rho, lam, nbr, nbr_count, cnb, cnb_count, moved,
+                           counters);
+  if (S.ehist) {
+    e = wave_max(e);
+    if (lane == 0 && e > 0.0f) atomicMax(&S.ehist[eslot], __float_as_uint(e));
+  }
 }
 
 // A particle is dirty when it or a particle in its list was moved by the
@@ -161,9 +193,10 @@ __global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, in
 constexpr uint32_t kMaxMovedMarked = 4096;
 constexpr uint32_t kMaxMarkedPerMoved = 8192;
 
-__global__ void k_mark_dirty(int it, uint32_t stamp, KParams K, const uint32_t* __restrict__ key,
+__global__ void k_mark_dirty(int it, uint32_t stamp, SolveCtl S, KParams K, const uint32_t* __restrict__ key,
                              const uint32_t* __restrict__ start, const uint32_t* __restrict__ moved,
                              uint32_t* __restrict__ dirty, unsigned long long* __restrict__ counters) {
+  if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   const uint32_t nmoved = (uint32_t)counters[C_MOVED_N + (it & 1)];
   if (nmoved > kMaxMovedMarked) {
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_MOVED_ALL + (it & 1)] = 1ull;
@@ -197,13 +230,14 @@ __global__ void k_mark_dirty(int it, uint32_t stamp, KParams K, const uint32_t* 
 
 // compute_pressure_lambdas (particles.h:318-342) from xt for the dirty
 // particles of this iteration (plain walks).
-__global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, uint32_t stamp, KParams K,
-                                                          const float4* __restrict__ xt,
+__global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, uint32_t stamp, SolveCtl S,
+                                                          KParams K, const float4* __restrict__ xt,
                                                           const float* __restrict__ rho, float* __restrict__ lam,
                                                           const int32_t* __restrict__ nbr,
                                                           const int32_t* __restrict__ nbr_count,
                                                           const uint32_t* __restrict__ dirty,
                                                           const unsigned long long* __restrict__ counters) {
+  if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   if (counters[C_HACK_FLAG + (it & 1)] == 0ull) return;
   const bool all = counters[C_MOVED_ALL + (it & 1)] != 0ull;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
@@ -237,8 +271,12 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
 
 // compute_pressure_dps (particles.h:348-358) + update_position_stars
 // (:363-370); on the last iteration also update_velocities (:375-385).
-__global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, KParams K,
-                                                        const float4* __restrict__ xt,
+// In an iterate-to-tolerance solve, slot `it` first checks the error its
+// density pass measured: a rejected warm start (slot 0 error above the error
+// before the warm start) restores xsave; an error <= tol ends the solve with
+// x* = xt.  Neither applies an update.
+__global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, SolveCtl S,
+                                                        KParams K, const float4* __restrict__ xt,
                                                         const float* __restrict__ lam,
                                                         const float4* __restrict__ x, float4* __restrict__ xs,
                                                         float4* __restrict__ v, const int32_t* __restrict__ nbr,
@@ -247,13 +285,31 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
                                                         const int32_t* __restrict__ cnb_count,
                                                         const uint32_t* __restrict__ dirty,
                                                         unsigned long long* __restrict__ counters) {
+  if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   // clear the next iteration's flags (nobody reads them until the next density pass)
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (first) {
     counters[C_HACK_FLAG + ((it + 1) & 1)] = 0ull;
     counters[C_MOVED_N + ((it + 1) & 1)] = 0ull;
     counters[C_MOVED_ALL + ((it + 1) & 1)] = 0ull;
   }
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (S.ehist) {
+    const float e = __uint_as_float(S.ehist[it + 1]);
+    if (S.warm && it == 0 && e > __uint_as_float(S.ehist[0])) {
+      if (first) counters[C_WARM_KEPT] = 0ull;
+      if (p < n) xs[p] = S.xsave[p];
+      return;
+    }
+    if (S.tol >= 0.0f && e <= S.tol) {
+      if (first) {
+        counters[C_DONE] = 1ull;
+        counters[C_ITERS] = (unsigned long long)(it + 1);
+      }
+      if (p < n) xs[p] = xt[p];
+      return;
+    }
+  }
   if (p >= n) return;
   // a dirty particle (moved by the density fix, or next to one) walks its whole list with the test
   const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull &&
@@ -285,11 +341,62 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
   f3 nw = add3(xp, smul3(K.inv_rho0, dp));
   boundary(nw, K);
   st3(xs, p, nw);
+  if (S.lsum) S.lsum[p] += lp;
   if (last) {
     f3 vel = div3(sub3(nw, ld3(x, p)), K.dt);
     if ((double)magsq3(vel) < K.vthr) vel = {0.f, 0.f, 0.f};
     st3(v, p, vel);
   }
+}
+
+// Warm start of an iterate-to-tolerance solve: one compute_pressure_dps +
+// update_position_stars pass (particles.h:348-370) with the given
+// multipliers in place of lambda, over the full lists with the within-h test.
+// xt holds the position stars after the first density pass; they are kept in
+// xsave for the fallback.
+__global__ void __launch_bounds__(kBlock) k_warm_dp(uint32_t n, KParams K, const float4* __restrict__ xt,
+                                                    const float* __restrict__ wlam,
+                                                    const int32_t* __restrict__ nbr,
+                                                    const int32_t* __restrict__ nbr_count, float4* __restrict__ xs,
+                                                    float4* __restrict__ xsave) {
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float4 xp4 = xt[p];
+  const f3 xp = xyz(xp4);
+  const float lp = wlam[p];
+  f3 dp = {0.f, 0.f, 0.f};
+  const int cnt = nbr_count[p];
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(nbr, n, k0, cnt, p, q);
+    float4 t[kBatch];
+    float lq[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      t[u] = xt[q[u]];
+      lq[u] = wlam[q[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const f3 d = sub3(xp, xyz(t[u]));
+      const float d2 = magsq3(d);
+      if (k0 + u >= cnt || !within_h(d2, K)) continue;
+      const float r = sqrtf(d2);
+      acc3(dp, smul3(lp + lq[u], spiky_in(d, r, K)));
+    }
+  }
+  f3 nw = add3(xp, smul3(K.inv_rho0, dp));
+  boundary(nw, K);
+  st3(xs, p, nw);
+  xsave[p] = xp4;
+}
+
+// After the Jacobi loop: the kept warm-start multipliers join the sum.
+__global__ void k_add_warm(uint32_t n, const float* __restrict__ wlam, float* __restrict__ lsum,
+                           const unsigned long long* __restrict__ counters) {
+  if (counters[C_WARM_KEPT] == 0ull) return;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) lsum[p] += wlam[p];
 }
 
 // XSPH sum (particles.h:441-444) over a compact row, from global memory
@@ -465,33 +572,42 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     timer_end((c), t_);                                                                                     \
   } while (0)
 
-void launch_density_lambda_masked(pbf_context* c, int it) {
-  PBF_LAUNCH_STAGED(c, "density_lambda", k_density_lambda_c, it, c->K, (const float4*)c->xs, (const float4*)c->v,
-                    c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb,
-                    c->cnb_count, c->moved, c->counters);
+void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S) {
+  PBF_LAUNCH_STAGED(c, "density_lambda", k_density_lambda_c, it, eslot, S, c->K, (const float4*)c->xs,
+                    (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr,
+                    (const int32_t*)c->nbr_count, c->cnb, c->cnb_count, c->moved, c->counters);
 }
 
-void launch_mark_dirty(pbf_context* c, int it) {
+void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
   KernelTimer* t = timer_begin(c, "mark_dirty");
-  hipLaunchKernelGGL(k_mark_dirty, dim3(16), dim3(kBlock), 0, c->stream, it, c->dirty_stamp, c->K,
+  hipLaunchKernelGGL(k_mark_dirty, dim3(16), dim3(kBlock), 0, c->stream, it, c->dirty_stamp, S, c->K,
                      (const uint32_t*)c->key, (const uint32_t*)c->cell_start, (const uint32_t*)c->moved, c->dirty,
                      c->counters);
   timer_end(c, t);
 }
 
-void launch_lambda_repair(pbf_context* c, int it) {
-  PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->dirty_stamp, c->K, (const float4*)c->xt,
+void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S) {
+  PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->dirty_stamp, S, c->K, (const float4*)c->xt,
              (const float*)c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
              (const uint32_t*)c->dirty, (const unsigned long long*)c->counters);
 }
 
-void launch_dp_update_masked(pbf_context* c, int it, int last) {
-  PBF_LAUNCH(c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, c->K, (const float4*)c->xt,
+void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S) {
+  PBF_LAUNCH(c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
              (const float*)c->lam, (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
              (const int32_t*)c->nbr_count, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count,
              (const uint32_t*)c->dirty, c->counters);
+}
+
+void launch_warm_dp(pbf_context* c) {
+  PBF_LAUNCH(c, "warm_dp", k_warm_dp, c->K, (const float4*)c->xt, (const float*)c->wlam, (const int32_t*)c->nbr,
+             (const int32_t*)c->nbr_count, c->xs, c->xsave);
+}
+
+void launch_add_warm(pbf_context* c) {
+  PBF_LAUNCH(c, "add_warm", k_add_warm, (const float*)c->wlam, c->lsum, (const unsigned long long*)c->counters);
 }
 
 void launch_viscosity_first(pbf_context* c) {

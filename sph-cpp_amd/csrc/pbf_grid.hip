@@ -216,14 +216,16 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
                                                     const float4* __restrict__ v, const float* __restrict__ rho,
                                                     const float* __restrict__ lam, const float4* __restrict__ scr,
                                                     const uint32_t* __restrict__ oid,
-                                                    const uint32_t* __restrict__ key, float4* __restrict__ xo,
+                                                    const uint32_t* __restrict__ key,
+                                                    const float* __restrict__ lsum, float4* __restrict__ xo,
                                                     float4* __restrict__ xso, float4* __restrict__ vo,
                                                     float* __restrict__ rhoo, float* __restrict__ lamo,
                                                     float4* __restrict__ scro, uint32_t* __restrict__ oido,
-                                                    uint32_t* __restrict__ keyo) {
+                                                    uint32_t* __restrict__ keyo, float* __restrict__ lsumo) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t p = perm[s];
+  if (lsum) lsumo[s] = lsum[p];
   xo[s] = x[p];
   xso[s] = xs[p];
   vo[s] = v[p];
@@ -241,8 +243,10 @@ void launch_reorder(pbf_context* c) {
   hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->perm,
                      (const float4*)c->x, (const float4*)c->xs, (const float4*)c->v, (const float*)c->rho,
                      (const float*)c->lam, (const float4*)c->scr, (const uint32_t*)c->oid,
-                     (const uint32_t*)c->key, c->xa, c->xsa, c->va, c->rhoa, c->lama, c->scra, c->oida, c->keya);
+                     (const uint32_t*)c->key, c->track_lsum ? (const float*)c->lsum : nullptr, c->xa, c->xsa, c->va,
+                     c->rhoa, c->lama, c->scra, c->oida, c->keya, c->lsuma);
   timer_end(c, t);
+  if (c->track_lsum) std::swap(c->lsum, c->lsuma);
   std::swap(c->x, c->xa);
   std::swap(c->xs, c->xsa);
   std::swap(c->v, c->va);

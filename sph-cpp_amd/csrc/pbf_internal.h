@@ -20,7 +20,12 @@
 //   nbr_count        int32[n]
 //   cnb, cnb_count   int32[K][n], int32[n]  compact lists of the entries within h, column
 //                                major (built by the density and first XSPH passes)
-//   counters         u64[8]      overflow, outside, hacks, hack flag, e_max bits, ...
+//   counters         u64[16]     overflow, outside, hacks, hack flags, solve state, ...
+//   ehist            uint32[kMaxSolveSlots+1]  max density error per solve slot (float bits)
+//   lsum, lsuma      float[n]    multipliers summed over the updates of the last solve
+//                                (carried through the reorder while a warm start may use them)
+//   wlam, wgiven     float[n]    warm-start multipliers (slot order / original order)
+//   xsave            float4[n]   position stars before a warm start (for its fallback)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -64,7 +69,22 @@ enum Counter {
   C_EMAX = 6,       // float bits of the max density error (atomicMax on non-negative floats)
   C_MOVED_N = 7,    // two counts (7, 8): particles moved by the density fix, per iteration parity
   C_MOVED_ALL = 9,  // two flags (9, 10): too many moved particles or marked cells -> treat all as dirty
+  C_DONE = 11,      // iterate-to-tolerance: the error fell below the tolerance
+  C_ITERS = 12,     // iterate-to-tolerance: slots used
+  C_WARM_KEPT = 13, // the warm-start positions were kept
   C_NUM = 16
+};
+
+constexpr uint32_t kMaxSolveSlots = PBF_MAX_SOLVE_ITERATIONS;
+
+// Per-launch control of the Jacobi loop kernels.  A plain fixed-iteration
+// step passes tol < 0 and null pointers.
+struct SolveCtl {
+  float tol;             // >= 0: stop once the slot's max density error is <= tol
+  int warm;              // a warm-start update was applied before slot 0
+  uint32_t* ehist;       // [0] error before the warm start, [k+1] error measured in slot k
+  float* lsum;           // sum of the applied multipliers (null: not kept)
+  const float4* xsave;   // position stars to restore when the warm start is rejected
 };
 
 struct KernelTimer {
@@ -89,6 +109,8 @@ struct pbf_context {
   bool lists_valid = false;
   uint32_t last_iterations = 0;
   uint32_t dirty_stamp = 0;  // incremented per Jacobi iteration of the fused step
+  bool last_was_solve = false;  // stats.iterations comes from the device counter
+  bool track_lsum = false;      // lsum follows the particles through the reorder
 
   float4 *x = nullptr, *xs = nullptr, *v = nullptr;
   float4 *xa = nullptr, *xsa = nullptr, *va = nullptr;
@@ -108,6 +130,10 @@ struct pbf_context {
   int32_t *nbr = nullptr, *nbr_count = nullptr;
   int32_t *cnb = nullptr, *cnb_count = nullptr;
   unsigned long long* counters = nullptr;
+  uint32_t* ehist = nullptr;
+  float *lsum = nullptr, *lsuma = nullptr, *wlam = nullptr, *wgiven = nullptr;
+  float4* xsave = nullptr;
+  bool has_given = false;
   float* stage = nullptr;  // device staging for original-order transfers (4 floats / particle)
   uint64_t* stage64 = nullptr;
 
@@ -148,10 +174,12 @@ enum { F_FORCES = 1, F_PREDICT = 2 };
 void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e);
 void launch_density(pbf_context* c, uint64_t b, uint64_t e);  // xs -> xt, rho
 // ---- fused-step launchers (pbf_fused.hip): they build and use the compact lists ----
-void launch_density_lambda_masked(pbf_context* c, int it);   // xs -> xt, rho, lam, compact lists
-void launch_mark_dirty(pbf_context* c, int it);               // particles next to a moved one
-void launch_lambda_repair(pbf_context* c, int it);            // lam from xt for dirty particles
-void launch_dp_update_masked(pbf_context* c, int it, int last);
+void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S);  // xs -> xt, rho, lam
+void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S);     // particles next to a moved one
+void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S);  // lam from xt for dirty particles
+void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S);
+void launch_warm_dp(pbf_context* c);      // xt, wlam -> xs (warm start), xsave = xt
+void launch_add_warm(pbf_context* c);     // lsum += wlam when the warm start was kept
 void launch_viscosity_first(pbf_context* c);                  // v -> va, compact lists of the final positions
 void launch_vorticity_masked(pbf_context* c);                 // va -> scr
 void launch_apply_vorticity_masked(pbf_context* c);           // va += ...
@@ -167,6 +195,7 @@ void launch_vorticity(pbf_context* c, const float4* vin, uint64_t b, uint64_t e)
 void launch_apply_vorticity(pbf_context* c, float4* vio, uint64_t b, uint64_t e);
 void launch_surface_tension(pbf_context* c, uint64_t cb, uint64_t ce, uint64_t b, uint64_t e);
 void launch_copy_f4(const float4* src, float4* dst, uint64_t n, hipStream_t s);
+void launch_copy_f1(const float* src, float* dst, uint64_t n, hipStream_t s);
 // original-order transfer helpers
 void launch_gather_out_f4(pbf_context* c, const float4* src, float* dst3, int comps);
 void launch_gather_out_f1(pbf_context* c, const float* src, float* dst);

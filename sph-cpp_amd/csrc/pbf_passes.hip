@@ -421,6 +421,14 @@ void launch_copy_f4(const float4* src, float4* dst, uint64_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_copy_f4, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, n);
 }
 
+__global__ void k_copy_f1(const float* __restrict__ src, float* __restrict__ dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+void launch_copy_f1(const float* src, float* dst, uint64_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_copy_f1, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, n);
+}
+
 // ---- original-order transfers through the staging buffer ----
 __global__ void k_gather_out_f4(uint32_t n, const uint32_t* __restrict__ oid, const float4* __restrict__ src,
                                 float* __restrict__ dst, int comps) {

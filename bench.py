@@ -40,6 +40,7 @@ ALG_BYTES = {
     "density_lambda": 12 + 4 + 4,          # R x*; W rho, lambda
     "dp_update": 12 + 4 + 12,              # R x*, lambda; W x*
     "fill_neighbors": 12 + 4,              # R x*; W count (lists not counted)
+    "fill_heavy": 0,                       # the same lists, finished for crowded cells
     "viscosity": 12 + 12 + 4 + 12,         # R x*, v, rho; W v
     "viscosity_positions": 12 + 12 + 4 + 12 + 12,
     "vorticity": 12 + 12 + 12,             # R x*, v; W omega
@@ -94,8 +95,19 @@ def _profile_traffic(kernel: str):
         return None
 
 
-KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "fill_neighbors", "density_lambda", "mark_dirty",
-           "lambda_repair", "dp_update", "viscosity", "vorticity", "apply_vorticity", "viscosity_positions"]
+KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "fill_neighbors", "fill_heavy", "density_lambda",
+           "mark_dirty", "lambda_repair", "dp_update", "viscosity", "vorticity", "apply_vorticity",
+           "viscosity_positions"]
+
+# HIP kernel behind each timer (the names rocprofv3 reports)
+HIP_KERNEL = {
+    "forces_predict": "pbf::k_forces_predict", "bin": "pbf::k_bin", "reorder": "pbf::k_reorder",
+    "fill_neighbors": "pbf::k_fill_neighbors", "fill_heavy": "pbf::k_fill_heavy",
+    "density_lambda": "pbf::k_density_lambda_c", "mark_dirty": "pbf::k_mark_dirty",
+    "lambda_repair": "pbf::k_lambda_repair", "dp_update": "pbf::k_dp_update_c",
+    "viscosity": "pbf::k_viscosity_first", "vorticity": "pbf::k_vorticity_c",
+    "apply_vorticity": "pbf::k_apply_vorticity_c", "viscosity_positions": "pbf::k_viscosity_last",
+}
 
 
 def timed_run(pbf, params, pos, vel, steps, profile=True):
@@ -126,6 +138,135 @@ def timed_run(pbf, params, pos, vel, steps, profile=True):
     return res
 
 
+# ---------------------------------------------------------------------------
+# rocprofv3 output -> the text summaries kept under profiles/
+PROFILE_COMMANDS = {
+    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/trace -o c2 -- "
+             "python3 bench.py --no-cpu --no-variants --no-warm",
+    "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o c2 -- "
+             "python3 bench.py --no-cpu --no-variants --no-warm",
+    "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o c2 -- "
+             "python3 bench.py --no-cpu --no-variants --no-warm",
+    "cal_fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/cal_fetch -o cp -- "
+                 "python3 bench.py --calibrate-copy 512",
+    "cal_write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/cal_write -o cp -- "
+                 "python3 bench.py --calibrate-copy 512",
+}
+
+
+def _short(name: str) -> str:
+    return name.split("(")[0].replace("void ", "").strip()
+
+
+def _pmc(path: str, counter: str) -> dict:
+    import csv
+    out: dict = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                out.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+    return out
+
+
+def summarize(d: str, workload: str) -> None:
+    """Write profiles/<workload>_kernels.md, <workload>_hbm_traffic.md and
+    pmc_traffic.json from the rocprofv3 runs of PROFILE_COMMANDS under d."""
+    import csv
+    prof = os.path.join(ROOT, "profiles")
+    os.makedirs(prof, exist_ok=True)
+    rows = list(csv.DictReader(open(os.path.join(d, "trace", "c2_kernel_stats.csv"))))
+    lines = [f"# Kernel times: {workload}", "",
+             "Made by (from the repository root, on one MI355X):", "",
+             "    " + PROFILE_COMMANDS["trace"], "",
+             "then `python3 bench.py --summarize gpurun_out`.  The run is the benchmark: a warm-up context, then",
+             "200 substeps of the 1,048,576-particle dam break from the initial scene (cap 60).", "",
+             "| kernel | calls | avg us | min us | max us | total ms | % |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.1f} |")
+    open(os.path.join(prof, f"{workload}_kernels.md"), "w").write("\n".join(lines) + "\n")
+
+    fetch = _pmc(os.path.join(d, "pmc_fetch", "c2_counter_collection.csv"), "FETCH_SIZE")
+    write = _pmc(os.path.join(d, "pmc_write", "c2_counter_collection.csv"), "WRITE_SIZE")
+    cf = _pmc(os.path.join(d, "cal_fetch", "cp_counter_collection.csv"), "FETCH_SIZE")
+    cw = _pmc(os.path.join(d, "cal_write", "cp_counter_collection.csv"), "WRITE_SIZE")
+    copy_kib = 512 * 1024
+    cal_f = [v for vs in cf.values() for v in vs if v > copy_kib / 4]
+    cal_w = [v for vs in cw.values() for v in vs if v > copy_kib / 4]
+    f_scale = copy_kib / statistics.median(cal_f)
+    w_scale = copy_kib / statistics.median(cal_w)
+    traffic = {}
+    lines = [f"# HBM traffic per launch: {workload}", "",
+             "Two counter runs (one counter each) of the benchmark command, and two of a plain 512 MiB",
+             "device-to-device copy that calibrates them:", ""]
+    lines += ["    " + PROFILE_COMMANDS[k] for k in ("fetch", "write", "cal_fetch", "cal_write")]
+    lines += ["", f"Calibration: the 512 MiB copy reads FETCH_SIZE = {statistics.median(cal_f):.1f} KiB and "
+              f"WRITE_SIZE = {statistics.median(cal_w):.1f} KiB per dispatch, so FETCH_SIZE is scaled by "
+              f"{f_scale:.3f} and WRITE_SIZE by {w_scale:.3f}.", "",
+              "| kernel | launches | read MB/launch | written MB/launch | total MB/launch |",
+              "|---|---:|---:|---:|---:|"]
+    for k in sorted(fetch, key=lambda k: -sum(fetch[k])):
+        fv, wv = fetch[k], write.get(k, [0.0])
+        rb = statistics.mean(fv) * 1024 * f_scale
+        wb = statistics.mean(wv) * 1024 * w_scale
+        traffic[k] = {"bytes_per_launch": round(rb + wb), "read_bytes": round(rb), "write_bytes": round(wb),
+                      "launches": len(fv)}
+        lines.append(f"| `{k}` | {len(fv)} | {rb / 1e6:.1f} | {wb / 1e6:.1f} | {(rb + wb) / 1e6:.1f} |")
+    open(os.path.join(prof, f"{workload}_hbm_traffic.md"), "w").write("\n".join(lines) + "\n")
+    with open(os.path.join(prof, "pmc_traffic.json"), "w") as f:
+        json.dump({"workload": workload, "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+                   **traffic}, f, indent=1)
+
+
+WARM_MODEL = os.path.join(ROOT, "sph-cpp_amd", "warm_model.npy")
+
+
+def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=128):
+    """Plain vs warm-started iterate-to-tolerance solves on the same scene:
+    each mode runs its own trajectory of `steps` substeps, every substep
+    iterating until the mean density error is <= tol (at most max_slots)."""
+    if not os.path.exists(WARM_MODEL):
+        return {"skipped": "no trained warm-start model (sph-cpp_amd/train_warm.py)"}
+    weights = np.load(WARM_MODEL, allow_pickle=False)
+    model = pbf.WarmModel(16)
+    model.set_params(weights)
+    out = {"criterion": f"mean density error <= {tol} per substep (at most {max_slots} Jacobi slots)",
+           "tolerance": tol, "max_slots": max_slots, "steps": steps}
+    for name, mode in (("plain", pbf.PBF_WARM_NONE), ("previous", pbf.PBF_WARM_PREVIOUS),
+                       ("learned", pbf.PBF_WARM_LEARNED)):
+        params, pos, vel = scenes.config_c2(seed=seed, lattice_counts=counts)
+        params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
+        n = pos.shape[0]
+        g = pbf.Particles(params)
+        g.add(pos, pos, vel)
+        if mode == pbf.PBF_WARM_LEARNED:
+            g.set_warm_model(model)
+        g.sync()
+        slots, met, emax, kept = [], 0, [], 0
+        t0 = time.perf_counter()
+        for st in range(steps):
+            if st % 50 == 0:
+                print(f"warm-start comparison: {name}, substep {st}", file=sys.stderr, flush=True)
+            g.solve(tol, max_slots, mode)
+            info = g.solve_info()  # waits for the substep
+            k = info["iterations"]
+            slots.append(k)
+            met += int(info["mean_error_history"][k - 1] <= np.float32(tol))
+            emax.append(float(info["max_error_history"][k - 1]))
+            kept += int(info["warm_start_kept"])
+        wall = time.perf_counter() - t0
+        g.close()
+        out[name] = {"value": n * steps / wall, "ms_per_step": wall * 1e3 / steps,
+                     "mean_slots": float(np.mean(slots)), "max_slots_used": int(np.max(slots)),
+                     "steps_meeting_tolerance": met, "warm_start_kept": kept,
+                     "final_max_error_median": float(np.median(emax))}
+    model.close()
+    out["learned_vs_plain"] = out["learned"]["value"] / out["plain"]["value"]
+    out["previous_vs_plain"] = out["previous"]["value"] / out["plain"]["value"]
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,7 +280,16 @@ def main(argv=None):
     ap.add_argument("--no-variants", dest="variants", action="store_false", help="skip the cap-128 variant")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device to device 5 times (PMC byte-counter calibration)")
+    ap.add_argument("--no-warm", dest="warm", action="store_false",
+                    help="skip the plain vs warm-start comparison at a density-error tolerance")
+    ap.add_argument("--warm-steps", type=int, default=200)
+    ap.add_argument("--summarize", default="", metavar="DIR",
+                    help="only write the profiles/ summaries from the rocprofv3 outputs under DIR")
     args = ap.parse_args(argv)
+
+    if args.summarize:
+        summarize(args.summarize, "pbf_step_dambreak_1M")
+        return 0
 
     if args.calibrate_copy:
         pbf = _load("sph-cpp_amd")
@@ -178,11 +328,12 @@ def main(argv=None):
     roof = {
         "bound": "hbm",
         "kernel": dom,
+        "hip_kernel": HIP_KERNEL.get(dom, ""),
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": _profile_traffic(dom),
+        "traffic": _profile_traffic(HIP_KERNEL.get(dom, dom)),
         "alg_bytes_per_launch": alg,
         "avg_launch_ms": round(avg_launch_ms, 5),
         "launches": kd["launches"],
@@ -228,6 +379,8 @@ def main(argv=None):
         out["variants"] = {"cap128": {"value": vr["value"], "ms_per_step": vr["seconds"] * 1e3 / args.steps,
                                       "stats_last_step": vr["stats"],
                                       "note": "same scene with a 128-entry neighbour list (no list saturation)"}}
+    if args.warm:
+        out["warm_start"] = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(params, pos, vel)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]

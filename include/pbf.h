@@ -79,6 +79,9 @@ typedef struct pbf_params {
  * 27-cell walk, so without this option a full list ends the walk early.  Lists
  * are identical either way. */
 #define PBF_OPT_COUNT_OVERFLOW 1u
+/* pbf_solve compares the tolerance with the mean density error
+ * (1/n) sum_p max(rho_p/rho0 - 1, 0) instead of the maximum. */
+#define PBF_OPT_SOLVE_MEAN_ERROR 2u
 
 typedef struct pbf_context pbf_context;
 
@@ -178,20 +181,82 @@ int pbf_step(pbf_context* ctx, uint32_t nsteps);
  *   PBF_WARM_GIVEN     pbf_set_warm_lambdas (original particle order)
  *   PBF_WARM_PREVIOUS  the multipliers summed over the previous solve or step
  * Every solve keeps the sum of the multipliers it applied (the warm ones included);
- * pbf_get_lambda_sums returns it. */
-enum { PBF_WARM_NONE = 0, PBF_WARM_GIVEN = 1, PBF_WARM_PREVIOUS = 2 };
+ * pbf_get_lambda_sums returns it.  Slots are queued in chunks of 8; between chunks
+ * the call waits for the device to say whether the solve has ended. */
+enum { PBF_WARM_NONE = 0, PBF_WARM_GIVEN = 1, PBF_WARM_PREVIOUS = 2, PBF_WARM_LEARNED = 3 };
 
+/* The error e_k compared with the tolerance is the maximum by default and the mean with
+ * PBF_OPT_SOLVE_MEAN_ERROR; both are recorded.  The mean is summed in fixed point
+ * (each particle's error times 2^32, truncated to an integer), so it does not depend
+ * on the order of the sum. */
 typedef struct pbf_solve_info {
   uint32_t iterations;              /* slots used (<= max_iterations) */
   int32_t warm_start_kept;          /* 1: warm-start positions kept; 0: rejected or none */
-  float error_before_warm_start;    /* e_pre (warm start only) */
-  float error_history[PBF_MAX_SOLVE_ITERATIONS];  /* e_k of every used slot */
+  float max_error_before_warm_start;
+  float mean_error_before_warm_start;
+  float max_error_history[PBF_MAX_SOLVE_ITERATIONS];   /* of every used slot */
+  float mean_error_history[PBF_MAX_SOLVE_ITERATIONS];
 } pbf_solve_info;
 
 int pbf_solve(pbf_context* ctx, float tolerance, uint32_t max_iterations, int warm_start);
 int pbf_set_warm_lambdas(pbf_context* ctx, const float* lambdas);
 int pbf_get_lambda_sums(pbf_context* ctx, float* out);
+/* replace the kept sums (original order): the next solve's PBF_WARM_PREVIOUS source
+ * and feature 2 of the learned warm start */
+int pbf_set_lambda_sums(pbf_context* ctx, const float* in);
 int pbf_get_solve_info(pbf_context* ctx, pbf_solve_info* out);
+
+/* ---- learned warm start (PBF_WARM_LEARNED; no counterpart in the reference) ----
+ * Features of particle p, from the first density pass of a warm-started solve
+ * (x* after external forces and prediction):
+ *   0  rho_p/rho0 - 1                      4  neighbours within h / 32
+ *   1  lambda_p of that pass / S           5  v_p.y * dt / h
+ *   2  previous solve's multiplier sum / S 6  mean (y_q - y_p) over neighbours within h / h
+ *   3  sum of |grad C|^2 terms * 1e-5      7  m dt sum_q (v_q - v_p) . grad_q C  (velocity
+ *                                             divergence: the density change over one step)
+ * with S = PBF_WARM_LAMBDA_SCALE and grad_q C = -(1/rho0) grad W(x_p - x_q) (particles.h:332),
+ * limited to [-1,4], [-64,0], [-64,0], [0,8], [0,4], [-4,4], [-2,2], [-4,4].
+ * A network 8 -> 64 -> 64 -> 1 (SiLU after both hidden layers; BF16 MFMA operands, FP32
+ * accumulation) gives y_p, and the warm multiplier is Lambda_p = Lambda_prev,p + S * y_p:
+ * the previous solve's sum, corrected (a network that outputs 0 is PBF_WARM_PREVIOUS).
+ * Parameters (PBF_WARM_PARAMS floats): W1[64][8], b1[64], W2[64][64] (row = output
+ * unit), b2[64], w3[64], b3.  Training minimises the mean squared error against targets
+ * (Lambda_p - Lambda_prev,p) / S, Lambda being the multiplier sum a solve applied
+ * (pbf_warm_model_add_from_context), with Adam (beta 0.9, 0.999, eps 1e-8) on FP32 master
+ * weights. */
+#define PBF_WARM_FEATURES 8
+#define PBF_WARM_HIDDEN 64
+#define PBF_WARM_PARAMS 4801
+#define PBF_WARM_LAMBDA_SCALE 1e-4f
+
+typedef struct pbf_warm_model pbf_warm_model;
+int pbf_warm_model_create(uint64_t max_samples, pbf_warm_model** out);
+int pbf_warm_model_destroy(pbf_warm_model* model);
+int pbf_warm_model_init(pbf_warm_model* model, uint64_t seed);
+int pbf_warm_model_set_params(pbf_warm_model* model, const float* params);
+int pbf_warm_model_get_params(pbf_warm_model* model, float* params);
+/* sample store: features (count x 8) and targets (count, already divided by S) */
+int pbf_warm_model_add_samples(pbf_warm_model* model, const float* features, const float* targets,
+                               uint64_t count);
+/* samples from the context's last warm-started solve, for every stride-th particle in
+ * the context's cell order (stride >= 1): its features, and as target the multiplier sum
+ * it applied minus the previous sums, over S, limited to [-64, 64] */
+int pbf_warm_model_add_from_context(pbf_warm_model* model, pbf_context* ctx, uint32_t stride);
+int pbf_warm_model_clear_samples(pbf_warm_model* model);
+uint64_t pbf_warm_model_sample_count(const pbf_warm_model* model);
+/* gradient and loss of samples [first, first + count) (no update) */
+int pbf_warm_model_gradient(pbf_warm_model* model, uint64_t first, uint32_t count, float* grad, float* loss);
+/* one Adam update with a given gradient */
+int pbf_warm_model_adam_step(pbf_warm_model* model, const float* grad, float learning_rate);
+/* epochs of shuffled minibatches; *mean_loss = mean batch loss over the run */
+int pbf_warm_model_train(pbf_warm_model* model, uint32_t epochs, uint32_t batch, float learning_rate,
+                         uint64_t seed, float* mean_loss);
+/* network output (Lambda / S) for count feature rows */
+int pbf_warm_model_predict(pbf_warm_model* model, const float* features, uint64_t count, float* out);
+/* copy the model's weights into the context (used by PBF_WARM_LEARNED) */
+int pbf_set_warm_model(pbf_context* ctx, pbf_warm_model* model);
+/* features of the last warm-started solve, n x 8, original order */
+int pbf_get_warm_features(pbf_context* ctx, float* out);
 
 /* Block until the context's queued work is done. */
 int pbf_sync(pbf_context* ctx);

@@ -129,6 +129,9 @@ struct Sim {
   size_t n = 0;
   std::vector<V3<R>> x, xs, v, scratch, tmpv;
   std::vector<R> rho, lam;
+  // multipliers summed over the last iterate-to-tolerance solve (its warm
+  // update included); a feature of the next solve's learned warm start
+  std::vector<R> lsum_prev;
   // neighbours (neighbors.h:279-284); list entries are size_t like NeighborList
   std::vector<uint64_t> keys, ids, begin, end, lists;
   std::vector<int> counts;
@@ -508,18 +511,104 @@ struct Sim {
     }
     return m;
   }
+  // mean density error summed in fixed point: each particle's error (as a
+  // float) times 2^32, truncated, so that the sum does not depend on order
+  float emean_fixed(size_t b, size_t e) const {
+    unsigned long long sum = 0;
+    for (size_t p = b; p < e; ++p) {
+      const R c = rho[p] / rho0 - static_cast<R>(1.0);
+      const float ef = (float)(c > (R)0 ? c : (R)0);
+      sum += (unsigned long long)(std::fmin(ef, 2147483648.0f) * 4294967296.0f);
+    }
+    return (float)((double)sum / 4294967296.0 / (double)(e > b ? e - b : 1));
+  }
+
+  // The first density pass of a warm-started solve as the device fuses it
+  // with the multipliers: one walk per particle computes compute_densities
+  // (particles.h:292-313, with the near-coincidence fix of :303-305) and the
+  // compute_pressure_lambdas terms (:318-342), so terms after a fix use the
+  // moved position.  Reads x* as it was when the pass began (ORC_JACOBI).
+  // Writes rho, lam, x* and the 8 warm-start features of every particle:
+  //   C, lambda/S, previous multiplier sum/S, 1e-5 * sum |grad C|^2 terms,
+  //   (neighbours within h)/32, v.y dt/h, mean(y_q - y_p)/h,
+  //   m dt sum_q (v_q - v_p) . grad C_q  (the density change the velocities predict)
+  // with S = 1e-4 and grad C_q = -(1/rho0) grad W(x_p - x_q), each limited to
+  // the range the device uses: [-1,4], [-64,0], [-64,0], [0,8], [0,4], [-4,4],
+  // [-2,2], [-4,4].
+  void density_features(std::vector<float>& feat) {
+    const R W0 = poly6((R)0.0);
+    const std::vector<V3<R>> src = xs;
+    feat.assign(n * 8, 0.0f);
+    const float inv_s = 1.0f / 1e-4f;
+    long long hk = 0;
+    auto body = [&](size_t p) {
+      R density = mass * W0;
+      V3<R> xp = src[p];
+      long long lh = 0;
+      R sum_C = 0.0;
+      V3<R> sg = {(R)0, (R)0, (R)0};
+      int within = 0;
+      R dy = 0.0, div = 0.0;
+      const uint64_t* L = &lists[p * (size_t)cap];
+      for (int k = 0; k < counts[p]; ++k) {
+        const size_t q = (size_t)L[k];
+        if (mag(sub(xp, src[q])) < 0.00000001) {
+          dec(xp, divs(muls(v[p], dt), (R)50.0));
+          ++lh;
+        }
+        const R r = mag(sub(xp, src[q]));
+        density += mass * poly6(r);
+        if (r > h) continue;
+        const V3<R> gradient = smul((R)-1.0 / rho0, spiky(xp, src[q]));
+        dec(sg, gradient);
+        sum_C += magsq(gradient);
+        ++within;
+        dy -= (xp.y - src[q].y);
+        const V3<R> dv = sub(v[p], v[q]);
+        div -= (dv.x * gradient.x + dv.y * gradient.y) + dv.z * gradient.z;
+      }
+      rho[p] = density;
+      xs[p] = xp;
+      sum_C += magsq(sg);
+      const R constraint = density / rho0 - static_cast<R>(1.0);
+      const R C_p = (constraint < 0.0 ? 0.0 : constraint);
+      lam[p] = -C_p / (sum_C + eps);
+      float* f = &feat[p * 8];
+      const R dyn = dy / (R)(within > 0 ? within : 1);
+      auto clampf = [](float x, float lo, float hi) { return x > lo ? (x < hi ? x : hi) : lo; };
+      f[0] = clampf((float)constraint, -1.0f, 4.0f);
+      f[1] = clampf((float)lam[p] * inv_s, -64.0f, 0.0f);
+      f[2] = clampf((float)lsum_prev[p] * inv_s, -64.0f, 0.0f);
+      f[3] = clampf((float)sum_C * 1e-5f, 0.0f, 8.0f);
+      f[4] = clampf((float)within / 32.0f, 0.0f, 4.0f);
+      f[5] = clampf((float)(v[p].y * dt / h), -4.0f, 4.0f);
+      f[6] = clampf((float)(dyn / h), -2.0f, 2.0f);
+      f[7] = clampf((float)(div * mass * dt), -4.0f, 4.0f);
+      return lh;
+    };
+    if (threads > 1) {
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : hk)
+      for (long long p = 0; p < (long long)n; ++p) hk += body((size_t)p);
+    } else {
+      for (size_t p = 0; p < n; ++p) hk += body(p);
+    }
+    hacks += hk;
+  }
 
   // Iterate-to-tolerance step (no reference counterpart: the reference runs
   // a fixed solve_step_count of Jacobi iterations, Simulation/Source/main.cpp:82).
-  // Iteration slot k: density pass, e_k = max density error; stop if
-  // e_k <= tol, else lambdas, dps, position stars.  Warm start (warm != NULL,
-  // one multiplier per particle): a density pass first records e_pre, then
-  // one dp pass with lam = warm moves x*; if slot 0 then measures e_0 > e_pre,
-  // x* returns to its value after that first density pass and slot 0 makes no
-  // update (the fallback to a zero warm start).  lam_sum (optional) receives
-  // the sum of the multipliers of every update that was applied, the warm
-  // one included.  Returns the number of slots used (<= max_iter).
-  int solve(R tol, int max_iter, const R* warm, R* hist, R* lam_sum, int* kept) {
+  // Iteration slot k: density pass, e_k = density error (max, or with
+  // use_mean the fixed-point mean); stop if e_k <= tol, else lambdas, dps,
+  // position stars.  Warm start (warm != NULL, one multiplier per particle):
+  // a fused density pass (density_features) first records e_pre and the
+  // features, then one dp pass with lam = warm moves x*; if slot 0 then
+  // measures e_0 > e_pre, x* returns to its value after that first pass and
+  // slot 0 makes no update (the fallback to a zero warm start).  lam_sum
+  // (optional) receives the sum of the multipliers of every update that was
+  // applied, the warm one included; it is also kept for the next solve's
+  // features.  Returns the number of slots used (<= max_iter).
+  int solve(R tol, int max_iter, int use_mean, const R* warm, float* hist_max, float* hist_mean, R* lam_sum,
+            std::vector<float>* feat, int* kept) {
     apply_external_forces(0, n);
     predict_positions(0, n);
     find_neighbors(0, n, 0, n);
@@ -527,9 +616,12 @@ struct Sim {
     std::vector<V3<R>> saved;
     R e_pre = (R)0;
     int keep = 0;
+    auto error = [&]() -> R { return use_mean ? (R)emean_fixed(0, n) : emax_r(0, n); };
     if (warm) {
-      compute_densities(0, n);
-      e_pre = emax_r(0, n);
+      std::vector<float> f;
+      density_features(f);
+      if (feat) *feat = f;
+      e_pre = error();
       saved = xs;
       lam.assign(warm, warm + n);
       compute_pressure_dps(0, n);
@@ -540,8 +632,9 @@ struct Sim {
     bool done = false;
     for (int it = 0; it < max_iter && !done; ++it) {
       compute_densities(0, n);
-      const R e = emax_r(0, n);
-      if (hist) hist[it] = e;
+      const R e = error();
+      if (hist_max) hist_max[it] = (float)emax_r(0, n);
+      if (hist_mean) hist_mean[it] = emean_fixed(0, n);
       used = it + 1;
       if (warm && it == 0 && e > e_pre) {
         xs = saved;
@@ -560,6 +653,7 @@ struct Sim {
     if (warm && keep)
       for (size_t p = 0; p < n; ++p) acc[p] += warm[p];
     if (lam_sum) std::copy(acc.begin(), acc.end(), lam_sum);
+    lsum_prev = acc;
     if (kept) *kept = keep;
     update_velocities(0, n);
     if (gamma != (R)0) apply_surface_tension(0, n, 0, n);
@@ -616,6 +710,7 @@ static void load(Sim<R>& S, size_t n, const T* x, const T* xs, const T* v) {
   }
   S.lists.assign(n * (size_t)S.cap, 0);
   S.counts.assign(n, 0);
+  S.lsum_prev.assign(n, (R)0);
 }
 
 template <class R>
@@ -728,17 +823,25 @@ int orc_step(void* h, int nsteps) {
   return 0;
 }
 
-// iterate-to-tolerance step; warm / lam_sum are n floats in particle order
-// (either may be NULL), hist receives max_iter floats (may be NULL)
-int orc_solve(void* h, float tol, int max_iter, const float* warm, float* hist, float* lam_sum, int* kept) {
+// iterate-to-tolerance step; warm / lam_sum are n floats and feat n*8 floats
+// in particle order, hist_max / hist_mean max_iter floats (any may be NULL;
+// feat is written only when warm is given)
+int orc_solve(void* h, float tol, int max_iter, int use_mean, const float* warm, float* hist_max, float* hist_mean,
+              float* lam_sum, float* feat, int* kept) {
   Handle* H = (Handle*)h;
   if (max_iter < 1) return -1;
-  if (!H->dbl) return H->f.solve(tol, max_iter, warm, hist, lam_sum, kept);
-  std::vector<double> w, hs((size_t)max_iter, 0.0), ls(H->d.n, 0.0);
-  if (warm) w.assign(warm, warm + H->d.n);
-  const int used = H->d.solve(tol, max_iter, warm ? w.data() : nullptr, hs.data(), ls.data(), kept);
-  for (int i = 0; hist && i < max_iter; ++i) hist[i] = (float)hs[(size_t)i];
-  for (size_t i = 0; lam_sum && i < H->d.n; ++i) lam_sum[i] = (float)ls[i];
+  std::vector<float> f;
+  int used;
+  if (!H->dbl) {
+    used = H->f.solve(tol, max_iter, use_mean, warm, hist_max, hist_mean, lam_sum, warm ? &f : nullptr, kept);
+  } else {
+    std::vector<double> w, ls(H->d.n, 0.0);
+    if (warm) w.assign(warm, warm + H->d.n);
+    used = H->d.solve(tol, max_iter, use_mean, warm ? w.data() : nullptr, hist_max, hist_mean, ls.data(),
+                      warm ? &f : nullptr, kept);
+    for (size_t i = 0; lam_sum && i < H->d.n; ++i) lam_sum[i] = (float)ls[i];
+  }
+  if (feat && warm) std::copy(f.begin(), f.end(), feat);
   return used;
 }
 

@@ -93,15 +93,22 @@ class Stats(C.Structure):
 
 
 PBF_MAX_SOLVE_ITERATIONS = 256
-PBF_WARM_NONE, PBF_WARM_GIVEN, PBF_WARM_PREVIOUS = 0, 1, 2
+PBF_WARM_NONE, PBF_WARM_GIVEN, PBF_WARM_PREVIOUS, PBF_WARM_LEARNED = 0, 1, 2, 3
+PBF_OPT_SOLVE_MEAN_ERROR = 2
+PBF_WARM_FEATURES = 8
+PBF_WARM_HIDDEN = 64
+PBF_WARM_PARAMS = 4801
+PBF_WARM_LAMBDA_SCALE = 1e-4
 
 
 class SolveInfo(C.Structure):
     _fields_ = [
         ("iterations", C.c_uint32),
         ("warm_start_kept", C.c_int32),
-        ("error_before_warm_start", C.c_float),
-        ("error_history", C.c_float * PBF_MAX_SOLVE_ITERATIONS),
+        ("max_error_before_warm_start", C.c_float),
+        ("mean_error_before_warm_start", C.c_float),
+        ("max_error_history", C.c_float * PBF_MAX_SOLVE_ITERATIONS),
+        ("mean_error_history", C.c_float * PBF_MAX_SOLVE_ITERATIONS),
     ]
 
 
@@ -152,6 +159,23 @@ def lib():
     _sig(L, "pbf_set_warm_lambdas", [_P, _F])
     _sig(L, "pbf_get_lambda_sums", [_P, _F])
     _sig(L, "pbf_get_solve_info", [_P, C.POINTER(SolveInfo)])
+    _sig(L, "pbf_get_warm_features", [_P, _F])
+    _sig(L, "pbf_set_lambda_sums", [_P, _F])
+    _sig(L, "pbf_set_warm_model", [_P, _P])
+    _sig(L, "pbf_warm_model_create", [C.c_uint64, C.POINTER(_P)])
+    _sig(L, "pbf_warm_model_destroy", [_P])
+    _sig(L, "pbf_warm_model_init", [_P, C.c_uint64])
+    _sig(L, "pbf_warm_model_set_params", [_P, _F])
+    _sig(L, "pbf_warm_model_get_params", [_P, _F])
+    _sig(L, "pbf_warm_model_add_samples", [_P, _F, _F, C.c_uint64])
+    _sig(L, "pbf_warm_model_add_from_context", [_P, _P, C.c_uint32])
+    _sig(L, "pbf_warm_model_clear_samples", [_P])
+    L.pbf_warm_model_sample_count.argtypes = [_P]
+    L.pbf_warm_model_sample_count.restype = C.c_uint64
+    _sig(L, "pbf_warm_model_gradient", [_P, C.c_uint64, C.c_uint32, _F, _F])
+    _sig(L, "pbf_warm_model_adam_step", [_P, _F, C.c_float])
+    _sig(L, "pbf_warm_model_train", [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _F])
+    _sig(L, "pbf_warm_model_predict", [_P, _F, C.c_uint64, _F])
     _sig(L, "pbf_sync", [_P])
     _sig(L, "pbf_get_stats", [_P, C.POINTER(Stats)])
     _sig(L, "pbf_grid_dims", [_P, _U64])
@@ -355,13 +379,29 @@ class Particles:
         _check(lib().pbf_get_lambda_sums(self._ctx, out.ctypes.data_as(_F)))
         return out
 
+    def set_lambda_sums(self, sums) -> None:
+        a = np.ascontiguousarray(sums, dtype=np.float32).reshape(-1)
+        if a.shape[0] != self.local_count():
+            raise ValueError("one sum per local particle")
+        _check(lib().pbf_set_lambda_sums(self._ctx, a.ctypes.data_as(_F)))
+
     def solve_info(self) -> dict:
         si = SolveInfo()
         _check(lib().pbf_get_solve_info(self._ctx, C.byref(si)))
         k = si.iterations
         return {"iterations": k, "warm_start_kept": bool(si.warm_start_kept),
-                "error_before_warm_start": si.error_before_warm_start,
-                "error_history": np.array(si.error_history[:k], dtype=np.float32)}
+                "max_error_before_warm_start": si.max_error_before_warm_start,
+                "mean_error_before_warm_start": si.mean_error_before_warm_start,
+                "max_error_history": np.array(si.max_error_history[:k], dtype=np.float32),
+                "mean_error_history": np.array(si.mean_error_history[:k], dtype=np.float32)}
+
+    def warm_features(self) -> np.ndarray:
+        out = np.empty((self.local_count(), PBF_WARM_FEATURES), dtype=np.float32)
+        _check(lib().pbf_get_warm_features(self._ctx, out.ctypes.data_as(_F)))
+        return out
+
+    def set_warm_model(self, model: "WarmModel") -> None:
+        _check(lib().pbf_set_warm_model(self._ctx, model._h))
 
     def sync(self) -> None:
         _check(lib().pbf_sync(self._ctx))
@@ -480,3 +520,77 @@ def bound_counting(sorted_keys: np.ndarray, begin: int, end: int, upper: bool) -
     f = lib().pbf_upper_bound_counting if upper else lib().pbf_lower_bound_counting
     _check(f(kb.ptr, sk.size, span(begin, end), ob.ptr))
     return ob.to_array(np.uint32, end - begin)
+
+
+class WarmModel:
+    """Learned warm start: the 8-64-64-1 network of pbf_warm_model_* (training
+    and inference on the GPU).  Targets and outputs are Lambda / PBF_WARM_LAMBDA_SCALE."""
+
+    def __init__(self, max_samples: int):
+        h = C.c_void_p()
+        _check(lib().pbf_warm_model_create(int(max_samples), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            lib().pbf_warm_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, seed: int = 0) -> None:
+        _check(lib().pbf_warm_model_init(self._h, int(seed)))
+
+    def set_params(self, params) -> None:
+        a = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        if a.shape[0] != PBF_WARM_PARAMS:
+            raise ValueError(f"{PBF_WARM_PARAMS} parameters expected")
+        _check(lib().pbf_warm_model_set_params(self._h, a.ctypes.data_as(_F)))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(PBF_WARM_PARAMS, dtype=np.float32)
+        _check(lib().pbf_warm_model_get_params(self._h, out.ctypes.data_as(_F)))
+        return out
+
+    def add_samples(self, features, targets) -> None:
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PBF_WARM_FEATURES)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        if f.shape[0] != t.shape[0]:
+            raise ValueError("one target per feature row")
+        _check(lib().pbf_warm_model_add_samples(self._h, f.ctypes.data_as(_F), t.ctypes.data_as(_F), t.shape[0]))
+
+    def add_from(self, particles: "Particles", stride: int = 1) -> None:
+        _check(lib().pbf_warm_model_add_from_context(self._h, particles._ctx, int(stride)))
+
+    def clear_samples(self) -> None:
+        _check(lib().pbf_warm_model_clear_samples(self._h))
+
+    def sample_count(self) -> int:
+        return int(lib().pbf_warm_model_sample_count(self._h))
+
+    def gradient(self, first: int, count: int):
+        g = np.empty(PBF_WARM_PARAMS, dtype=np.float32)
+        loss = C.c_float(0.0)
+        _check(lib().pbf_warm_model_gradient(self._h, int(first), int(count), g.ctypes.data_as(_F),
+                                             C.byref(loss)))
+        return g, float(loss.value)
+
+    def adam_step(self, grad, learning_rate: float) -> None:
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        _check(lib().pbf_warm_model_adam_step(self._h, g.ctypes.data_as(_F), float(learning_rate)))
+
+    def train(self, epochs: int, batch: int, learning_rate: float, seed: int = 0) -> float:
+        loss = C.c_float(0.0)
+        _check(lib().pbf_warm_model_train(self._h, int(epochs), int(batch), float(learning_rate), int(seed),
+                                          C.byref(loss)))
+        return float(loss.value)
+
+    def predict(self, features) -> np.ndarray:
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PBF_WARM_FEATURES)
+        out = np.empty(f.shape[0], dtype=np.float32)
+        _check(lib().pbf_warm_model_predict(self._h, f.ctypes.data_as(_F), f.shape[0], out.ctypes.data_as(_F)))
+        return out

@@ -176,7 +176,8 @@ static void free_all(pbf_context* c) {
                   c->keya, c->slot, c->tmp,  c->tmpo, c->nbr,  c->nbr_count, c->counters, c->stage,
                   c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
                   c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty,
-                  c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave};
+                  c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave, c->esum, c->feat, c->mw1,
+                  c->mw2, c->mp};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -191,6 +192,10 @@ static int need_lists(pbf_context* c) {
 __global__ void k_iota(uint32_t* p, uint64_t first, uint64_t count) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[first + i] = (uint32_t)(first + i);
+}
+__global__ void k_widen3(const float* __restrict__ src, float4* __restrict__ dst, uint64_t first, uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[first + i] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f);
 }
 __global__ void k_zero_f4(float4* p, uint64_t first, uint64_t count) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -276,8 +281,12 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->moved, N)) || (rc = dalloc(&c->dirty, N)) ||
       (rc = dalloc(&c->ehist, (uint64_t)kMaxSolveSlots + 1)) || (rc = dalloc(&c->lsum, N)) ||
       (rc = dalloc(&c->lsuma, N)) || (rc = dalloc(&c->wlam, N)) || (rc = dalloc(&c->wgiven, N)) ||
-      (rc = dalloc(&c->xsave, N)) ||
+      (rc = dalloc(&c->xsave, N)) || (rc = dalloc(&c->esum, (uint64_t)kMaxSolveSlots + 1)) ||
+      (rc = dalloc(&c->feat, N * PBF_WARM_FEATURES)) || (rc = dalloc(&c->mw1, (uint64_t)PBF_WARM_HIDDEN * 16)) ||
+      (rc = dalloc(&c->mw2, (uint64_t)PBF_WARM_HIDDEN * PBF_WARM_HIDDEN)) ||
+      (rc = dalloc(&c->mp, (uint64_t)PBF_WARM_PARAMS)) ||
       (rc = dalloc(&c->scan_partials, scan_partials_size(N + 1))) ||
+      (rc = check_hip(hipHostMalloc((void**)&c->done_host, sizeof(unsigned long long)), "hipHostMalloc")) ||
       (rc = check_hip(hipEventCreate(&c->region_a), "hipEventCreate")) ||
       (rc = check_hip(hipEventCreate(&c->region_b), "hipEventCreate"))) {
     free_all(c);
@@ -290,6 +299,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   launch_fill_u32(c->dirty, 0u, N, c->stream);
   launch_fill_u32((uint32_t*)c->lsum, 0u, N, c->stream);
   launch_fill_u32(c->ehist, 0u, (uint64_t)kMaxSolveSlots + 1, c->stream);
+  launch_fill_u64(c->esum, 0ull, (uint64_t)kMaxSolveSlots + 1, c->stream);
   *out = c;
   return check_hip(hipStreamSynchronize(c->stream), "pbf_create");
 }
@@ -299,6 +309,7 @@ int pbf_destroy(pbf_context* c) {
   hipStreamSynchronize(c->stream);
   timers_resolve(c);
   free_all(c);
+  if (c->done_host) hipHostFree(c->done_host);
   if (c->region_a) hipEventDestroy(c->region_a);
   if (c->region_b) hipEventDestroy(c->region_b);
   hipStreamDestroy(c->stream);
@@ -341,16 +352,18 @@ int pbf_add(pbf_context* c, const float* pos, const float* pos_star, const float
   if (c->n + count > c->capacity) return set_error(PBF_ERR_CAPACITY, "Not enough capacity to push_back");
   if (!count) return PBF_OK;
   const uint64_t first = c->n;
-  int rc;
-  const size_t w = 3 * sizeof(float);
-  if ((rc = check_hip(hipMemcpy2DAsync(c->x + first, sizeof(float4), pos, w, w, count, hipMemcpyHostToDevice,
-                                       c->stream), "add positions")) ||
-      (rc = check_hip(hipMemcpy2DAsync(c->xs + first, sizeof(float4), pos_star, w, w, count,
-                                       hipMemcpyHostToDevice, c->stream), "add position stars")) ||
-      (rc = check_hip(hipMemcpy2DAsync(c->v + first, sizeof(float4), vel, w, w, count, hipMemcpyHostToDevice,
-                                       c->stream), "add velocities")))
-    return rc;
   const uint32_t nb = (uint32_t)((count + 255) / 256);
+  // one contiguous upload per array into the staging buffer, then a kernel
+  // widens Vec<float,3> to the float4 layout (a pitched host-to-device copy
+  // of 12-byte rows is far slower)
+  const float* src[3] = {pos, pos_star, vel};
+  float4* dst[3] = {c->x, c->xs, c->v};
+  for (int a = 0; a < 3; ++a) {
+    int rc = check_hip(hipMemcpyAsync(c->stage, src[a], count * 3 * sizeof(float), hipMemcpyHostToDevice,
+                                      c->stream), "add");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_widen3, dim3(nb), dim3(256), 0, c->stream, (const float*)c->stage, dst[a], first, count);
+  }
   hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, c->stream, c->oid, first, count);
   hipLaunchKernelGGL(k_zero_f4, dim3(nb), dim3(256), 0, c->stream, c->scr, first, count);
   hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, c->rho, first, count);
@@ -544,7 +557,7 @@ static void step_tail(pbf_context* c) {
 int pbf_step(pbf_context* c, uint32_t nsteps) {
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   const uint64_t n = c->n;
-  const SolveCtl S{-1.0f, 0, nullptr, c->track_lsum ? c->lsum : nullptr, nullptr};
+  const SolveCtl S{-1.0f, 0, 0, nullptr, nullptr, c->track_lsum ? c->lsum : nullptr, nullptr};
   for (uint32_t st = 0; st < nsteps; ++st) {
     launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
     launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
@@ -574,10 +587,12 @@ int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm
   if (!(tolerance >= 0.0f)) return set_error(PBF_ERR_ARG, "tolerance must be >= 0");
   if (max_iterations < 1 || max_iterations > kMaxSolveSlots)
     return set_error(PBF_ERR_ARG, "max_iterations must be in [1, PBF_MAX_SOLVE_ITERATIONS]");
-  if (warm_start < PBF_WARM_NONE || warm_start > PBF_WARM_PREVIOUS)
+  if (warm_start < PBF_WARM_NONE || warm_start > PBF_WARM_LEARNED)
     return set_error(PBF_ERR_ARG, "unknown warm start");
   if (warm_start == PBF_WARM_GIVEN && !c->has_given)
     return set_error(PBF_ERR_STATE, "PBF_WARM_GIVEN without pbf_set_warm_lambdas");
+  if (warm_start == PBF_WARM_LEARNED && !c->has_model)
+    return set_error(PBF_ERR_STATE, "PBF_WARM_LEARNED without pbf_set_warm_model");
   const uint64_t n = c->n;
   const int warm = warm_start != PBF_WARM_NONE;
   const bool had_lsum = c->track_lsum;
@@ -586,29 +601,44 @@ int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm
   launch_fill_u64(c->counters + C_ITERS, (unsigned long long)max_iterations, 1, c->stream);
   launch_fill_u64(c->counters + C_WARM_KEPT, (unsigned long long)warm, 1, c->stream);
   launch_fill_u32(c->ehist, 0u, (uint64_t)max_iterations + 1, c->stream);
+  launch_fill_u64(c->esum, 0ull, (uint64_t)max_iterations + 1, c->stream);
   launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
-  find_neighbors_impl(c, n, 0, n);
-  if (warm_start == PBF_WARM_PREVIOUS) {
-    if (had_lsum) launch_copy_f1(c->lsum, c->wlam, n, c->stream);
-    else launch_fill_u32((uint32_t*)c->wlam, 0u, n, c->stream);
-  } else if (warm_start == PBF_WARM_GIVEN) {
-    launch_scatter_in_f1(c, c->wgiven, c->wlam);
-  }
+  find_neighbors_impl(c, n, 0, n);  // carries the previous sums to the new order
+  // lsuma: the previous sums; lsum: this solve's
+  std::swap(c->lsum, c->lsuma);
+  if (!had_lsum) launch_fill_u32((uint32_t*)c->lsuma, 0u, n, c->stream);
   launch_fill_u32((uint32_t*)c->lsum, 0u, n, c->stream);
-  const SolveCtl S{tolerance, warm, c->ehist, c->lsum, c->xsave};
+  if (warm_start == PBF_WARM_PREVIOUS) launch_copy_f1(c->lsuma, c->wlam, n, c->stream);
+  else if (warm_start == PBF_WARM_GIVEN) launch_scatter_in_f1(c, c->wgiven, c->wlam);
+  const int mean = (c->P.options & PBF_OPT_SOLVE_MEAN_ERROR) ? 1 : 0;
+  const SolveCtl S{tolerance, warm, mean, c->ehist, c->esum, c->lsum, c->xsave};
   if (warm) {
-    // first density pass (its error is the reference for the fallback), then the warm update
+    // first density pass (its error is the reference for the fallback; it
+    // writes the features), then the warm update
     ++c->dirty_stamp;
-    const SolveCtl S0{-1.0f, 0, c->ehist, nullptr, nullptr};
-    launch_density_lambda_masked(c, 1, 0, S0);
+    const SolveCtl S0{-1.0f, 0, mean, c->ehist, c->esum, nullptr, nullptr};
+    launch_density_lambda_masked(c, 1, 0, S0, FeatCtl{c->feat, c->lsuma});
+    c->feat_valid = true;
+    if (warm_start == PBF_WARM_LEARNED) launch_mlp_warm(c);
     launch_warm_dp(c);
   }
+  // Slots are queued in chunks; between chunks the host reads the done flag
+  // and stops queueing once the solve has ended (the kernels of a finished
+  // solve would return at once, but each launch still costs its dispatch).
+  constexpr uint32_t kChunk = 8;
   for (uint32_t it = 0; it < max_iterations; ++it) {
     ++c->dirty_stamp;
     launch_density_lambda_masked(c, (int)it, (int)it + 1, S);
     launch_mark_dirty(c, (int)it, S);
     launch_lambda_repair(c, (int)it, S);
     launch_dp_update_masked(c, (int)it, 0, S);
+    if ((it + 1) % kChunk == 0 && it + 1 < max_iterations) {
+      int rc = check_hip(hipMemcpyAsync(c->done_host, c->counters + C_DONE, sizeof(unsigned long long),
+                                        hipMemcpyDeviceToHost, c->stream), "pbf_solve");
+      if (!rc) rc = check_hip(hipStreamSynchronize(c->stream), "pbf_solve");
+      if (rc) return rc;
+      if (*c->done_host) break;
+    }
   }
   if (warm) launch_add_warm(c);
   launch_update_velocities(c, 0, n);
@@ -616,6 +646,21 @@ int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm
   c->last_was_solve = true;
   step_tail(c);
   return check_hip(hipGetLastError(), "pbf_solve");
+}
+
+int pbf_get_warm_features(pbf_context* c, float* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  if (!c->feat_valid) return set_error(PBF_ERR_STATE, "no warm-started pbf_solve has run");
+  if (!c->n) return PBF_OK;
+  float* tmp = nullptr;
+  int rc = check_hip(hipMallocAsync((void**)&tmp, c->n * PBF_WARM_FEATURES * sizeof(float), c->stream), "alloc");
+  if (rc) return rc;
+  launch_gather_out_rows(c, c->feat, tmp, PBF_WARM_FEATURES);
+  rc = check_hip(hipMemcpyAsync(out, tmp, c->n * PBF_WARM_FEATURES * sizeof(float), hipMemcpyDeviceToHost,
+                                c->stream), "download");
+  hipFreeAsync(tmp, c->stream);
+  if (rc) return rc;
+  return check_hip(hipStreamSynchronize(c->stream), "download");
 }
 
 int pbf_set_warm_lambdas(pbf_context* c, const float* lambdas) {
@@ -627,6 +672,19 @@ int pbf_set_warm_lambdas(pbf_context* c, const float* lambdas) {
     if ((rc = check_hip(hipStreamSynchronize(c->stream), "upload"))) return rc;
   }
   c->has_given = true;
+  return PBF_OK;
+}
+
+int pbf_set_lambda_sums(pbf_context* c, const float* in) {
+  if (!c || !in) return set_error(PBF_ERR_ARG, "null argument");
+  if (c->n) {
+    int rc = check_hip(hipMemcpyAsync(c->stage, in, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream),
+                       "upload");
+    if (rc) return rc;
+    launch_scatter_in_f1(c, c->stage, c->lsum);
+    if ((rc = check_hip(hipStreamSynchronize(c->stream), "upload"))) return rc;
+  }
+  c->track_lsum = true;
   return PBF_OK;
 }
 
@@ -645,17 +703,26 @@ int pbf_get_solve_info(pbf_context* c, pbf_solve_info* out) {
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
   if (!c->last_was_solve) return set_error(PBF_ERR_STATE, "the last step was not a pbf_solve");
   unsigned long long h[C_NUM];
+  static_assert(kMaxSolveSlots + 1 <= 1024, "history size");
   uint32_t e[kMaxSolveSlots + 1];
+  unsigned long long es[kMaxSolveSlots + 1];
   int rc = check_hip(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream), "solve info");
   if (rc) return rc;
-  if ((rc = check_hip(hipMemcpyAsync(e, c->ehist, sizeof(e), hipMemcpyDeviceToHost, c->stream), "solve info")))
+  if ((rc = check_hip(hipMemcpyAsync(e, c->ehist, sizeof(e), hipMemcpyDeviceToHost, c->stream), "solve info")) ||
+      (rc = check_hip(hipMemcpyAsync(es, c->esum, sizeof(es), hipMemcpyDeviceToHost, c->stream), "solve info")))
     return rc;
   if ((rc = check_hip(hipStreamSynchronize(c->stream), "solve info"))) return rc;
   std::memset(out, 0, sizeof(*out));
   out->iterations = (uint32_t)h[C_ITERS];
   out->warm_start_kept = (int32_t)h[C_WARM_KEPT];
-  std::memcpy(&out->error_before_warm_start, &e[0], 4);
-  for (uint32_t k = 0; k < out->iterations && k < kMaxSolveSlots; ++k) std::memcpy(&out->error_history[k], &e[k + 1], 4);
+  const double n = (double)(c->n ? c->n : 1);
+  auto mean = [&](unsigned long long s) { return (float)((double)s / 4294967296.0 / n); };
+  std::memcpy(&out->max_error_before_warm_start, &e[0], 4);
+  out->mean_error_before_warm_start = mean(es[0]);
+  for (uint32_t k = 0; k < out->iterations && k < kMaxSolveSlots; ++k) {
+    std::memcpy(&out->max_error_history[k], &e[k + 1], 4);
+    out->mean_error_history[k] = mean(es[k + 1]);
+  }
   return PBF_OK;
 }
 

@@ -54,10 +54,15 @@ __device__ __forceinline__ void flush_compact(const int32_t* __restrict__ s_list
   for (int k = 0; k < nc; ++k) cnb[(uint64_t)k * n + p] = s_list[k * kStageBlock + lane];
 }
 
+// x limited to [lo, hi] (NaN gives lo)
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return x > lo ? (x < hi ? x : hi) : lo; }
+
 // ---------------------------------------------------------------------------
 // compute_densities (particles.h:292-313) + compute_pressure_lambdas (:318-342)
-// xs -> xt, rho, lam; compact row of the entries within h.
-// Returns this particle's density error max(rho/rho0 - 1, 0).
+// xs -> xt, rho, lam; compact row of the entries within h.  With FEAT, also
+// the learned warm start's features of the particle (FeatCtl).
+// Returns the particle's density error max(rho/rho0 - 1, 0).
+template <bool FEAT>
 __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int it, const KParams& K,
                                                     int32_t* __restrict__ s_list, uint32_t lane,
                                                     const float4* __restrict__ xs, const float4* __restrict__ v,
@@ -66,7 +71,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
                                                     const int32_t* __restrict__ nbr_count,
                                                     int32_t* __restrict__ cnb, int32_t* __restrict__ cnb_count,
                                                     uint32_t* __restrict__ moved,
-                                                    unsigned long long* __restrict__ counters) {
+                                                    unsigned long long* __restrict__ counters, const FeatCtl& F) {
   f3 xp = ld3(xs, p);
   const int cnt = nbr_count[p];
   // phase 1: compact row of the entries within h; stop at a near-coincident one
@@ -96,6 +101,9 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   float sum_C = 0.0f;
   f3 sg = {0.f, 0.f, 0.f};
   uint32_t hacks = 0;
+  int within = nc;
+  float dy_sum = 0.0f, div = 0.0f;
+  const f3 vp = FEAT ? ld3(v, p) : f3{0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < nc; k0 += kBatch) {
     uint32_t q[kBatch];
     load_batch_lds(s_list, lane, k0, nc, p, q);
@@ -111,12 +119,18 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
       const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
       dec3(sg, grad);
       sum_C += magsq3(grad);
+      if (FEAT) {
+        dy_sum -= d.y;
+        const f3 dv = sub3(vp, ld3(v, q[u]));
+        div -= (dv.x * grad.x + dv.y * grad.y) + dv.z * grad.z;
+      }
     }
   }
   if (hack_at >= 0) {
     // particles.h:304-305: move x*_p, then finish this particle with a plain walk
     for (int k = hack_at; k < cnt; ++k) {
-      const f3 xq = ld3(xs, (uint32_t)nbr[(uint64_t)k * n + p]);
+      const uint32_t qk = (uint32_t)nbr[(uint64_t)k * n + p];
+      const f3 xq = ld3(xs, qk);
       f3 d = sub3(xp, xq);
       float d2 = magsq3(d);
       if (!within_h(d2, K)) continue;
@@ -132,6 +146,12 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
       const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
       dec3(sg, grad);
       sum_C += magsq3(grad);
+      if (FEAT) {
+        ++within;
+        dy_sum -= d.y;
+        const f3 dv = sub3(vp, ld3(v, qk));
+        div -= (dv.x * grad.x + dv.y * grad.y) + dv.z * grad.z;
+      }
     }
   }
   rho[p] = density;
@@ -139,27 +159,49 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   sum_C += magsq3(sg);
   const float constraint = density / K.rho0 - 1.0f;
   const float C_p = ((double)constraint < 0.0) ? 0.0f : constraint;
-  lam[p] = -C_p / (sum_C + K.eps);
+  const float lp = -C_p / (sum_C + K.eps);
+  lam[p] = lp;
   if (hacks) {
     atomicAdd(&counters[C_HACKS], (unsigned long long)hacks);
     moved[atomicAdd(&counters[C_MOVED_N + (it & 1)], 1ull)] = p;
     atomicExch(&counters[C_HACK_FLAG + (it & 1)], 1ull);
   }
+  if (FEAT) {
+    const float inv_s = 1.0f / kWarmLambdaScale;
+    const float dyn = dy_sum / (float)(within > 0 ? within : 1);
+    float4* o = (float4*)(F.feat + (uint64_t)p * 8);
+    o[0] = make_float4(clampf(constraint, -1.0f, 4.0f), clampf(lp * inv_s, -64.0f, 0.0f),
+                       clampf(F.lprev[p] * inv_s, -64.0f, 0.0f), clampf(sum_C * 1e-5f, 0.0f, 8.0f));
+    o[1] = make_float4(clampf((float)within / 32.0f, 0.0f, 4.0f), clampf(vp.y * K.dt / K.h, -4.0f, 4.0f),
+                       clampf(dyn / K.h, -2.0f, 2.0f), clampf(div * K.mass * K.dt, -4.0f, 4.0f));
+  }
   return constraint > 0.0f ? constraint : 0.0f;
 }
 
-// Max over the wavefront; every lane of the wavefront must call it.
+// Max and sum over the wavefront; every lane of the wavefront must call them.
 __device__ __forceinline__ float wave_max(float e) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) e = fmaxf(e, __shfl_xor(e, d, 64));
   return e;
 }
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long e) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) e += __shfl_xor(e, d, 64);
+  return e;
+}
+// a particle's error in 2^-32 units (errors above 2^31 saturate)
+__device__ __forceinline__ unsigned long long fixed_error(float e) {
+  return (unsigned long long)(fminf(e, 2147483648.0f) * 4294967296.0f);
+}
 
 // One block = one wavefront.  With S.ehist, the block's largest density
 // error goes into slot eslot (atomicMax on the bits of a non-negative float
-// orders like the float).  With S.tol >= 0 nothing runs once the solve is done.
+// orders like the float) and its fixed-point sum into S.esum[eslot].  With
+// S.tol >= 0 nothing runs once the solve is done.
+template <bool FEAT>
 __global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S,
-                                                                  KParams K, const float4* __restrict__ xs,
+                                                                  FeatCtl F, KParams K,
+                                                                  const float4* __restrict__ xs,
                                                                   const float4* __restrict__ v,
                                                                   float4* __restrict__ xt, float* __restrict__ rho,
                                                                   float* __restrict__ lam,
@@ -175,11 +217,15 @@ __global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, in
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   float e = 0.0f;
   if (p < n)
-    e = density_lambda_one(p, n, it, K, s_list, lane, xs, v, xt, rho, lam, nbr, nbr_count, cnb, cnb_count, moved,
-                           counters);
+    e = density_lambda_one<FEAT>(p, n, it, K, s_list, lane, xs, v, xt, rho, lam, nbr, nbr_count, cnb, cnb_count,
+                                 moved, counters, F);
   if (S.ehist) {
+    const unsigned long long es = wave_sum(fixed_error(e));
     e = wave_max(e);
-    if (lane == 0 && e > 0.0f) atomicMax(&S.ehist[eslot], __float_as_uint(e));
+    if (lane == 0) {
+      if (e > 0.0f) atomicMax(&S.ehist[eslot], __float_as_uint(e));
+      if (es) atomicAdd(&S.esum[eslot], es);
+    }
   }
 }
 
@@ -269,6 +315,10 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
   lam[p] = -C_p / (sum_C + K.eps);
 }
 
+__device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) {
+  return (float)((double)sum / 4294967296.0 / (double)n);
+}
+
 // compute_pressure_dps (particles.h:348-358) + update_position_stars
 // (:363-370); on the last iteration also update_velocities (:375-385).
 // In an iterate-to-tolerance solve, slot `it` first checks the error its
@@ -295,8 +345,9 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
   }
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (S.ehist) {
-    const float e = __uint_as_float(S.ehist[it + 1]);
-    if (S.warm && it == 0 && e > __uint_as_float(S.ehist[0])) {
+    const float e = S.mean ? mean_error(S.esum[it + 1], n) : __uint_as_float(S.ehist[it + 1]);
+    const float e_pre = S.mean ? mean_error(S.esum[0], n) : __uint_as_float(S.ehist[0]);
+    if (S.warm && it == 0 && e > e_pre) {
       if (first) counters[C_WARM_KEPT] = 0ull;
       if (p < n) xs[p] = S.xsave[p];
       return;
@@ -572,10 +623,15 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     timer_end((c), t_);                                                                                     \
   } while (0)
 
-void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S) {
-  PBF_LAUNCH_STAGED(c, "density_lambda", k_density_lambda_c, it, eslot, S, c->K, (const float4*)c->xs,
-                    (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr,
-                    (const int32_t*)c->nbr_count, c->cnb, c->cnb_count, c->moved, c->counters);
+void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S, const FeatCtl& F) {
+  if (F.feat)
+    PBF_LAUNCH_STAGED(c, "density_features", k_density_lambda_c<true>, it, eslot, S, F, c->K,
+                      (const float4*)c->xs, (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr,
+                      (const int32_t*)c->nbr_count, c->cnb, c->cnb_count, c->moved, c->counters);
+  else
+    PBF_LAUNCH_STAGED(c, "density_lambda", k_density_lambda_c<false>, it, eslot, S, F, c->K,
+                      (const float4*)c->xs, (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr,
+                      (const int32_t*)c->nbr_count, c->cnb, c->cnb_count, c->moved, c->counters);
 }
 
 void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S) {

@@ -424,7 +424,9 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
                      (uint32_t)(kFillBlock * c->K.cap * sizeof(int32_t)), c->stream, n, (uint32_t)fb,
                      (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
                      c->K, c->nbr, c->nbr_count, c->heavy, c->counters);
+  timer_end(c, t);
   // enough waves to cover the chip; each loops over the heavy list (usually empty)
+  t = timer_begin(c, "fill_heavy");
   const uint32_t blocks = std::min<uint32_t>(nblocks((uint64_t)n * 64), 2048u);
   hipLaunchKernelGGL(k_fill_heavy, dim3(blocks), dim3(kBlock), 0, c->stream, n, (const float4*)c->xs,
                      (const uint32_t*)c->cell_start, c->K, c->nbr, c->nbr_count, (const uint32_t*)c->heavy,

@@ -76,15 +76,24 @@ enum Counter {
 };
 
 constexpr uint32_t kMaxSolveSlots = PBF_MAX_SOLVE_ITERATIONS;
+constexpr float kWarmLambdaScale = PBF_WARM_LAMBDA_SCALE;
 
 // Per-launch control of the Jacobi loop kernels.  A plain fixed-iteration
 // step passes tol < 0 and null pointers.
 struct SolveCtl {
-  float tol;             // >= 0: stop once the slot's max density error is <= tol
-  int warm;              // a warm-start update was applied before slot 0
-  uint32_t* ehist;       // [0] error before the warm start, [k+1] error measured in slot k
-  float* lsum;           // sum of the applied multipliers (null: not kept)
-  const float4* xsave;   // position stars to restore when the warm start is rejected
+  float tol;                   // >= 0: stop once the slot's density error is <= tol
+  int warm;                    // a warm-start update was applied before slot 0
+  int mean;                    // compare the mean error (else the maximum)
+  uint32_t* ehist;             // max error as float bits: [0] before the warm start, [k+1] slot k
+  unsigned long long* esum;    // sum of the errors in 2^-32 units, same slots
+  float* lsum;                 // sum of the applied multipliers (null: not kept)
+  const float4* xsave;         // position stars to restore when the warm start is rejected
+};
+
+// Features for the learned warm start (written by the first density pass)
+struct FeatCtl {
+  float* feat;          // [n][8] (null: no features)
+  const float* lprev;   // previous multiplier sums
 };
 
 struct KernelTimer {
@@ -131,6 +140,13 @@ struct pbf_context {
   int32_t *cnb = nullptr, *cnb_count = nullptr;
   unsigned long long* counters = nullptr;
   uint32_t* ehist = nullptr;
+  unsigned long long* esum = nullptr;
+  float* feat = nullptr;       // [n][8] features of the last warm-started solve
+  bool feat_valid = false;
+  short *mw1 = nullptr, *mw2 = nullptr;  // learned warm start: bf16 weights
+  float* mp = nullptr;                   // and the fp32 parameters
+  bool has_model = false;
+  unsigned long long* done_host = nullptr;  // pinned copy of C_DONE, polled between slot chunks
   float *lsum = nullptr, *lsuma = nullptr, *wlam = nullptr, *wgiven = nullptr;
   float4* xsave = nullptr;
   bool has_given = false;
@@ -174,7 +190,8 @@ enum { F_FORCES = 1, F_PREDICT = 2 };
 void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e);
 void launch_density(pbf_context* c, uint64_t b, uint64_t e);  // xs -> xt, rho
 // ---- fused-step launchers (pbf_fused.hip): they build and use the compact lists ----
-void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S);  // xs -> xt, rho, lam
+void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S,
+                                  const FeatCtl& F = FeatCtl{nullptr, nullptr});  // xs -> xt, rho, lam
 void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S);     // particles next to a moved one
 void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S);  // lam from xt for dirty particles
 void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S);
@@ -196,9 +213,13 @@ void launch_apply_vorticity(pbf_context* c, float4* vio, uint64_t b, uint64_t e)
 void launch_surface_tension(pbf_context* c, uint64_t cb, uint64_t ce, uint64_t b, uint64_t e);
 void launch_copy_f4(const float4* src, float4* dst, uint64_t n, hipStream_t s);
 void launch_copy_f1(const float* src, float* dst, uint64_t n, hipStream_t s);
+void launch_scale_f1(const float* src, float* dst, float scale, uint64_t n, hipStream_t s);
+// ---- learned warm start (pbf_mlp.hip) ----
+void launch_mlp_warm(pbf_context* c);  // feat -> wlam
 // original-order transfer helpers
 void launch_gather_out_f4(pbf_context* c, const float4* src, float* dst3, int comps);
 void launch_gather_out_f1(pbf_context* c, const float* src, float* dst);
+void launch_gather_out_rows(pbf_context* c, const float* src, float* dst, int width);  // [n][width] rows
 void launch_scatter_in_f4(pbf_context* c, const float* src3, float4* dst);
 void launch_scatter_in_f1(pbf_context* c, const float* src, float* dst);
 

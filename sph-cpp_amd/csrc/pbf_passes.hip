@@ -428,6 +428,13 @@ __global__ void k_copy_f1(const float* __restrict__ src, float* __restrict__ dst
 void launch_copy_f1(const float* src, float* dst, uint64_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_copy_f1, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, n);
 }
+__global__ void k_scale_f1(const float* __restrict__ src, float* __restrict__ dst, float scale, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i] * scale;
+}
+void launch_scale_f1(const float* src, float* dst, float scale, uint64_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_scale_f1, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, scale, n);
+}
 
 // ---- original-order transfers through the staging buffer ----
 __global__ void k_gather_out_f4(uint32_t n, const uint32_t* __restrict__ oid, const float4* __restrict__ src,
@@ -464,6 +471,18 @@ void launch_gather_out_f4(pbf_context* c, const float4* src, float* dst3, int co
   const uint32_t n = (uint32_t)c->n;
   if (n) hipLaunchKernelGGL(k_gather_out_f4, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->oid,
                             src, dst3, comps);
+}
+__global__ void k_gather_out_rows(uint32_t n, const uint32_t* __restrict__ oid, const float* __restrict__ src,
+                                  float* __restrict__ dst, int width) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint64_t o = oid[s];
+  for (int k = 0; k < width; ++k) dst[o * width + k] = src[(uint64_t)s * width + k];
+}
+void launch_gather_out_rows(pbf_context* c, const float* src, float* dst, int width) {
+  const uint32_t n = (uint32_t)c->n;
+  if (n) hipLaunchKernelGGL(k_gather_out_rows, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n,
+                            (const uint32_t*)c->oid, src, dst, width);
 }
 void launch_gather_out_f1(pbf_context* c, const float* src, float* dst) {
   const uint32_t n = (uint32_t)c->n;

@@ -72,9 +72,9 @@ def jitter(pos: np.ndarray, seed: int, amplitude: float = JITTER) -> np.ndarray:
     return (pos + j).astype(np.float32)
 
 
-def dam_break(counts=(32, 32, 32), box=(3.2, 1.6, 1.6), seed=0, capacity=None, **kw):
+def dam_break(counts=(32, 32, 32), box=(3.2, 1.6, 1.6), seed=0, capacity=None, jitter_amplitude=JITTER, **kw):
     """Single dam break: a lattice block at the origin corner of the box."""
-    pos = jitter(lattice(counts), seed)
+    pos = jitter(lattice(counts), seed, jitter_amplitude)
     n = pos.shape[0]
     params = pbf_params(box, capacity or n, **kw)
     return params, pos, np.zeros_like(pos)

@@ -1,0 +1,116 @@
+"""Train the learned warm start's network on the GPU.
+
+Training pairs come from randomised dam breaks (seeded block sizes at box
+scales 1, 2 and 4 of the 3.2 x 1.6 x 1.6 box, jitter uniform in
+[0, 0.05 h]).  Each scene advances with the solves the warm start serves:
+iterate to a mean density error of ``tol``, warm-started from the previous
+substep's multiplier sums.  After every ``every``-th substep whose warm start
+was kept, the features of that solve's first density pass become samples,
+with target the change of the multiplier sum over the substep,
+(Lambda - Lambda_prev) / S: the network learns to extrapolate the previous
+sums to the current substep.  It is trained with the library's own MFMA
+forward/backward and Adam kernels (pbf_warm_model_train).
+
+    python sph-cpp_amd/train_warm.py [--pairs 2000000] [--epochs 40] [--out sph-cpp_amd/warm_model.npy]
+
+Prints one JSON line per scene and a summary; writes the 4801 parameters
+(float32, numpy .npy).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+pbf = importlib.import_module("sph-cpp_amd")
+scenes = importlib.import_module("sph-cpp_amd.scenes")
+
+MODEL_PATH = os.path.join(HERE, "warm_model.npy")
+
+
+def random_scene(rng, cap=60):
+    scale = int(rng.choice([1, 2, 4]))
+    box = (3.2 * scale, 1.6 * scale, 1.6 * scale)
+    counts = (int(rng.integers(16, 49)) * scale, int(rng.integers(16, 33)) * scale, int(rng.integers(12, 33)) * scale)
+    amp = float(rng.uniform(0.0, 0.05 * scenes.H))
+    seed = int(rng.integers(1 << 31))
+    params, pos, vel = scenes.dam_break(counts, box, seed=seed, cap=cap, jitter_amplitude=amp)
+    params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
+    return params, pos, vel, {"scale": scale, "counts": counts, "jitter": round(amp, 6), "seed": seed}
+
+
+def generate(model, pairs, seed=0, steps=200, cap=60, tol=0.01, max_slots=128, per_sample=10_000,
+             scenes_min=8, log=print):
+    """Each scene contributes at most pairs / scenes_min samples, each sampled
+    substep at most per_sample (a stride over the cell order), and the
+    sampled substeps are spread over the whole trajectory."""
+    rng = np.random.default_rng(seed)
+    k = 0
+    t0 = time.time()
+    budget = pairs // scenes_min + 1
+    while model.sample_count() < pairs:
+        start = model.sample_count()
+        params, pos, vel, info = random_scene(rng, cap)
+        t = pbf.Particles(params)
+        t.add(pos, pos, vel)
+        log(json.dumps({"scene": k, **info, "particles": len(pos)}))
+        stride = max(1, len(pos) // per_sample)
+        per_substep = (len(pos) + stride - 1) // stride
+        every = max(2, steps * per_substep // budget)
+        kept = 0
+        for step in range(1, steps + 1):
+            t.solve(tol, max_slots, pbf.PBF_WARM_PREVIOUS if step > 1 else pbf.PBF_WARM_NONE)
+            if step > 1 and step % every == 0 and t.solve_info()["warm_start_kept"]:
+                model.add_from(t, stride)
+                kept += 1
+                if model.sample_count() >= pairs or model.sample_count() - start >= budget:
+                    break
+        t.close()
+        log(json.dumps({"scene": k, "sampled_substeps": kept, "samples": model.sample_count(),
+                        "seconds": round(time.time() - t0, 1)}))
+        k += 1
+
+
+def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print):
+    model = pbf.WarmModel(pairs + 2 * 40_000)
+    t0 = time.time()
+    generate(model, pairs, seed, log=log)
+    t1 = time.time()
+    model.init(seed)
+    n = model.sample_count()
+    _, l0 = model.gradient(0, min(n, 200_000))
+    half = epochs // 2
+    m1 = model.train(half, batch, lr, seed + 1)
+    m2 = model.train(epochs - half, batch, lr / 3, seed + 2)
+    _, l1 = model.gradient(0, min(n, 200_000))
+    t2 = time.time()
+    summary = {"samples": n, "epochs": epochs, "batch": batch, "loss_before": l0, "loss_after": l1,
+               "mean_batch_loss": [m1, m2], "data_seconds": round(t1 - t0, 1), "train_seconds": round(t2 - t1, 1)}
+    return model, summary
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pairs", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=MODEL_PATH)
+    a = ap.parse_args(argv)
+    model, summary = train(a.pairs, a.epochs, a.batch, a.lr, a.seed, log=lambda s: print(s, flush=True))
+    np.save(a.out, model.params())
+    print(json.dumps({"out": a.out, **summary}), flush=True)
+    model.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -67,7 +67,7 @@ def lib():
         L.orc_bin_id.argtypes = [P, D]
         L.orc_bin_id.restype = C.c_uint64
         L.orc_boundary.argtypes = [P, D]
-        L.orc_solve.argtypes = [P, C.c_float, C.c_int, F, F, F, C.POINTER(C.c_int)]
+        L.orc_solve.argtypes = [P, C.c_float, C.c_int, C.c_int, F, F, F, F, F, C.POINTER(C.c_int)]
         L.orc_solve.restype = C.c_int
         _lib = L
     return _lib
@@ -192,19 +192,27 @@ class Oracle:
     def emax(self):
         return float(self.L.orc_emax(self.h))
 
-    def solve(self, tol, max_iter, warm=None):
-        """Iterate-to-tolerance step; returns (slots used, e_max history,
-        cumulative multipliers, warm start kept)."""
+    def solve(self, tol, max_iter, warm=None, mean=False, features=False):
+        """Iterate-to-tolerance step.  Returns (slots used, max-error history,
+        cumulative multipliers, warm start kept); with mean=True the mean
+        error is compared and the history is the mean; with features=True a
+        fifth item holds the warm-start features (n x 8, warm starts only)."""
         F = C.POINTER(C.c_float)
-        hist = np.zeros(max_iter, np.float32)
+        hmax = np.zeros(max_iter, np.float32)
+        hmean = np.zeros(max_iter, np.float32)
         lsum = np.zeros(self.n, np.float32)
+        feat = np.zeros((self.n, 8), np.float32)
         kept = C.c_int(0)
         w = None if warm is None else np.ascontiguousarray(warm, dtype=np.float32)
-        used = self.L.orc_solve(self.h, float(tol), int(max_iter), None if w is None else w.ctypes.data_as(F),
-                                hist.ctypes.data_as(F), lsum.ctypes.data_as(F), C.byref(kept))
+        used = self.L.orc_solve(self.h, float(tol), int(max_iter), int(bool(mean)),
+                                None if w is None else w.ctypes.data_as(F), hmax.ctypes.data_as(F),
+                                hmean.ctypes.data_as(F), lsum.ctypes.data_as(F), feat.ctypes.data_as(F),
+                                C.byref(kept))
         if used < 0:
             raise RuntimeError("oracle solve failed")
-        return used, hist[:used], lsum, bool(kept.value)
+        self.last_solve = {"max": hmax[:used], "mean": hmean[:used]}
+        out = (used, (hmean if mean else hmax)[:used], lsum, bool(kept.value))
+        return out + (feat,) if features else out
 
     def poly6(self, r):
         return float(self.L.orc_poly6(self.h, float(r)))

@@ -45,7 +45,7 @@ def _check(g, o, used, hist, lsum, kept, what):
     print(f"{what}: slots {info['iterations']} (oracle {used}), kept {info['warm_start_kept']}", flush=True)
     assert info["iterations"] == used, what
     assert info["warm_start_kept"] == kept, what
-    assert np.array_equal(info["error_history"], hist), what
+    assert np.array_equal(info["max_error_history"], hist), what
     st = o.state()
     assert np.array_equal(g.positions(), st["x"].astype(np.float32)), what + ": positions"
     assert np.array_equal(g.velocities(), st["v"].astype(np.float32)), what + ": velocities"
@@ -94,7 +94,7 @@ def test_warm_start_given(pbf, scenes):
     g.set_warm_lambdas(lam40)
     g.solve(0.0, 6, pbf.PBF_WARM_GIVEN)
     info = g.solve_info()
-    assert np.float32(info["error_before_warm_start"]) > info["error_history"][0]
+    assert np.float32(info["max_error_before_warm_start"]) > info["max_error_history"][0]
     _check(g, o, used, hist, lsum, kept, "warm given")
     g.close(); o.close()
 

@@ -1,0 +1,215 @@
+"""GPU checks of the learned warm start.
+
+The network's kernels (BF16 MFMA, FP32 accumulation) are compared with the
+numpy restatement in oracle/warm_mlp.py, which rounds the same operands to
+bfloat16: outputs within 2e-3 of the output scale, gradients within 1%
+(relative L2), Adam updates within float32 rounding.  The features come from
+the solve's first density pass and must equal the CPU oracle's bit for bit;
+a PBF_WARM_LEARNED solve must equal a PBF_WARM_GIVEN solve fed the same
+network outputs, bit for bit.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import warm_mlp as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, 8)).astype(np.float32)
+    y = (np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2] - 0.3 * X[:, 3] ** 2 + 0.2 * X[:, 7]).astype(np.float32)
+    return X, y
+
+
+def test_forward_matches_oracle(pbf):
+    m = pbf.WarmModel(4096)
+    m.init(7)
+    p = m.params()
+    X, _ = _data(1001, 1)
+    out = m.predict(X)
+    ref, _ = M.forward(p, X, rounding=True)
+    err = np.abs(out.astype(np.float64) - ref).max()
+    print(f"forward: max |gpu - oracle| {err:.2e}, output scale {np.abs(ref).max():.2e}", flush=True)
+    assert err <= 2e-3 * max(1.0, np.abs(ref).max())
+    # one row's output does not depend on the rows sharing its tile
+    assert m.predict(X[5:6])[0] == out[5]
+    m.close()
+
+
+@pytest.mark.parametrize("count", [1000, 20000])
+def test_gradient_matches_oracle(pbf, count):
+    m = pbf.WarmModel(count)
+    m.init(11)
+    p = m.params()
+    X, y = _data(count, 2)
+    m.add_samples(X, y)
+    g, loss = m.gradient(0, count)
+    gref, lref = M.gradient(p, X, y, rounding=True)
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print(f"gradient ({count} samples): relative L2 error {rel:.2e}, loss {loss:.5f} vs {lref:.5f}", flush=True)
+    assert rel <= 1e-2
+    assert abs(loss - lref) <= 1e-3 * lref
+    # each block of parameters on its own
+    for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2), (M.OFF_B2, M.OFF_W3),
+                 (M.OFF_W3, M.NPAR)):
+        assert np.linalg.norm(g[a:b] - gref[a:b]) <= 2e-2 * np.linalg.norm(gref[a:b]) + 1e-7
+    m.close()
+
+
+def test_adam_matches_oracle(pbf):
+    m = pbf.WarmModel(16)
+    m.init(3)
+    p0 = m.params()
+    rng = np.random.default_rng(4)
+    g1 = rng.normal(size=M.NPAR).astype(np.float32)
+    g2 = rng.normal(size=M.NPAR).astype(np.float32)
+    m.adam_step(g1, 1e-3)
+    z = np.zeros(M.NPAR, np.float32)
+    r1, mm, vv = M.adam(p0, g1, z, z, 1, 1e-3)
+    assert np.allclose(m.params(), r1, rtol=1e-6, atol=1e-9)
+    m.adam_step(g2, 1e-3)
+    r2, _, _ = M.adam(r1, g2, mm, vv, 2, 1e-3)
+    assert np.allclose(m.params(), r2, rtol=1e-6, atol=1e-9)
+    m.close()
+
+
+def test_training_reduces_loss_deterministically(pbf):
+    X, y = _data(16384, 5)
+    runs = []
+    for _ in range(2):
+        m = pbf.WarmModel(len(y))
+        m.init(1)
+        m.add_samples(X, y)
+        _, l0 = m.gradient(0, len(y))
+        mean = m.train(epochs=30, batch=1024, learning_rate=3e-3, seed=9)
+        _, l1 = m.gradient(0, len(y))
+        runs.append(m.params())
+        m.close()
+        print(f"training: loss {l0:.4f} -> {l1:.4f} (mean batch loss {mean:.4f})", flush=True)
+        assert l1 < 0.2 * l0
+    assert np.array_equal(runs[0], runs[1])
+
+
+def _state(scenes, steps=10):
+    params, pos, vel = scenes.config_c1(seed=6)
+    o = O.Oracle(O.params_from_pbf(params), mode=O.JACOBI, threads=THREADS)
+    o.set_state(pos, pos, vel)
+    o.step(steps)
+    st = o.state()
+    o.close()
+    return params, st["x"].astype(np.float32), st["v"].astype(np.float32)
+
+
+def test_features_match_oracle(pbf, scenes):
+    params, x, v = _state(scenes)
+    o = O.Oracle(O.params_from_pbf(params), mode=O.JACOBI, threads=THREADS)
+    o.set_state(x, x, v)
+    g = pbf.Particles(params)
+    g.add(x, x, v)
+    w = np.zeros(len(x), np.float32)
+    _, _, lsum, kept, feat = o.solve(0.0, 5, warm=w, features=True)
+    g.set_warm_lambdas(w)
+    g.solve(0.0, 5, pbf.PBF_WARM_GIVEN)
+    assert np.array_equal(g.warm_features(), feat), "features, first solve"
+    # the second solve's feature 2 is the first solve's multiplier sums
+    _, _, lsum2, kept2, feat2 = o.solve(0.0, 5, warm=lsum, features=True)
+    g.solve(0.0, 5, pbf.PBF_WARM_PREVIOUS)
+    assert np.array_equal(g.warm_features(), feat2), "features, second solve"
+    assert np.array_equal(feat2[:, 2], (lsum * np.float32(1e4)).astype(np.float32))
+    assert np.array_equal(g.lambda_sums(), lsum2)
+    g.close()
+    o.close()
+
+
+def test_learned_equals_given_prediction(pbf, scenes):
+    """Lambda = Lambda_prev + S * net(features), fed back as a given warm start."""
+    params, x, v = _state(scenes)
+    m = pbf.WarmModel(len(x))
+    m.init(5)
+    a = pbf.Particles(params)
+    a.add(x, x, v)
+    a.solve(0.0, 3)  # previous sums for the learned solve
+    prev = a.lambda_sums()
+    assert np.any(prev != 0)
+    b = pbf.Particles(params)
+    b.add(x, x, v)
+    b.solve(0.0, 3)
+    a.set_warm_model(m)
+    a.solve(0.0, 6, pbf.PBF_WARM_LEARNED)
+    warm = prev + np.float32(pbf.PBF_WARM_LAMBDA_SCALE) * m.predict(a.warm_features())
+    b.set_warm_lambdas(warm.astype(np.float32))
+    b.solve(0.0, 6, pbf.PBF_WARM_GIVEN)
+    ia, ib = a.solve_info(), b.solve_info()
+    assert ia["iterations"] == ib["iterations"] and ia["warm_start_kept"] == ib["warm_start_kept"]
+    assert np.array_equal(ia["max_error_history"], ib["max_error_history"])
+    for k in ("positions", "velocities", "lambda_sums"):
+        assert np.array_equal(getattr(a, k)(), getattr(b, k)()), k
+    a.close(); b.close(); m.close()
+
+
+def test_mean_error_criterion_matches_oracle(pbf, scenes):
+    params, x, v = _state(scenes)
+    params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
+    o = O.Oracle(O.params_from_pbf(params), mode=O.JACOBI, threads=THREADS)
+    o.set_state(x, x, v)
+    _, hist, _, _ = o.solve(0.0, 12, mean=True)
+    o.close()
+    tol = float(hist[4])
+    o = O.Oracle(O.params_from_pbf(params), mode=O.JACOBI, threads=THREADS)
+    o.set_state(x, x, v)
+    used, hist, lsum, kept = o.solve(tol, 12, mean=True)
+    g = pbf.Particles(params)
+    g.add(x, x, v)
+    g.solve(tol, 12)
+    info = g.solve_info()
+    print(f"mean criterion: slots {info['iterations']} (oracle {used})", flush=True)
+    assert info["iterations"] == used
+    assert np.array_equal(info["mean_error_history"], hist)
+    assert np.array_equal(info["max_error_history"], o.last_solve["max"])
+    assert np.array_equal(g.positions(), o.state()["x"].astype(np.float32))
+    g.close(); o.close()
+
+
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sph-cpp_amd", "warm_model.npy")
+
+
+def test_trained_warm_start_gain(pbf, scenes):
+    """What the trained network (sph-cpp_amd/warm_model.npy, made by
+    sph-cpp_amd/train_warm.py on the GPU) gains over the plain solve: a C1
+    dam break advanced 60 substeps, each iterated to a mean density error of
+    1%, needs at least 1.5x fewer Jacobi slots with the learned warm start."""
+    if not os.path.exists(MODEL):
+        pytest.skip("no trained model in the tree")
+    m = pbf.WarmModel(16)
+    m.set_params(np.load(MODEL, allow_pickle=False))
+    res = {}
+    for name, mode in (("plain", pbf.PBF_WARM_NONE), ("learned", pbf.PBF_WARM_LEARNED)):
+        params, pos, vel = scenes.config_c1(seed=11)
+        params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
+        g = pbf.Particles(params)
+        g.add(pos, pos, vel)
+        if mode:
+            g.set_warm_model(m)
+        slots, met = 0, 0
+        for _ in range(60):
+            g.solve(0.01, 128, mode)
+            info = g.solve_info()
+            slots += info["iterations"]
+            met += int(info["mean_error_history"][-1] <= np.float32(0.01))
+        res[name] = (slots, met)
+        g.close()
+    m.close()
+    print(f"slots over 60 substeps: plain {res['plain'][0]} ({res['plain'][1]} met), "
+          f"learned {res['learned'][0]} ({res['learned'][1]} met)", flush=True)
+    assert res["learned"][1] >= res["plain"][1]
+    assert res["learned"][0] * 1.5 <= res["plain"][0]

@@ -222,6 +222,30 @@ def summarize(d: str, workload: str) -> None:
 WARM_MODEL = os.path.join(ROOT, "sph-cpp_amd", "warm_model.npy")
 
 
+def batched_run(pbf, scenes, steps, seed, num_scenes=256):
+    """Independent dam breaks of 4,096..65,536 particles (log-uniform) in one
+    batched context: eager steps and graph-replayed steps, each from the
+    initial scenes after 2 warm-up steps."""
+    params, boxes, sc = scenes.batched_dam_breaks(num_scenes=num_scenes, seed=seed)
+    n = sum(p.shape[0] for p, _ in sc)
+    out = {"scenes": num_scenes, "particles": n, "steps": steps}
+    for name in ("eager", "graph"):
+        g = pbf.Particles(params, boxes=boxes)
+        for k, (pos, vel) in enumerate(sc):
+            g.add_to_scene(k, pos, pos, vel)
+        run = g.step_graph if name == "graph" else g.step
+        run(2)
+        g.sync()
+        t0 = time.perf_counter()
+        run(steps)
+        g.sync()
+        wall = time.perf_counter() - t0
+        out[name] = {"value": n * steps / wall, "ms_per_step": wall * 1e3 / steps}
+        g.close()
+        print(f"batched scenes: {name} done", file=sys.stderr, flush=True)
+    return out
+
+
 def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=128):
     """Plain vs warm-started iterate-to-tolerance solves on the same scene:
     each mode runs its own trajectory of `steps` substeps, every substep
@@ -379,6 +403,8 @@ def main(argv=None):
         out["variants"] = {"cap128": {"value": vr["value"], "ms_per_step": vr["seconds"] * 1e3 / args.steps,
                                       "stats_last_step": vr["stats"],
                                       "note": "same scene with a 128-entry neighbour list (no list saturation)"}}
+    if args.variants:
+        out["variants"]["batched_256_scenes"] = batched_run(pbf, scenes, 100, args.seed)
     if args.warm:
         out["warm_start"] = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps)
     if not args.no_cpu:

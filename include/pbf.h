@@ -166,6 +166,20 @@ int pbf_update_positions(pbf_context* ctx, pbf_index_span span);        /* parti
  * (resident = local = all particles), nsteps times, with fused kernels. */
 int pbf_step(pbf_context* ctx, uint32_t nsteps);
 
+/* ---- batched scenes (no counterpart in the reference) ----
+ * Independent scenes in one context and one set of launches.  Every parameter is shared
+ * except the box: scene s occupies [0, boxes[3s .. 3s+2]] in its own coordinates, with its
+ * own grid derived as for a single scene (neighbors.h:69-71).  A particle's cell key is its
+ * scene's first key plus the key in its scene's grid, so the sort keeps scenes apart and a
+ * neighbour walk never leaves its scene: every scene evolves exactly as it would alone
+ * (the max density error and the tolerance of pbf_solve are taken over all scenes).
+ * pbf_add adds to scene 0; pbf_grid_dims reports scene 0's grid; pbf_num_cells the total. */
+int pbf_create_batched(const pbf_params* params, uint32_t num_scenes, const float* boxes,
+                       pbf_context** out);
+int pbf_add_to_scene(pbf_context* ctx, uint32_t scene, const float* positions,
+                     const float* position_stars, const float* velocities, uint64_t count);
+uint32_t pbf_num_scenes(const pbf_context* ctx);
+
 /* ---- iterate-to-tolerance solve (no counterpart in the reference, which runs a
  * fixed solve_step_count of Jacobi iterations, Simulation/Source/main.cpp:82) ----
  * One step like pbf_step, except the pressure loop: iteration slot k runs the density
@@ -257,6 +271,12 @@ int pbf_warm_model_predict(pbf_warm_model* model, const float* features, uint64_
 int pbf_set_warm_model(pbf_context* ctx, pbf_warm_model* model);
 /* features of the last warm-started solve, n x 8, original order */
 int pbf_get_warm_features(pbf_context* ctx, float* out);
+
+/* pbf_step with the launches of two steps captured once into a HIP graph and replayed
+ * (an odd count ends with one ordinary step).  Same results as pbf_step.  The graph is
+ * captured again when the particle count, the parameters or the buffer roles changed
+ * (add, remove, set_params, per-pass calls). */
+int pbf_step_graph(pbf_context* ctx, uint32_t nsteps);
 
 /* Block until the context's queued work is done. */
 int pbf_sync(pbf_context* ctx);

@@ -156,6 +156,11 @@ def lib():
     _sig(L, "pbf_apply_surface_tension", [_P, IndexSpan, IndexSpan])
     _sig(L, "pbf_step", [_P, C.c_uint32])
     _sig(L, "pbf_solve", [_P, C.c_float, C.c_uint32, C.c_int])
+    _sig(L, "pbf_step_graph", [_P, C.c_uint32])
+    _sig(L, "pbf_create_batched", [C.POINTER(Params), C.c_uint32, _F, C.POINTER(_P)])
+    _sig(L, "pbf_add_to_scene", [_P, C.c_uint32, _F, _F, _F, C.c_uint64])
+    L.pbf_num_scenes.argtypes = [_P]
+    L.pbf_num_scenes.restype = C.c_uint32
     _sig(L, "pbf_set_warm_lambdas", [_P, _F])
     _sig(L, "pbf_get_lambda_sums", [_P, _F])
     _sig(L, "pbf_get_solve_info", [_P, C.POINTER(SolveInfo)])
@@ -222,11 +227,17 @@ def span(begin: int, end: int) -> IndexSpan:
 class Particles:
     """sim::Particles<float,3> on the device (particles.h:41-504)."""
 
-    def __init__(self, params: Params):
+    def __init__(self, params: Params, boxes=None):
+        """One domain; with ``boxes`` (num_scenes x 3 box extents) a batched
+        context of independent scenes (pbf_create_batched)."""
         L = lib()
         self._ctx = _P()
         self.params = params.copy()
-        _check(L.pbf_create(C.byref(self.params), C.byref(self._ctx)))
+        if boxes is None:
+            _check(L.pbf_create(C.byref(self.params), C.byref(self._ctx)))
+        else:
+            b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 3)
+            _check(L.pbf_create_batched(C.byref(self.params), b.shape[0], _fptr(b), C.byref(self._ctx)))
 
     def close(self) -> None:
         if self._ctx:
@@ -260,6 +271,15 @@ class Particles:
         xs = x if position_stars is None else np.ascontiguousarray(position_stars, dtype=np.float32).reshape(-1, 3)
         v = np.zeros_like(x) if velocities is None else np.ascontiguousarray(velocities, np.float32).reshape(-1, 3)
         _check(lib().pbf_add(self._ctx, _fptr(x), _fptr(xs), _fptr(v), x.shape[0]))
+
+    def add_to_scene(self, scene: int, positions, position_stars=None, velocities=None) -> None:
+        x = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        xs = x if position_stars is None else np.ascontiguousarray(position_stars, dtype=np.float32).reshape(-1, 3)
+        v = np.zeros_like(x) if velocities is None else np.ascontiguousarray(velocities, np.float32).reshape(-1, 3)
+        _check(lib().pbf_add_to_scene(self._ctx, int(scene), _fptr(x), _fptr(xs), _fptr(v), x.shape[0]))
+
+    def num_scenes(self) -> int:
+        return int(lib().pbf_num_scenes(self._ctx))
 
     def remove(self, count: int) -> None:
         _check(lib().pbf_remove(self._ctx, int(count)))
@@ -402,6 +422,10 @@ class Particles:
 
     def set_warm_model(self, model: "WarmModel") -> None:
         _check(lib().pbf_set_warm_model(self._ctx, model._h))
+
+    def step_graph(self, nsteps: int = 1) -> None:
+        """pbf_step through a captured HIP graph of two steps."""
+        _check(lib().pbf_step_graph(self._ctx, int(nsteps)))
 
     def sync(self) -> None:
         _check(lib().pbf_sync(self._ctx))

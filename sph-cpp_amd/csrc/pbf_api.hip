@@ -32,6 +32,7 @@ int check_hip(hipError_t e, const char* what) {
 }
 
 KernelTimer* timer_begin(pbf_context* c, const char* name) {
+  if (c->capturing) return nullptr;
   for (auto& t : c->timers) {
     if (t.name == name) {
       hipEvent_t a, b;
@@ -145,7 +146,25 @@ static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
   K.cap = (int)P.max_neighbors;
   K.stride = (K.cap + 7) / 8 * 8;
   K.count_overflow = (P.options & PBF_OPT_COUNT_OVERFLOW) ? 1 : 0;
+  K.batched = 0;
+  K.scenes = nullptr;
+  K.sid = nullptr;
   return PBF_OK;
+}
+
+// Batched contexts keep their scene grids when parameters are replaced.
+static void keep_scenes(const pbf_context* c, KParams& K) {
+  if (c->nscenes <= 1 && !c->K.batched) return;
+  const SceneGeo& g0 = c->scene_geo[0];
+  K.dimx = g0.dimx;
+  K.dimy = g0.dimy;
+  K.dimz = g0.dimz;
+  K.dxy = g0.dxy;
+  for (int i = 0; i < 3; ++i) K.bmax[i] = g0.bmax[i];
+  K.ncells = c->K.ncells;
+  K.batched = 1;
+  K.scenes = c->scenes;
+  K.sid = c->sid;
 }
 
 template <class T>
@@ -177,7 +196,7 @@ static void free_all(pbf_context* c) {
                   c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
                   c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty,
                   c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave, c->esum, c->feat, c->mw1,
-                  c->mw2, c->mp};
+                  c->mw2, c->mp, c->scenes, c->sid, c->sida};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -218,6 +237,20 @@ __global__ void k_compact_perm(uint32_t n, const uint32_t* __restrict__ oid, uin
 
 // neighbour search body shared by pbf_find_neighbors and pbf_step
 static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe);
+
+void drop_graph(pbf_context* c) {
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+}
+
+// The buffers whose roles the host swaps (reorder, per-pass calls): a graph
+// is valid only while they are where they were at capture.
+static std::vector<void*> buffer_sig(const pbf_context* c) {
+  return {c->x, c->xs, c->v, c->xa, c->xsa, c->va, c->xt, c->scr, c->scra, c->rho, c->lam, c->rhoa, c->lama,
+          c->oid, c->oida, c->key, c->keya, c->lsum, c->lsuma, c->sid, c->sida, (void*)(uintptr_t)c->track_lsum};
+}
 
 }  // namespace pbf
 
@@ -307,6 +340,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
 int pbf_destroy(pbf_context* c) {
   if (!c) return PBF_OK;
   hipStreamSynchronize(c->stream);
+  drop_graph(c);
   timers_resolve(c);
   free_all(c);
   if (c->done_host) hipHostFree(c->done_host);
@@ -325,9 +359,15 @@ int pbf_set_params(pbf_context* c, const pbf_params* params) {
   uint64_t dims[3];
   int rc = compute_kparams(*params, K, dims);
   if (rc) return rc;
+  if (c->K.batched) {
+    if (K.s != c->K.s) return set_error(PBF_ERR_UNSUPPORTED, "a batched context keeps its bin spacing");
+    keep_scenes(c, K);
+    for (int i = 0; i < 3; ++i) dims[i] = c->dims[i];
+  }
   const bool grid_changed = K.ncells != c->K.ncells || K.s != c->K.s || K.dimx != c->K.dimx || K.dimy != c->K.dimy;
   c->P = *params;
   c->K = K;
+  ++c->version;
   for (int i = 0; i < 3; ++i) c->dims[i] = dims[i];
   c->ncells = K.ncells;
   if (grid_changed) {
@@ -370,10 +410,82 @@ int pbf_add(pbf_context* c, const float* pos, const float* pos_star, const float
   hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, c->lam, first, count);
   hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, c->lsum, first, count);
   hipLaunchKernelGGL(k_zero_f1, dim3(nb), dim3(256), 0, c->stream, (float*)c->nbr_count, first, count);
+  if (c->K.batched) launch_fill_u32(c->sid + first, 0u, count, c->stream);
   c->n += count;
   c->lists_valid = false;
   return check_hip(hipStreamSynchronize(c->stream), "pbf_add");
 }
+
+int pbf_add_to_scene(pbf_context* c, uint32_t scene, const float* pos, const float* pos_star, const float* vel,
+                     uint64_t count) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  if (scene >= c->nscenes) return set_error(PBF_ERR_ARG, "scene index out of range");
+  const uint64_t first = c->n;
+  int rc = pbf_add(c, pos, pos_star, vel, count);
+  if (rc || !count || !c->K.batched) return rc;
+  launch_fill_u32(c->sid + first, scene, count, c->stream);
+  return check_hip(hipStreamSynchronize(c->stream), "pbf_add_to_scene");
+}
+
+// Independent scenes in one context: each scene's box is [0, boxes[3s..3s+2]]
+// and its grid is derived from it as for a single scene; all other
+// parameters are shared.
+int pbf_create_batched(const pbf_params* params, uint32_t num_scenes, const float* boxes, pbf_context** out) {
+  if (!params || !boxes || !out) return set_error(PBF_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (num_scenes == 0) return set_error(PBF_ERR_ARG, "num_scenes must be positive");
+  std::vector<SceneGeo> geo(num_scenes);
+  uint64_t total = 0;
+  for (uint32_t sc = 0; sc < num_scenes; ++sc) {
+    pbf_params ps = *params;
+    for (int i = 0; i < 3; ++i) {
+      if (!(boxes[3 * sc + i] > 0.0f)) return set_error(PBF_ERR_ARG, "scene boxes must be positive");
+      ps.boundary_max[i] = boxes[3 * sc + i];
+    }
+    KParams Ks;
+    uint64_t d[3];
+    int rc = compute_kparams(ps, Ks, d);
+    if (rc) return rc;
+    SceneGeo& g = geo[sc];
+    std::memset(&g, 0, sizeof(g));
+    g.dimx = Ks.dimx;
+    g.dimy = Ks.dimy;
+    g.dimz = Ks.dimz;
+    g.off = (uint32_t)total;
+    g.ncells = Ks.ncells;
+    g.dxy = Ks.dxy;
+    for (int i = 0; i < 3; ++i) g.bmax[i] = Ks.bmax[i];
+    total += Ks.ncells;
+    if (total >= 4000000000ull) return set_error(PBF_ERR_UNSUPPORTED, "scenes hold more than 4e9 cells");
+  }
+  pbf_params p0 = *params;
+  for (int i = 0; i < 3; ++i) p0.boundary_max[i] = boxes[i];
+  pbf_context* c = nullptr;
+  int rc = pbf_create(&p0, &c);
+  if (rc) return rc;
+  const uint64_t N = c->capacity;
+  c->nscenes = num_scenes;
+  c->scene_geo = geo;
+  if ((rc = dalloc(&c->scenes, (uint64_t)num_scenes)) || (rc = dalloc(&c->sid, N)) || (rc = dalloc(&c->sida, N)) ||
+      (rc = check_hip(hipMemcpy(c->scenes, geo.data(), num_scenes * sizeof(SceneGeo), hipMemcpyHostToDevice),
+                      "upload"))) {
+    pbf_destroy(c);
+    return rc;
+  }
+  c->K.ncells = (uint32_t)total;
+  c->ncells = total;
+  c->K.batched = 1;
+  c->K.scenes = c->scenes;
+  c->K.sid = c->sid;
+  if ((rc = alloc_grid(c))) {
+    pbf_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return PBF_OK;
+}
+
+uint32_t pbf_num_scenes(const pbf_context* c) { return c ? c->nscenes : 0; }
 
 int pbf_remove(pbf_context* c, uint64_t count) {
   if (!c) return set_error(PBF_ERR_ARG, "null argument");
@@ -554,15 +666,16 @@ static void step_tail(pbf_context* c) {
 }
 
 // Simulation/Source/main.cpp:71-124 on one domain, fused kernels.
-int pbf_step(pbf_context* c, uint32_t nsteps) {
-  if (!c) return set_error(PBF_ERR_ARG, "null context");
+static void step_body(pbf_context* c) {
   const uint64_t n = c->n;
-  const SolveCtl S{-1.0f, 0, 0, nullptr, nullptr, c->track_lsum ? c->lsum : nullptr, nullptr};
-  for (uint32_t st = 0; st < nsteps; ++st) {
+  {
+    // the multiplier sums move with the reorder inside find_neighbors
     launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
+    if (c->capturing) launch_fill_u32(c->dirty, 0u, n, c->stream);  // stamps repeat on every replay
     launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
     find_neighbors_impl(c, n, 0, n);
     if (c->track_lsum) launch_fill_u32((uint32_t*)c->lsum, 0u, n, c->stream);
+    const SolveCtl S{-1.0f, 0, 0, nullptr, nullptr, c->track_lsum ? c->lsum : nullptr, nullptr};
     const uint32_t I = c->P.solve_step_count;
     for (uint32_t it = 0; it < I; ++it) {
       ++c->dirty_stamp;
@@ -576,7 +689,50 @@ int pbf_step(pbf_context* c, uint32_t nsteps) {
     c->last_was_solve = false;
     step_tail(c);
   }
+}
+
+int pbf_step(pbf_context* c, uint32_t nsteps) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  for (uint32_t st = 0; st < nsteps; ++st) step_body(c);
   return check_hip(hipGetLastError(), "pbf_step");
+}
+
+// Two fused steps captured into one HIP graph and replayed.  Two, because a
+// step ends by swapping the reordered buffers on the host: after two steps
+// every buffer is back where the captured launches expect it.  An odd step
+// count ends with one ordinary step.  The graph is captured again after
+// anything that changes a step's launches (particle count, parameters).
+int pbf_step_graph(pbf_context* c, uint32_t nsteps) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  int rc;
+  if (nsteps >= 2 && (!c->graph_exec || c->graph_n != c->n || c->graph_version != c->version ||
+                      c->graph_sig != buffer_sig(c))) {
+    drop_graph(c);
+    c->capturing = true;
+    rc = check_hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (!rc) {
+      step_body(c);
+      step_body(c);
+    }
+    const int rc2 = check_hip(hipStreamEndCapture(c->stream, &c->graph), "hipStreamEndCapture");
+    c->capturing = false;
+    if (rc || (rc = rc2)) {
+      drop_graph(c);
+      return rc;
+    }
+    if ((rc = check_hip(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate"))) {
+      drop_graph(c);
+      return rc;
+    }
+    c->graph_n = c->n;
+    c->graph_version = c->version;
+    c->graph_sig = buffer_sig(c);
+  }
+  uint32_t st = 0;
+  for (; st + 2 <= nsteps; st += 2)
+    if ((rc = check_hip(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch"))) return rc;
+  if (st < nsteps) step_body(c);
+  return check_hip(hipGetLastError(), "pbf_step_graph");
 }
 
 // One step with an iterate-to-tolerance pressure solve.  The loop runs on the
@@ -861,6 +1017,7 @@ int pbf_profile_kernels(pbf_context* c, const char* names) {
   hipStreamSynchronize(c->stream);
   timers_resolve(c);
   c->timers.clear();
+  ++c->version;
   std::string all(names);
   size_t pos = 0;
   while (pos <= all.size()) {

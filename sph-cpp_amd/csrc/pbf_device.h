@@ -79,8 +79,21 @@ __device__ __forceinline__ float cspline(float r, const KParams& K) {
   return K.cspline_norm * (h - r) * (h - r) * (h - r) * r * r * r;
 }
 
+// The grid and box a slot's particle lives in: the context's own, or its
+// scene's in a batched context.
+struct GridView {
+  uint32_t dimx, dimy, dimz, off, ncells;
+  uint64_t dxy;
+  float bmax[3];
+};
+__device__ __forceinline__ GridView grid_of(const KParams& K, uint32_t p) {
+  if (!K.batched) return GridView{K.dimx, K.dimy, K.dimz, 0u, K.ncells, K.dxy, {K.bmax[0], K.bmax[1], K.bmax[2]}};
+  const SceneGeo g = K.scenes[K.sid[p]];
+  return GridView{g.dimx, g.dimy, g.dimz, g.off, g.ncells, g.dxy, {g.bmax[0], g.bmax[1], g.bmax[2]}};
+}
+
 // apply_boundary_conditions (particles.h:510-526)
-__device__ __forceinline__ void boundary(f3& p, const KParams& K) {
+__device__ __forceinline__ void boundary(f3& p, const KParams& K, const GridView& G) {
   const float mover_radius = 0.2f;
   const f3 mc = {K.mover[0], K.mover[1], K.mover[2]};
   const float dr_squared = magsq3(sub3(p, mc));
@@ -88,22 +101,24 @@ __device__ __forceinline__ void boundary(f3& p, const KParams& K) {
     const float dr = sqrtf(dr_squared);
     acc3(p, div3(smul3(mover_radius - dr, sub3(p, mc)), dr));
   }
-  if (p.x < K.bmin[0]) p.x = K.bmin[0]; else if (p.x > K.bmax[0]) p.x = K.bmax[0];
-  if (p.y < K.bmin[1]) p.y = K.bmin[1]; else if (p.y > K.bmax[1]) p.y = K.bmax[1];
-  if (p.z < K.bmin[2]) p.z = K.bmin[2]; else if (p.z > K.bmax[2]) p.z = K.bmax[2];
+  if (p.x < K.bmin[0]) p.x = K.bmin[0]; else if (p.x > G.bmax[0]) p.x = G.bmax[0];
+  if (p.y < K.bmin[1]) p.y = K.bmin[1]; else if (p.y > G.bmax[1]) p.y = G.bmax[1];
+  if (p.z < K.bmin[2]) p.z = K.bmin[2]; else if (p.z > G.bmax[2]) p.z = G.bmax[2];
 }
+__device__ __forceinline__ void boundary(f3& p, const KParams& K, uint32_t slot) { boundary(p, K, grid_of(K, slot)); }
 
 // calculate_bin_id one axis (neighbors.h:110-111): floor((p + s) / s)
 __device__ __forceinline__ float bin_coord(float p, float s) { return floorf((p + s) / s); }
 
-// Key of a cell given per-axis coordinates; anything that is not a cell of
-// the grid (negative, NaN, past the last cell) maps to ncells ("outside").
-__device__ __forceinline__ uint32_t cell_key(float fx, float fy, float fz, const KParams& K) {
+// Global key of a cell given per-axis coordinates in grid G; anything that
+// is not a cell of G (negative, NaN, past the last cell) maps to K.ncells
+// ("outside"), so a walk never leaves its scene.
+__device__ __forceinline__ uint32_t cell_key(float fx, float fy, float fz, const GridView& G, const KParams& K) {
   if (!(fx >= 0.f) || !(fy >= 0.f) || !(fz >= 0.f)) return K.ncells;
   if (fx > 4.0e9f || fy > 4.0e9f || fz > 4.0e9f) return K.ncells;
   const uint64_t bx = (uint64_t)fx, by = (uint64_t)fy, bz = (uint64_t)fz;
-  const uint64_t k = K.dxy * bz + (by * (uint64_t)K.dimx + bx);
-  return k < (uint64_t)K.ncells ? (uint32_t)k : K.ncells;
+  const uint64_t k = G.dxy * bz + (by * (uint64_t)G.dimx + bx);
+  return k < (uint64_t)G.ncells ? G.off + (uint32_t)k : K.ncells;
 }
 
 // Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD works on

@@ -249,26 +249,30 @@ __global__ void k_mark_dirty(int it, uint32_t stamp, SolveCtl S, KParams K, cons
     return;
   }
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nmoved; i += gridDim.x * blockDim.x) {
-    const uint32_t kq = key[moved[i]];
+    const uint32_t q = moved[i];
+    const uint32_t kq = key[q];
     if (kq >= K.ncells) {  // outside the grid: no cell to start from
       counters[C_MOVED_ALL + (it & 1)] = 1ull;
       continue;
     }
-    const int cz = (int)(kq / K.dxy), cy = (int)((kq % K.dxy) / K.dimx), cx = (int)(kq % K.dimx);
+    // cell coordinates in the moved particle's own grid (its scene's)
+    const GridView G = grid_of(K, q);
+    const uint32_t lq = kq - G.off;
+    const int cz = (int)(lq / G.dxy), cy = (int)((lq % G.dxy) / G.dimx), cx = (int)(lq % G.dimx);
     uint32_t total = 0;
-    for (int z = max(cz - 2, 0); z <= min(cz + 2, (int)K.dimz - 1); ++z)
-      for (int y = max(cy - 2, 0); y <= min(cy + 2, (int)K.dimy - 1); ++y) {
-        const uint64_t row = (uint64_t)z * K.dxy + (uint64_t)y * K.dimx;
-        total += start[row + min(cx + 2, (int)K.dimx - 1) + 1] - start[row + max(cx - 2, 0)];
+    for (int z = max(cz - 2, 0); z <= min(cz + 2, (int)G.dimz - 1); ++z)
+      for (int y = max(cy - 2, 0); y <= min(cy + 2, (int)G.dimy - 1); ++y) {
+        const uint64_t row = G.off + (uint64_t)z * G.dxy + (uint64_t)y * G.dimx;
+        total += start[row + min(cx + 2, (int)G.dimx - 1) + 1] - start[row + max(cx - 2, 0)];
       }
     if (total > kMaxMarkedPerMoved) {  // crowded neighbourhood: cheaper to treat everybody as dirty
       counters[C_MOVED_ALL + (it & 1)] = 1ull;
       continue;
     }
-    for (int z = max(cz - 2, 0); z <= min(cz + 2, (int)K.dimz - 1); ++z)
-      for (int y = max(cy - 2, 0); y <= min(cy + 2, (int)K.dimy - 1); ++y) {
-        const uint64_t row = (uint64_t)z * K.dxy + (uint64_t)y * K.dimx;
-        const uint32_t b = start[row + max(cx - 2, 0)], e = start[row + min(cx + 2, (int)K.dimx - 1) + 1];
+    for (int z = max(cz - 2, 0); z <= min(cz + 2, (int)G.dimz - 1); ++z)
+      for (int y = max(cy - 2, 0); y <= min(cy + 2, (int)G.dimy - 1); ++y) {
+        const uint64_t row = G.off + (uint64_t)z * G.dxy + (uint64_t)y * G.dimx;
+        const uint32_t b = start[row + max(cx - 2, 0)], e = start[row + min(cx + 2, (int)G.dimx - 1) + 1];
         for (uint32_t p = b; p < e; ++p) dirty[p] = stamp;
       }
   }
@@ -390,7 +394,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
     }
   }
   f3 nw = add3(xp, smul3(K.inv_rho0, dp));
-  boundary(nw, K);
+  boundary(nw, K, p);
   st3(xs, p, nw);
   if (S.lsum) S.lsum[p] += lp;
   if (last) {
@@ -437,7 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_dp(uint32_t n, KParams K, const
     }
   }
   f3 nw = add3(xp, smul3(K.inv_rho0, dp));
-  boundary(nw, K);
+  boundary(nw, K, p);
   st3(xs, p, nw);
   xsave[p] = xp4;
 }

@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(const float4* __restrict__ xs, u
   uint32_t k = K.ncells;  // particles past the bin span are placed in no cell
   if (o < bin_end) {
     const f3 ps = ld3(xs, p);
-    k = cell_key(bin_coord(ps.x, K.s), bin_coord(ps.y, K.s), bin_coord(ps.z, K.s), K);
+    k = cell_key(bin_coord(ps.x, K.s), bin_coord(ps.y, K.s), bin_coord(ps.z, K.s), grid_of(K, p), K);
     if (k == K.ncells) atomicAdd(&counters[C_OUTSIDE], 1ull);
   }
   key[p] = k;
@@ -217,15 +217,18 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
                                                     const float* __restrict__ lam, const float4* __restrict__ scr,
                                                     const uint32_t* __restrict__ oid,
                                                     const uint32_t* __restrict__ key,
-                                                    const float* __restrict__ lsum, float4* __restrict__ xo,
+                                                    const float* __restrict__ lsum,
+                                                    const uint32_t* __restrict__ sid, float4* __restrict__ xo,
                                                     float4* __restrict__ xso, float4* __restrict__ vo,
                                                     float* __restrict__ rhoo, float* __restrict__ lamo,
                                                     float4* __restrict__ scro, uint32_t* __restrict__ oido,
-                                                    uint32_t* __restrict__ keyo, float* __restrict__ lsumo) {
+                                                    uint32_t* __restrict__ keyo, float* __restrict__ lsumo,
+                                                    uint32_t* __restrict__ sido) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t p = perm[s];
   if (lsum) lsumo[s] = lsum[p];
+  if (sid) sido[s] = sid[p];
   xo[s] = x[p];
   xso[s] = xs[p];
   vo[s] = v[p];
@@ -243,10 +246,15 @@ void launch_reorder(pbf_context* c) {
   hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->perm,
                      (const float4*)c->x, (const float4*)c->xs, (const float4*)c->v, (const float*)c->rho,
                      (const float*)c->lam, (const float4*)c->scr, (const uint32_t*)c->oid,
-                     (const uint32_t*)c->key, c->track_lsum ? (const float*)c->lsum : nullptr, c->xa, c->xsa, c->va,
-                     c->rhoa, c->lama, c->scra, c->oida, c->keya, c->lsuma);
+                     (const uint32_t*)c->key, c->track_lsum ? (const float*)c->lsum : nullptr,
+                     c->K.batched ? (const uint32_t*)c->sid : nullptr, c->xa, c->xsa, c->va, c->rhoa, c->lama,
+                     c->scra, c->oida, c->keya, c->lsuma, c->sida);
   timer_end(c, t);
   if (c->track_lsum) std::swap(c->lsum, c->lsuma);
+  if (c->K.batched) {
+    std::swap(c->sid, c->sida);
+    c->K.sid = c->sid;
+  }
   std::swap(c->x, c->xa);
   std::swap(c->xs, c->xsa);
   std::swap(c->v, c->va);
@@ -310,13 +318,14 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
   }
   const f3 ps = ld3(xs, p);
   const WalkCells w = walk_cells(ps, K.s);
+  const GridView G = grid_of(K, p);
   if (K.count_overflow) {
     // every candidate must be seen: route by the total number of candidates
     uint32_t total = 0;
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
         for (int k = 0; k < 3; ++k) {
-          const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], K);
+          const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], G, K);
           if (cell < K.ncells) total += start[cell + 1] - start[cell];
         }
     if (total > kHeavyCandidates) {
@@ -334,7 +343,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 3; ++k) {
-        const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], K);
+        const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], G, K);
         if (cell >= K.ncells) continue;
         const uint32_t b = start[cell], e = start[cell + 1];
         // candidates of a cell are contiguous: load 8 at a time, test in order
@@ -388,11 +397,12 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
     const uint32_t p = heavy[h];
     const f3 ps = ld3(xs, p);
     const WalkCells w = walk_cells(ps, K.s);
+    const GridView G = grid_of(K, p);
     uint32_t count = 0;  // accepted so far, uncapped (uniform across the wave)
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
         for (int k = 0; k < 3; ++k) {
-          const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], K);
+          const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], G, K);
           if (cell >= K.ncells) continue;
           const uint32_t b = start[cell], e = start[cell + 1];
           for (uint32_t base = b; base < e; base += 64) {

@@ -38,6 +38,15 @@
 
 namespace pbf {
 
+// One scene of a batched context: its box (from the origin) and its grid,
+// whose cells take the global keys [off, off + ncells).
+struct SceneGeo {
+  uint32_t dimx, dimy, dimz, off;
+  uint32_t ncells, pad;
+  uint64_t dxy;
+  float bmax[3], pad2;
+};
+
 // Constants of one step, computed on the host exactly as the reference
 // computes them (kernels.h norms in double, parameters.h derivations).
 struct KParams {
@@ -58,6 +67,10 @@ struct KParams {
   int cap;
   int stride;                    // list row length: cap rounded up to a multiple of 8
   int count_overflow;            // PBF_OPT_COUNT_OVERFLOW
+  // batched scenes: ncells is the total over the scenes, sid the scene of each slot
+  int batched;
+  const SceneGeo* scenes;
+  const uint32_t* sid;
 };
 
 enum Counter {
@@ -146,6 +159,17 @@ struct pbf_context {
   short *mw1 = nullptr, *mw2 = nullptr;  // learned warm start: bf16 weights
   float* mp = nullptr;                   // and the fp32 parameters
   bool has_model = false;
+  uint32_t nscenes = 1;
+  std::vector<pbf::SceneGeo> scene_geo;   // host copy (batched contexts)
+  pbf::SceneGeo* scenes = nullptr;        // device copy
+  uint32_t *sid = nullptr, *sida = nullptr;  // scene of each slot (batched contexts)
+  // captured pair of fused steps (pbf_step_graph)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_n = 0, graph_version = 0;
+  std::vector<void*> graph_sig;  // buffer pointers when the graph was captured
+  uint64_t version = 0;  // bumped by anything that changes the launches of a step
+  bool capturing = false;
   unsigned long long* done_host = nullptr;  // pinned copy of C_DONE, polled between slot chunks
   float *lsum = nullptr, *lsuma = nullptr, *wlam = nullptr, *wgiven = nullptr;
   float4* xsave = nullptr;

@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_predict(uint32_t n, uint32_t 
   }
   if (flags & F_PREDICT) {
     f3 ps = add3(ld3(x, p), mul3(vp, K.dt));
-    boundary(ps, K);
+    boundary(ps, K, p);
     st3(xs, p, ps);
   }
 }
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(kBlock) k_update_position_stars(uint32_t n, ui
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n || !in_span(oid[p], b, e)) return;
   f3 nw = add3(ld3(xs, p), ld3(scr, p));
-  boundary(nw, K);
+  boundary(nw, K, p);
   st3(xs, p, nw);
 }
 

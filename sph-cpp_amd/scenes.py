@@ -113,3 +113,29 @@ def seconds_per_substep_fmt(x: float) -> str:
 
 def particle_substeps_per_s(n: int, substeps: int, seconds: float) -> float:
     return n * substeps / seconds if seconds > 0 else math.inf
+
+
+def batched_dam_breaks(num_scenes=256, seed=0, nmin=4096, nmax=65536, **kw):
+    """Independent dam breaks of uneven size for one batched context.
+
+    Scene s holds a lattice block of about N_s particles, N_s log-uniform in
+    [nmin, nmax]; the block's x:y and z:y aspect ratios are uniform in
+    [0.5, 2]; its box is twice the block's length and the block's height and
+    depth (the C1 dam break's proportions).  Returns (params, boxes, scenes)
+    with boxes an (S, 3) array and scenes a list of (positions, velocities).
+    """
+    rng = np.random.default_rng(seed)
+    s = np.float32(SPACING)
+    boxes, out = [], []
+    total = 0
+    for k in range(num_scenes):
+        n = math.exp(rng.uniform(math.log(nmin), math.log(nmax)))
+        a, b = rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)
+        ny = max(2, int(round((n / (a * b)) ** (1.0 / 3.0))))
+        nx, nz = max(2, int(round(a * ny))), max(2, int(round(b * ny)))
+        pos = jitter(lattice((nx, ny, nz)), int(rng.integers(1 << 31)))
+        boxes.append((float(2 * nx * s), float(ny * s), float(nz * s)))
+        out.append((pos, np.zeros_like(pos)))
+        total += pos.shape[0]
+    params = pbf_params(boxes[0], total, **kw)
+    return params, np.array(boxes, dtype=np.float32), out

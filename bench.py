@@ -110,12 +110,14 @@ HIP_KERNEL = {
 }
 
 
-def timed_run(pbf, params, pos, vel, steps, profile=True):
+def timed_run(pbf, params, pos, vel, steps, profile=True, tuning=None):
     """K substeps from the initial scene on a fresh context; HIP events around
     every kernel launch on the library's stream when profile is set."""
     n = pos.shape[0]
     g = pbf.Particles(params)
     g.add(pos, pos, vel)
+    if tuning:
+        g.set_tuning(tuning["block_size"], tuning["heavy_budget"], tuning["heavy_blocks"])
     if profile:
         g.profile_kernels(KERNELS)
     g.sync()
@@ -229,10 +231,15 @@ def batched_run(pbf, scenes, steps, seed, num_scenes=256):
     params, boxes, sc = scenes.batched_dam_breaks(num_scenes=num_scenes, seed=seed)
     n = sum(p.shape[0] for p, _ in sc)
     out = {"scenes": num_scenes, "particles": n, "steps": steps}
+    tuning = None
     for name in ("eager", "graph"):
         g = pbf.Particles(params, boxes=boxes)
         for k, (pos, vel) in enumerate(sc):
             g.add_to_scene(k, pos, pos, vel)
+        if tuning is None:
+            tuning = g.autotune(steps=2)
+            out["launch_tuning"] = tuning
+        g.set_tuning(tuning["block_size"], tuning["heavy_budget"], tuning["heavy_blocks"])
         run = g.step_graph if name == "graph" else g.step
         run(2)
         g.sync()
@@ -307,6 +314,8 @@ def main(argv=None):
     ap.add_argument("--no-warm", dest="warm", action="store_false",
                     help="skip the plain vs warm-start comparison at a density-error tolerance")
     ap.add_argument("--warm-steps", type=int, default=200)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch settings instead of running pbf_autotune first")
     ap.add_argument("--summarize", default="", metavar="DIR",
                     help="only write the profiles/ summaries from the rocprofv3 outputs under DIR")
     args = ap.parse_args(argv)
@@ -335,14 +344,16 @@ def main(argv=None):
     params, pos, vel = scenes.config_c2(seed=args.seed, lattice_counts=counts, cap=args.cap)
     n = pos.shape[0]
 
-    # warm-up on a throwaway context (module load, first launches)
+    # warm-up on a throwaway context (module load, first launches); the
+    # autotuner then picks the launch settings halfway through the run
     w = pbf.Particles(params)
     w.add(pos, pos, vel)
-    w.step(args.warmup)
+    w.step(max(args.warmup, args.steps // 2 if args.autotune else 0))
+    tuning = w.autotune(steps=3) if args.autotune else None
     w.sync()
     w.close()
 
-    res = timed_run(pbf, params, pos, vel, args.steps)
+    res = timed_run(pbf, params, pos, vel, args.steps, tuning=tuning)
     dom = args.kernel or max(res["kernels"], key=lambda k: res["kernels"][k]["total_ms"])
     value = res["value"]
     kd = res["kernels"][dom]
@@ -391,6 +402,7 @@ def main(argv=None):
             "jacobi_iterations": I,
             "neighbor_cap": int(params.max_neighbors),
             "cells": res["cells"],
+            "launch_tuning": tuning,
         },
         "roofline": roof,
         "step_roofline": step_roof,
@@ -399,7 +411,7 @@ def main(argv=None):
     }
     if args.variants:
         vparams, _, _ = scenes.config_c2(seed=args.seed, lattice_counts=counts, cap=128)
-        vr = timed_run(pbf, vparams, pos, vel, args.steps, profile=False)
+        vr = timed_run(pbf, vparams, pos, vel, args.steps, profile=False, tuning=tuning)
         out["variants"] = {"cap128": {"value": vr["value"], "ms_per_step": vr["seconds"] * 1e3 / args.steps,
                                       "stats_last_step": vr["stats"],
                                       "note": "same scene with a 128-entry neighbour list (no list saturation)"}}

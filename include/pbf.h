@@ -278,6 +278,24 @@ int pbf_get_warm_features(pbf_context* ctx, float* out);
  * (add, remove, set_params, per-pass calls). */
 int pbf_step_graph(pbf_context* ctx, uint32_t nsteps);
 
+/* ---- launch tuning (no counterpart in the reference) ----
+ * Launch parameters that do not change any result: the block size of the fused
+ * per-particle kernels (64, 128 or 256), the candidate budget after which the neighbour
+ * walk of one particle is handed from one lane to a whole wavefront (64..65536), and the
+ * number of blocks of that wavefront builder (1..65536).  The bin spacing and the list cap
+ * are not tuned: both are part of the reference's neighbour semantics (neighbors.h:39,198).
+ * pbf_autotune times `steps` fused steps from the current state for each candidate,
+ * restores the state (the following steps are bit-identical to an untuned run), keeps
+ * the fastest setting and returns it with its mean step time. */
+typedef struct pbf_tuning {
+  uint32_t block_size;
+  uint32_t heavy_budget;
+  uint32_t heavy_blocks;
+} pbf_tuning;
+int pbf_set_tuning(pbf_context* ctx, const pbf_tuning* tuning);
+int pbf_get_tuning(const pbf_context* ctx, pbf_tuning* out);
+int pbf_autotune(pbf_context* ctx, uint32_t steps, pbf_tuning* best, double* best_ms_per_step);
+
 /* Block until the context's queued work is done. */
 int pbf_sync(pbf_context* ctx);
 

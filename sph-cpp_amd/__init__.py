@@ -101,6 +101,10 @@ PBF_WARM_PARAMS = 4801
 PBF_WARM_LAMBDA_SCALE = 1e-4
 
 
+class Tuning(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("heavy_budget", C.c_uint32), ("heavy_blocks", C.c_uint32)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [
         ("iterations", C.c_uint32),
@@ -157,6 +161,9 @@ def lib():
     _sig(L, "pbf_step", [_P, C.c_uint32])
     _sig(L, "pbf_solve", [_P, C.c_float, C.c_uint32, C.c_int])
     _sig(L, "pbf_step_graph", [_P, C.c_uint32])
+    _sig(L, "pbf_set_tuning", [_P, C.POINTER(Tuning)])
+    _sig(L, "pbf_get_tuning", [_P, C.POINTER(Tuning)])
+    _sig(L, "pbf_autotune", [_P, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_double)])
     _sig(L, "pbf_create_batched", [C.POINTER(Params), C.c_uint32, _F, C.POINTER(_P)])
     _sig(L, "pbf_add_to_scene", [_P, C.c_uint32, _F, _F, _F, C.c_uint64])
     L.pbf_num_scenes.argtypes = [_P]
@@ -422,6 +429,22 @@ class Particles:
 
     def set_warm_model(self, model: "WarmModel") -> None:
         _check(lib().pbf_set_warm_model(self._ctx, model._h))
+
+    def set_tuning(self, block_size: int = 256, heavy_budget: int = 1536, heavy_blocks: int = 2048) -> None:
+        t = Tuning(block_size, heavy_budget, heavy_blocks)
+        _check(lib().pbf_set_tuning(self._ctx, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().pbf_get_tuning(self._ctx, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def autotune(self, steps: int = 3) -> dict:
+        """pbf_autotune: time the launch settings from the current state, keep the fastest."""
+        t = Tuning()
+        ms = C.c_double(0.0)
+        _check(lib().pbf_autotune(self._ctx, int(steps), C.byref(t), C.byref(ms)))
+        return {**{k: getattr(t, k) for k, _ in Tuning._fields_}, "ms_per_step": ms.value}
 
     def step_graph(self, nsteps: int = 1) -> None:
         """pbf_step through a captured HIP graph of two steps."""

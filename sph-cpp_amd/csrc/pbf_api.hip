@@ -149,6 +149,7 @@ static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
   K.batched = 0;
   K.scenes = nullptr;
   K.sid = nullptr;
+  K.heavy_budget = 1536u;
   return PBF_OK;
 }
 
@@ -365,6 +366,7 @@ int pbf_set_params(pbf_context* c, const pbf_params* params) {
     for (int i = 0; i < 3; ++i) dims[i] = c->dims[i];
   }
   const bool grid_changed = K.ncells != c->K.ncells || K.s != c->K.s || K.dimx != c->K.dimx || K.dimy != c->K.dimy;
+  K.heavy_budget = c->tune.heavy_budget;
   c->P = *params;
   c->K = K;
   ++c->version;
@@ -879,6 +881,104 @@ int pbf_get_solve_info(pbf_context* c, pbf_solve_info* out) {
     std::memcpy(&out->max_error_history[k], &e[k + 1], 4);
     out->mean_error_history[k] = mean(es[k + 1]);
   }
+  return PBF_OK;
+}
+
+int pbf_set_tuning(pbf_context* c, const pbf_tuning* t) {
+  if (!c || !t) return set_error(PBF_ERR_ARG, "null argument");
+  if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
+    return set_error(PBF_ERR_ARG, "block_size must be 64, 128 or 256");
+  if (t->heavy_budget < 64 || t->heavy_budget > 65536) return set_error(PBF_ERR_ARG, "heavy_budget out of range");
+  if (t->heavy_blocks < 1 || t->heavy_blocks > 65536) return set_error(PBF_ERR_ARG, "heavy_blocks out of range");
+  c->tune = *t;
+  c->K.heavy_budget = t->heavy_budget;
+  ++c->version;
+  return PBF_OK;
+}
+
+int pbf_get_tuning(const pbf_context* c, pbf_tuning* out) {
+  if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
+  *out = c->tune;
+  return PBF_OK;
+}
+
+// Times every candidate from the same saved state (positions, velocities,
+// original ids, and the scene ids and multiplier sums where kept), then
+// restores that state: the next step sorts by (cell, original id) and
+// recomputes everything else, so it is the step an untuned run would take.
+int pbf_autotune(pbf_context* c, uint32_t steps, pbf_tuning* best, double* best_ms) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  if (steps == 0) return set_error(PBF_ERR_ARG, "steps must be positive");
+  const uint64_t n = c->n;
+  if (!n) return set_error(PBF_ERR_STATE, "no particles");
+  float4 *sx = nullptr, *sv = nullptr;
+  uint32_t *so = nullptr, *ss = nullptr;
+  float* sl = nullptr;
+  int rc;
+  if ((rc = dalloc(&sx, n)) || (rc = dalloc(&sv, n)) || (rc = dalloc(&so, n)) || (rc = dalloc(&ss, n)) ||
+      (rc = dalloc(&sl, n))) {
+    dfree(sx); dfree(sv); dfree(so); dfree(ss); dfree(sl);
+    return rc;
+  }
+  auto copy = [&](void* d, const void* s, uint64_t bytes) {
+    return check_hip(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream), "autotune copy");
+  };
+  auto save = [&]() {
+    int r = copy(sx, c->x, n * 16);
+    if (!r) r = copy(sv, c->v, n * 16);
+    if (!r) r = copy(so, c->oid, n * 4);
+    if (!r && c->K.batched) r = copy(ss, c->sid, n * 4);
+    if (!r && c->track_lsum) r = copy(sl, c->lsum, n * 4);
+    return r;
+  };
+  auto restore = [&]() {
+    int r = copy(c->x, sx, n * 16);
+    if (!r) r = copy(c->v, sv, n * 16);
+    if (!r) r = copy(c->oid, so, n * 4);
+    if (!r && c->K.batched) r = copy(c->sid, ss, n * 4);
+    if (!r && c->track_lsum) r = copy(c->lsum, sl, n * 4);
+    c->lists_valid = false;
+    return r;
+  };
+  const pbf_tuning start = c->tune;
+  pbf_tuning best_t = start;
+  double best_v = 1e300;
+  hipEvent_t a = nullptr, b = nullptr;
+  rc = save();
+  if (!rc) rc = check_hip(hipEventCreate(&a), "hipEventCreate");
+  if (!rc) rc = check_hip(hipEventCreate(&b), "hipEventCreate");
+  const uint32_t blocks[] = {64u, 128u, 256u};
+  const uint32_t budgets[] = {768u, 1536u, 3072u};
+  const uint32_t hblocks[] = {1024u, 2048u, 8192u};
+  for (uint32_t bs : blocks)
+    for (uint32_t hb : budgets)
+      for (uint32_t hk : hblocks) {
+        if (rc) break;
+        const pbf_tuning t{bs, hb, hk};
+        if ((rc = pbf_set_tuning(c, &t)) || (rc = restore())) break;
+        step_body(c);  // first launches of this setting, untimed
+        if ((rc = restore())) break;
+        hipEventRecord(a, c->stream);
+        for (uint32_t st = 0; st < steps; ++st) step_body(c);
+        hipEventRecord(b, c->stream);
+        if ((rc = check_hip(hipEventSynchronize(b), "autotune"))) break;
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a, b);
+        if (ms / steps < best_v) {
+          best_v = ms / steps;
+          best_t = t;
+        }
+      }
+  if (!rc) rc = restore();
+  if (!rc) rc = pbf_set_tuning(c, &best_t);
+  if (!rc) rc = check_hip(hipStreamSynchronize(c->stream), "autotune");
+  if (rc) pbf_set_tuning(c, &start);
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  dfree(sx); dfree(sv); dfree(so); dfree(ss); dfree(sl);
+  if (rc) return rc;
+  if (best) *best = best_t;
+  if (best_ms) *best_ms = best_v;
   return PBF_OK;
 }
 

@@ -613,7 +613,8 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     const uint32_t n_ = (uint32_t)(c)->n;                                                                 \
     if (!n_) return;                                                                                      \
     KernelTimer* t_ = timer_begin((c), name);                                                             \
-    hipLaunchKernelGGL(kern, dim3(nblocks(n_)), dim3(kBlock), 0, (c)->stream, n_, __VA_ARGS__);           \
+    const uint32_t b_ = (c)->tune.block_size;                                                             \
+    hipLaunchKernelGGL(kern, dim3((n_ + b_ - 1) / b_), dim3(b_), 0, (c)->stream, n_, __VA_ARGS__);         \
     timer_end((c), t_);                                                                                   \
   } while (0)
 

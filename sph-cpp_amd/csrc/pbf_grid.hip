@@ -272,10 +272,9 @@ void launch_reorder(pbf_context* c) {
 // d^2 < s^2 while the list has room.  Lists hold sorted slots, column major.
 //
 // Two tiers with identical results: one lane per particle when its 27 cells
-// hold at most kHeavyCandidates particles; otherwise the particle goes to a
+// hold at most K.heavy_budget particles; otherwise the particle goes to a
 // list that k_fill_heavy walks with a whole wavefront (64 candidates per
 // step, accepted ones appended in order through a ballot).
-constexpr uint32_t kHeavyCandidates = 1536;
 
 struct WalkCells {
   float cx[3], cy[3], cz[3];
@@ -328,13 +327,13 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
           const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], G, K);
           if (cell < K.ncells) total += start[cell + 1] - start[cell];
         }
-    if (total > kHeavyCandidates) {
+    if (total > K.heavy_budget) {
       heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
       return;
     }
   }
   // otherwise the walk ends when the list is full; a lane that has scanned
-  // kHeavyCandidates without filling its list hands the particle over (the
+  // K.heavy_budget without filling its list hands the particle over (the
   // wavefront builder starts again and writes the same entries)
   uint32_t scanned = 0;
   int count = 0;
@@ -349,7 +348,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
         // candidates of a cell are contiguous: load 8 at a time, test in order
         for (uint32_t q0 = b; q0 < e; q0 += 8) {
           scanned += 8;
-          if (scanned > kHeavyCandidates && !K.count_overflow) {
+          if (scanned > K.heavy_budget && !K.count_overflow) {
             heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
             return;
           }
@@ -437,7 +436,7 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
   timer_end(c, t);
   // enough waves to cover the chip; each loops over the heavy list (usually empty)
   t = timer_begin(c, "fill_heavy");
-  const uint32_t blocks = std::min<uint32_t>(nblocks((uint64_t)n * 64), 2048u);
+  const uint32_t blocks = std::min<uint32_t>(nblocks((uint64_t)n * 64), c->tune.heavy_blocks);
   hipLaunchKernelGGL(k_fill_heavy, dim3(blocks), dim3(kBlock), 0, c->stream, n, (const float4*)c->xs,
                      (const uint32_t*)c->cell_start, c->K, c->nbr, c->nbr_count, (const uint32_t*)c->heavy,
                      c->counters);

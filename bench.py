@@ -41,6 +41,7 @@ ALG_BYTES = {
     "dp_update": 12 + 4 + 12,              # R x*, lambda; W x*
     "fill_neighbors": 12 + 4,              # R x*; W count (lists not counted)
     "fill_heavy": 0,                       # the same lists, finished for crowded cells
+    "cell_box": 12 + 4,                    # R x*, key
     "viscosity": 12 + 12 + 4 + 12,         # R x*, v, rho; W v
     "viscosity_positions": 12 + 12 + 4 + 12 + 12,
     "vorticity": 12 + 12 + 12,             # R x*, v; W omega
@@ -95,14 +96,15 @@ def _profile_traffic(kernel: str):
         return None
 
 
-KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "fill_neighbors", "fill_heavy", "density_lambda",
+KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "cell_box", "fill_neighbors", "fill_heavy",
+           "density_lambda",
            "mark_dirty", "lambda_repair", "dp_update", "viscosity", "vorticity", "apply_vorticity",
            "viscosity_positions"]
 
 # HIP kernel behind each timer (the names rocprofv3 reports)
 HIP_KERNEL = {
     "forces_predict": "pbf::k_forces_predict", "bin": "pbf::k_bin", "reorder": "pbf::k_reorder",
-    "fill_neighbors": "pbf::k_fill_neighbors", "fill_heavy": "pbf::k_fill_heavy",
+    "cell_box": "pbf::k_cell_box", "fill_neighbors": "pbf::k_fill_neighbors", "fill_heavy": "pbf::k_fill_heavy",
     "density_lambda": "pbf::k_density_lambda_c", "mark_dirty": "pbf::k_mark_dirty",
     "lambda_repair": "pbf::k_lambda_repair", "dp_update": "pbf::k_dp_update_c",
     "viscosity": "pbf::k_viscosity_first", "vorticity": "pbf::k_vorticity_c",

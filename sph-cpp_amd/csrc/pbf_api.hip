@@ -183,9 +183,11 @@ static int alloc_grid(pbf_context* c) {
   const uint64_t need = c->ncells + 2;
   if (need <= c->grid_alloc) return PBF_OK;
   dfree(c->cell_start);
+  dfree(c->cbox);
   c->cell_start = nullptr;
+  c->cbox = nullptr;
   int rc;
-  if ((rc = dalloc(&c->cell_start, need))) return rc;
+  if ((rc = dalloc(&c->cell_start, need)) || (rc = dalloc(&c->cbox, 6 * need))) return rc;
   c->grid_alloc = need;
   return PBF_OK;
 }
@@ -197,7 +199,7 @@ static void free_all(pbf_context* c) {
                   c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
                   c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty,
                   c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave, c->esum, c->feat, c->mw1,
-                  c->mw2, c->mp, c->scenes, c->sid, c->sida};
+                  c->mw2, c->mp, c->scenes, c->sid, c->sida, c->cbox};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -1172,6 +1174,7 @@ static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, u
   launch_bin(c, bin_end);
   launch_sort(c);
   launch_reorder(c);
+  launch_cell_box(c);
   launch_fill_neighbors(c, fb, fe);
   c->lists_valid = true;
 }

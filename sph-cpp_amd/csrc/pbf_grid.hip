@@ -266,6 +266,90 @@ void launch_reorder(pbf_context* c) {
 }
 
 // ---------------------------------------------------------------------------
+// Bounding box of every cell's particles, 6 order-preserving integer
+// encodings per cell (min x y z, max x y z).  A neighbour walk skips a cell
+// whose box is at least s away from the walking particle: float subtraction,
+// squaring and addition are monotone, so every candidate's d^2 (computed with
+// the same operations) is at least the box bound and none could be accepted.
+// The lists are unchanged.  The wavefront builder, which handles the particles
+// next to crowded cells, tests cells of more than kBoxMin particles.
+constexpr uint32_t kBoxMin = 32;
+
+__device__ __forceinline__ uint32_t ord_enc(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_dec(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__global__ void k_box_init(uint32_t* __restrict__ box, uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) box[i] = (i % 6 < 3) ? 0xFFFFFFFFu : 0u;
+}
+
+// Slots are in cell order: each wavefront reduces its runs of equal keys and
+// the first lane of a run updates the cell's box.
+__global__ void __launch_bounds__(kBlock) k_cell_box(uint32_t n, uint32_t ncells, const uint32_t* __restrict__ key,
+                                                     const float4* __restrict__ xs, uint32_t* __restrict__ box) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = p < n;
+  const uint32_t k = valid ? key[p] : 0xFFFFFFFFu;
+  const float4 t = valid ? xs[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t mn[3] = {ord_enc(t.x), ord_enc(t.y), ord_enc(t.z)};
+  uint32_t mx[3] = {mn[0], mn[1], mn[2]};
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t kk = __shfl_down(k, d, 64);
+    uint32_t omn[3], omx[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      omn[a] = __shfl_down(mn[a], d, 64);
+      omx[a] = __shfl_down(mx[a], d, 64);
+    }
+    if (lane + d < 64 && kk == k) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = min(mn[a], omn[a]);
+        mx[a] = max(mx[a], omx[a]);
+      }
+    }
+  }
+  const uint32_t prev = __shfl_up(k, 1, 64);
+  if (valid && k < ncells && (lane == 0 || prev != k)) {
+    uint32_t* b = box + 6ull * k;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&b[a], mn[a]);
+      atomicMax(&b[3 + a], mx[a]);
+    }
+  }
+}
+
+void launch_cell_box(pbf_context* c) {
+  const uint32_t n = (uint32_t)c->n;
+  if (!n) return;
+  KernelTimer* t = timer_begin(c, "cell_box");
+  const uint64_t count = 6ull * c->K.ncells;
+  hipLaunchKernelGGL(k_box_init, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, c->cbox, count);
+  hipLaunchKernelGGL(k_cell_box, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, c->K.ncells,
+                     (const uint32_t*)c->key, (const float4*)c->xs, c->cbox);
+  timer_end(c, t);
+}
+
+// true when no particle of the cell can lie within s of ps (see above)
+__device__ __forceinline__ bool cell_far(const uint32_t* __restrict__ box, uint32_t cell, f3 ps, float ss) {
+  const uint32_t* b = box + 6ull * cell;
+  const float lx = ord_dec(b[0]), ly = ord_dec(b[1]), lz = ord_dec(b[2]);
+  const float hx = ord_dec(b[3]), hy = ord_dec(b[4]), hz = ord_dec(b[5]);
+  const float dx = ps.x < lx ? lx - ps.x : (ps.x > hx ? ps.x - hx : 0.0f);
+  const float dy = ps.y < ly ? ly - ps.y : (ps.y > hy ? ps.y - hy : 0.0f);
+  const float dz = ps.z < lz ? lz - ps.z : (ps.z > hz ? ps.z - hz : 0.0f);
+  return (dx * dx + dy * dy) + dz * dz >= ss;
+}
+
+// ---------------------------------------------------------------------------
 // fill_neighbors (neighbors.h:197-230).  A particle walks its 27 cells in the
 // reference's order (x offset outer, z inner; cell ids from the FP
 // coordinates x + i*s, neighbors.h:171-185), skips itself and accepts
@@ -301,7 +385,8 @@ constexpr int kFillBlock = 64;
 __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint32_t fb, uint32_t fe,
                                                                const float4* __restrict__ xs,
                                                                const uint32_t* __restrict__ oid,
-                                                               const uint32_t* __restrict__ start, KParams K,
+                                                               const uint32_t* __restrict__ start,
+                                                               const uint32_t* __restrict__ box, KParams K,
                                                                int32_t* __restrict__ nbr,
                                                                int32_t* __restrict__ nbr_count,
                                                                uint32_t* __restrict__ heavy,
@@ -382,7 +467,8 @@ done:
 // One wavefront per heavy particle; waves stride over the heavy list, whose
 // length is read once per wave (every wave exits after the last entry).
 __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4* __restrict__ xs,
-                                                       const uint32_t* __restrict__ start, KParams K,
+                                                       const uint32_t* __restrict__ start,
+                                                       const uint32_t* __restrict__ box, KParams K,
                                                        int32_t* __restrict__ nbr, int32_t* __restrict__ nbr_count,
                                                        const uint32_t* __restrict__ heavy,
                                                        unsigned long long* __restrict__ counters) {
@@ -404,6 +490,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
           const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], G, K);
           if (cell >= K.ncells) continue;
           const uint32_t b = start[cell], e = start[cell + 1];
+          if (e - b > kBoxMin && cell_far(box, cell, ps, K.ss)) continue;
           for (uint32_t base = b; base < e; base += 64) {
             const uint32_t q = base + lane;
             bool acc = false;
@@ -432,14 +519,14 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
   hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n, kFillBlock)), dim3(kFillBlock),
                      (uint32_t)(kFillBlock * c->K.cap * sizeof(int32_t)), c->stream, n, (uint32_t)fb,
                      (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
-                     c->K, c->nbr, c->nbr_count, c->heavy, c->counters);
+                     (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count, c->heavy, c->counters);
   timer_end(c, t);
   // enough waves to cover the chip; each loops over the heavy list (usually empty)
   t = timer_begin(c, "fill_heavy");
   const uint32_t blocks = std::min<uint32_t>(nblocks((uint64_t)n * 64), c->tune.heavy_blocks);
   hipLaunchKernelGGL(k_fill_heavy, dim3(blocks), dim3(kBlock), 0, c->stream, n, (const float4*)c->xs,
-                     (const uint32_t*)c->cell_start, c->K, c->nbr, c->nbr_count, (const uint32_t*)c->heavy,
-                     c->counters);
+                     (const uint32_t*)c->cell_start, (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count,
+                     (const uint32_t*)c->heavy, c->counters);
   timer_end(c, t);
 }
 

@@ -149,6 +149,7 @@ struct pbf_context {
   uint32_t* moved = nullptr;  // particles moved by the density fix in the current iteration
   uint32_t* dirty = nullptr;  // == dirty_stamp: the particle or one of its neighbours moved
   uint32_t* cell_start = nullptr;
+  uint32_t* cbox = nullptr;  // [6][cells] bounding box of each cell's particles (encoded)
   uint32_t* scan_partials = nullptr;
   int32_t *nbr = nullptr, *nbr_count = nullptr;
   int32_t *cnb = nullptr, *cnb_count = nullptr;
@@ -201,6 +202,7 @@ uint64_t scan_partials_size(uint64_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partials, hipStream_t s);
 void launch_bin(pbf_context* c, uint64_t bin_end);  // key, inverse permutation, keys by original index
 void launch_sort(pbf_context* c);            // radix sort -> perm, cell_start
+void launch_cell_box(pbf_context* c);        // bounding box of each cell's particles
 void launch_reorder(pbf_context* c);         // gather state by perm
 uint64_t radix_hist_size(uint64_t n);
 int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint64_t n, uint32_t key_bits,

@@ -144,51 +144,72 @@ def timed_run(pbf, params, pos, vel, steps, profile=True, tuning=None):
 
 # ---------------------------------------------------------------------------
 # rocprofv3 output -> the text summaries kept under profiles/
+PROFILED = "python3 bench.py --no-cpu --no-variants --no-warm"
 PROFILE_COMMANDS = {
-    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/trace -o c2 -- "
-             "python3 bench.py --no-cpu --no-variants --no-warm",
-    "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o c2 -- "
-             "python3 bench.py --no-cpu --no-variants --no-warm",
-    "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o c2 -- "
-             "python3 bench.py --no-cpu --no-variants --no-warm",
+    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/trace -o c2 -- " + PROFILED,
+    "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o c2 -- " + PROFILED,
+    "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o c2 -- " + PROFILED,
     "cal_fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/cal_fetch -o cp -- "
                  "python3 bench.py --calibrate-copy 512",
     "cal_write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/cal_write -o cp -- "
                  "python3 bench.py --calibrate-copy 512",
 }
+# The timed run is the last context the benchmark creates: its launches are
+# the ones after the last particle upload (k_widen3).
+TIMED_MARK = "pbf::k_widen3"
 
 
 def _short(name: str) -> str:
     return name.split("(")[0].replace("void ", "").strip()
 
 
-def _pmc(path: str, counter: str) -> dict:
+def _rows(path: str):
     import csv
-    out: dict = {}
     with open(path) as f:
-        for r in csv.DictReader(f):
-            if r["Counter_Name"] == counter:
-                out.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+        return list(csv.DictReader(f))
+
+
+def _timed(rows, order_key):
+    rows = sorted(rows, key=lambda r: int(r[order_key]))
+    last = max((i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == TIMED_MARK), default=-1)
+    return rows[last + 1:]
+
+
+def _pmc(path: str, counter: str) -> dict:
+    out: dict = {}
+    for r in _timed(_rows(path), "Dispatch_Id"):
+        if r["Counter_Name"] == counter:
+            out.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
     return out
 
 
 def summarize(d: str, workload: str) -> None:
     """Write profiles/<workload>_kernels.md, <workload>_hbm_traffic.md and
     pmc_traffic.json from the rocprofv3 runs of PROFILE_COMMANDS under d."""
-    import csv
     prof = os.path.join(ROOT, "profiles")
     os.makedirs(prof, exist_ok=True)
-    rows = list(csv.DictReader(open(os.path.join(d, "trace", "c2_kernel_stats.csv"))))
+    timed = _timed(_rows(os.path.join(d, "trace", "c2_kernel_trace.csv")), "Start_Timestamp")
+    per: dict = {}
+    for r in timed:
+        per.setdefault(_short(r["Kernel_Name"]), []).append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    total_us = sum(sum(v) for v in per.values())
     lines = [f"# Kernel times: {workload}", "",
              "Made by (from the repository root, on one MI355X):", "",
              "    " + PROFILE_COMMANDS["trace"], "",
-             "then `python3 bench.py --summarize gpurun_out`.  The run is the benchmark: a warm-up context, then",
-             "200 substeps of the 1,048,576-particle dam break from the initial scene (cap 60).", "",
-             "| kernel | calls | avg us | min us | max us | total ms | % |", "|---|---:|---:|---:|---:|---:|---:|"]
-    for r in rows:
-        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
-                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
-                     f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.1f} |")
+             "then `python3 bench.py --summarize gpurun_out`.  The command is the benchmark without the CPU",
+             "baseline, the variants and the warm-start comparison: a warm-up context advanced 100 substeps,",
+             "the autotuner on it, then the timed run of 200 substeps of the 1,048,576-particle dam break from",
+             "the initial scene (neighbour cap 60).  The table covers the timed run only (the launches after",
+             "its particle upload); the benchmark's own HIP-event times of the same launches are in its",
+             "`kernels_ms_per_step` and `roofline.avg_launch_ms`.", "",
+             "| kernel | calls | avg us | min us | max us | total ms | % of timed run |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    for k in sorted(per, key=lambda k: -sum(per[k])):
+        v = per[k]
+        lines.append(f"| `{k}` | {len(v)} | {statistics.mean(v):.1f} | {min(v):.1f} | {max(v):.1f} | "
+                     f"{sum(v) / 1e3:.2f} | {100 * sum(v) / total_us:.1f} |")
+    lines += ["", f"Sum of kernel time in the timed run: {total_us / 1e3:.1f} ms."]
     open(os.path.join(prof, f"{workload}_kernels.md"), "w").write("\n".join(lines) + "\n")
 
     fetch = _pmc(os.path.join(d, "pmc_fetch", "c2_counter_collection.csv"), "FETCH_SIZE")
@@ -207,7 +228,7 @@ def summarize(d: str, workload: str) -> None:
     lines += ["    " + PROFILE_COMMANDS[k] for k in ("fetch", "write", "cal_fetch", "cal_write")]
     lines += ["", f"Calibration: the 512 MiB copy reads FETCH_SIZE = {statistics.median(cal_f):.1f} KiB and "
               f"WRITE_SIZE = {statistics.median(cal_w):.1f} KiB per dispatch, so FETCH_SIZE is scaled by "
-              f"{f_scale:.3f} and WRITE_SIZE by {w_scale:.3f}.", "",
+              f"{f_scale:.3f} and WRITE_SIZE by {w_scale:.3f}.  Means over the timed run's launches.", "",
               "| kernel | launches | read MB/launch | written MB/launch | total MB/launch |",
               "|---|---:|---:|---:|---:|"]
     for k in sorted(fetch, key=lambda k: -sum(fetch[k])):

@@ -31,10 +31,30 @@ int check_hip(hipError_t e, const char* what) {
   return set_error(PBF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Events of finished launches are folded into the totals once a timer holds
+// kMaxPendingEvents pairs, so a long run keeps a bounded number of events
+// (the oldest launches have long finished: waiting on them costs nothing).
+constexpr size_t kMaxPendingEvents = 16;
+
+static void fold_oldest(KernelTimer& t, size_t count) {
+  for (size_t i = 0; i < count && i < t.pending.size(); ++i) {
+    auto& pr = t.pending[i];
+    hipEventSynchronize(pr.second);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, pr.first, pr.second);
+    t.total_ms += ms;
+    t.launches++;
+    hipEventDestroy(pr.first);
+    hipEventDestroy(pr.second);
+  }
+  t.pending.erase(t.pending.begin(), t.pending.begin() + std::min(count, t.pending.size()));
+}
+
 KernelTimer* timer_begin(pbf_context* c, const char* name) {
   if (c->capturing) return nullptr;
   for (auto& t : c->timers) {
     if (t.name == name) {
+      if (t.pending.size() >= kMaxPendingEvents) fold_oldest(t, kMaxPendingEvents / 2);
       hipEvent_t a, b;
       hipEventCreate(&a);
       hipEventCreate(&b);
@@ -51,18 +71,7 @@ void timer_end(pbf_context* c, KernelTimer* t) {
 }
 
 static void timers_resolve(pbf_context* c) {
-  for (auto& t : c->timers) {
-    for (auto& pr : t.pending) {
-      hipEventSynchronize(pr.second);
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, pr.first, pr.second);
-      t.total_ms += ms;
-      t.launches++;
-      hipEventDestroy(pr.first);
-      hipEventDestroy(pr.second);
-    }
-    t.pending.clear();
-  }
+  for (auto& t : c->timers) fold_oldest(t, t.pending.size());
 }
 
 // smallest float t with sqrtf(t) > h (sqrt with round-to-nearest is monotone)

@@ -247,6 +247,28 @@ def summarize(d: str, workload: str) -> None:
 WARM_MODEL = os.path.join(ROOT, "sph-cpp_amd", "warm_model.npy")
 
 
+def double_dam_break_run(pbf, scenes, steps, seed):
+    """Two 128 x 256 x 128 blocks on opposite walls of a 25.6 x 12.8 x 12.8 box
+    (8,388,608 particles), list cap 64 with the cap's rejections counted."""
+    params, pos, vel = scenes.double_dam_break(seed=seed, count_overflow=True)
+    n = pos.shape[0]
+    g = pbf.Particles(params)
+    g.add(pos, pos, vel)
+    g.sync()
+    t0 = time.perf_counter()
+    for st in range(steps):
+        if st % 25 == 0:
+            print(f"double dam break: substep {st}", file=sys.stderr, flush=True)
+        g.step(1)
+    g.sync()
+    wall = time.perf_counter() - t0
+    s = g.stats()
+    g.close()
+    return {"particles": n, "steps": steps, "neighbor_cap": 64, "value": n * steps / wall,
+            "ms_per_step": wall * 1e3 / steps, "neighbor_overflow_last_step": s["neighbor_overflow"],
+            "max_density_error_last_step": round(s["max_density_error"], 4)}
+
+
 def batched_run(pbf, scenes, steps, seed, num_scenes=256):
     """Independent dam breaks of 4,096..65,536 particles (log-uniform) in one
     batched context: eager steps and graph-replayed steps, each from the
@@ -440,6 +462,7 @@ def main(argv=None):
                                       "note": "same scene with a 128-entry neighbour list (no list saturation)"}}
     if args.variants:
         out["variants"]["batched_256_scenes"] = batched_run(pbf, scenes, 100, args.seed)
+        out["variants"]["double_dam_break_8M"] = double_dam_break_run(pbf, scenes, 100, args.seed)
     if args.warm:
         out["warm_start"] = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps)
     if not args.no_cpu:

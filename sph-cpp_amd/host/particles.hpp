@@ -33,6 +33,11 @@ inline pbf_index_span c_span(IndexSpan s) { return pbf_index_span{s.begin, s.end
 class Particles {
  public:
   explicit Particles(const pbf_params& parameters) : params_(parameters) { check(pbf_create(&params_, &ctx_)); }
+  // independent scenes in one context: boxes holds 3 extents per scene
+  Particles(const pbf_params& parameters, const std::vector<float>& boxes) : params_(parameters) {
+    if (boxes.size() % 3 != 0 || boxes.empty()) throw std::runtime_error("boxes: 3 extents per scene");
+    check(pbf_create_batched(&params_, (uint32_t)(boxes.size() / 3), boxes.data(), &ctx_));
+  }
   ~Particles() { pbf_destroy(ctx_); }
   Particles(const Particles&) = delete;
   Particles& operator=(const Particles&) = delete;
@@ -47,6 +52,10 @@ class Particles {
     check(pbf_add(ctx_, positions, position_stars, velocities, count));
   }
   void remove(std::size_t count) { check(pbf_remove(ctx_, count)); }
+  void add_to_scene(uint32_t scene, const float* positions, const float* position_stars, const float* velocities,
+                    std::size_t count) {
+    check(pbf_add_to_scene(ctx_, scene, positions, position_stars, velocities, count));
+  }
 
   // renderer re-broadcast (Simulation/Source/main.cpp:62-63)
   void set_parameters(const pbf_params& parameters) {
@@ -74,8 +83,24 @@ class Particles {
   void compute_vorticity(IndexSpan s) { check(pbf_compute_vorticity(ctx_, c_span(s))); }
   void apply_vorticity(IndexSpan s) { check(pbf_apply_vorticity(ctx_, c_span(s))); }
 
-  // the whole main.cpp step body, fused
+  // the whole main.cpp step body, fused; through a replayed HIP graph
   void step(unsigned nsteps) { check(pbf_step(ctx_, nsteps)); }
+  void step_graph(unsigned nsteps) { check(pbf_step_graph(ctx_, nsteps)); }
+  // one step iterating the pressure solve to a density-error tolerance
+  pbf_solve_info solve(float tolerance, unsigned max_iterations, int warm_start = PBF_WARM_NONE) {
+    check(pbf_solve(ctx_, tolerance, max_iterations, warm_start));
+    pbf_solve_info info;
+    check(pbf_get_solve_info(ctx_, &info));
+    return info;
+  }
+  void set_warm_lambdas(const float* lambdas) { check(pbf_set_warm_lambdas(ctx_, lambdas)); }
+  void set_warm_model(pbf_warm_model* model) { check(pbf_set_warm_model(ctx_, model)); }
+  pbf_tuning autotune(unsigned steps) {
+    pbf_tuning t;
+    double ms = 0.0;
+    check(pbf_autotune(ctx_, steps, &t, &ms));
+    return t;
+  }
   void sync() { check(pbf_sync(ctx_)); }
 
   // array getters (particles.h:91-141), original order

@@ -12,7 +12,10 @@
 // of AdiosWriter (adios_writer.h:78-114).
 //
 //   sph_run [--lattice NXxNYxNZ] [--box X Y Z] [--steps N] [--iters I] [--cap K]
-//           [--seed S] [--fused] [--write FILE] [--every K]
+//           [--seed S] [--fused] [--graph] [--tolerance T] [--write FILE] [--every K]
+// --graph replays the fused step through a HIP graph; --tolerance T runs
+// each step as pbf_solve to a max density error of T (at most 64 iterations,
+// warm-started from the previous step's multiplier sums).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -36,6 +39,8 @@ struct Options {
   int cap = 60;
   unsigned seed = 0;
   bool fused = false;
+  bool graph = false;
+  float tolerance = -1.0f;
   std::string write;
   int every = 1;
 };
@@ -62,6 +67,10 @@ Options parse(int argc, char** argv) {
       o.seed = (unsigned)std::strtoul(next(), nullptr, 10);
     } else if (a == "--fused") {
       o.fused = true;
+    } else if (a == "--graph") {
+      o.graph = true;
+    } else if (a == "--tolerance") {
+      o.tolerance = std::strtof(next(), nullptr);
     } else if (a == "--write") {
       o.write = next();
     } else if (a == "--every") {
@@ -135,7 +144,12 @@ int main(int argc, char** argv) {
 
     const auto t0 = std::chrono::steady_clock::now();
     for (int frame = 0; frame < o.steps; ++frame) {
-      if (o.fused) {
+      if (o.tolerance >= 0.0f) {
+        particles.solve(o.tolerance, 64, frame ? PBF_WARM_PREVIOUS : PBF_WARM_NONE);
+      } else if (o.graph) {
+        // the graph holds two steps: advance on even frames, covering the odd ones
+        if (frame % 2 == 0) particles.step_graph(frame + 1 < o.steps ? 2 : 1);
+      } else if (o.fused) {
         particles.step(1);
       } else {
         // Simulation/Source/main.cpp:71-124
@@ -156,14 +170,16 @@ int main(int argc, char** argv) {
         particles.apply_viscosity(all);
         particles.update_positions(all);
       }
-      if (!o.write.empty() && (frame + 1) % o.every == 0) particles.write_positions(o.write);
+      const bool advanced = !o.graph || frame % 2 == 1 || frame + 1 == o.steps;
+      if (!o.write.empty() && advanced && (frame + 1) % o.every == 0) particles.write_positions(o.write);
     }
     particles.sync();
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const pbf_stats st = particles.stats();
     std::printf("{\"particles\": %zu, \"steps\": %d, \"mode\": \"%s\", \"seconds\": %.6f, "
                 "\"particle_substeps_per_s\": %.6e, \"max_density_error\": %.6f, \"neighbor_overflow\": %llu}\n",
-                n, o.steps, o.fused ? "fused" : "per-pass", secs, (double)n * o.steps / secs,
+                n, o.steps, o.tolerance >= 0.0f ? "solve" : o.graph ? "graph" : o.fused ? "fused" : "per-pass", secs,
+                (double)n * o.steps / secs,
                 st.max_density_error, (unsigned long long)st.neighbor_overflow);
     return 0;
   } catch (const std::exception& e) {

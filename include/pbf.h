@@ -269,6 +269,9 @@ int pbf_warm_model_train(pbf_warm_model* model, uint32_t epochs, uint32_t batch,
 int pbf_warm_model_predict(pbf_warm_model* model, const float* features, uint64_t count, float* out);
 /* copy the model's weights into the context (used by PBF_WARM_LEARNED) */
 int pbf_set_warm_model(pbf_context* ctx, pbf_warm_model* model);
+/* scale of the network's correction: Lambda = Lambda_prev + gain * S * y (default 1, range [0, 4];
+ * 0 is PBF_WARM_PREVIOUS) */
+int pbf_set_warm_gain(pbf_context* ctx, float gain);
 /* features of the last warm-started solve, n x 8, original order */
 int pbf_get_warm_features(pbf_context* ctx, float* out);
 

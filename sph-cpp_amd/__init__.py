@@ -174,6 +174,7 @@ def lib():
     _sig(L, "pbf_get_warm_features", [_P, _F])
     _sig(L, "pbf_set_lambda_sums", [_P, _F])
     _sig(L, "pbf_set_warm_model", [_P, _P])
+    _sig(L, "pbf_set_warm_gain", [_P, C.c_float])
     _sig(L, "pbf_warm_model_create", [C.c_uint64, C.POINTER(_P)])
     _sig(L, "pbf_warm_model_destroy", [_P])
     _sig(L, "pbf_warm_model_init", [_P, C.c_uint64])
@@ -427,8 +428,9 @@ class Particles:
         _check(lib().pbf_get_warm_features(self._ctx, out.ctypes.data_as(_F)))
         return out
 
-    def set_warm_model(self, model: "WarmModel") -> None:
+    def set_warm_model(self, model: "WarmModel", gain: float = 1.0) -> None:
         _check(lib().pbf_set_warm_model(self._ctx, model._h))
+        _check(lib().pbf_set_warm_gain(self._ctx, float(gain)))
 
     def set_tuning(self, block_size: int = 256, heavy_budget: int = 1536, heavy_blocks: int = 2048) -> None:
         t = Tuning(block_size, heavy_budget, heavy_blocks)

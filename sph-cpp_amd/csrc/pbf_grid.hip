@@ -364,6 +364,10 @@ struct WalkCells {
   float cx[3], cy[3], cz[3];
 };
 
+// a[i] for a loop index i without indexing the array at run time (which
+// would put the array in scratch memory)
+__device__ __forceinline__ float pick3(const float (&a)[3], int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
 __device__ __forceinline__ WalkCells walk_cells(f3 ps, float s) {
   WalkCells w;
 #pragma unroll
@@ -409,7 +413,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
         for (int k = 0; k < 3; ++k) {
-          const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], G, K);
+          const uint32_t cell = cell_key(pick3(w.cx, i), pick3(w.cy, j), pick3(w.cz, k), G, K);
           if (cell < K.ncells) total += start[cell + 1] - start[cell];
         }
     if (total > K.heavy_budget) {
@@ -427,7 +431,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 3; ++k) {
-        const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], G, K);
+        const uint32_t cell = cell_key(pick3(w.cx, i), pick3(w.cy, j), pick3(w.cz, k), G, K);
         if (cell >= K.ncells) continue;
         const uint32_t b = start[cell], e = start[cell + 1];
         // candidates of a cell are contiguous: load 8 at a time, test in order
@@ -487,7 +491,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
         for (int k = 0; k < 3; ++k) {
-          const uint32_t cell = cell_key(w.cx[i], w.cy[j], w.cz[k], G, K);
+          const uint32_t cell = cell_key(pick3(w.cx, i), pick3(w.cy, j), pick3(w.cz, k), G, K);
           if (cell >= K.ncells) continue;
           const uint32_t b = start[cell], e = start[cell + 1];
           if (e - b > kBoxMin && cell_far(box, cell, ps, K.ss)) continue;

@@ -161,6 +161,7 @@ struct pbf_context {
   short *mw1 = nullptr, *mw2 = nullptr;  // learned warm start: bf16 weights
   float* mp = nullptr;                   // and the fp32 parameters
   bool has_model = false;
+  float warm_gain = 1.0f;  // learned warm start: Lambda_prev + warm_gain * S * net
   uint32_t nscenes = 1;
   std::vector<pbf::SceneGeo> scene_geo;   // host copy (batched contexts)
   pbf::SceneGeo* scenes = nullptr;        // device copy

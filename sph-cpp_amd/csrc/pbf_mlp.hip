@@ -388,7 +388,8 @@ void launch_mlp_warm(pbf_context* c) {
   const uint32_t groups = (n + 15) / 16;
   const uint32_t blocks = std::min<uint32_t>(groups, 4096u);
   hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(64), 0, c->stream, n, (const float*)c->feat,
-                     MlpW{c->mw1, c->mw2, nullptr, c->mp}, kWarmLambdaScale, (const float*)c->lsuma, c->wlam);
+                     MlpW{c->mw1, c->mw2, nullptr, c->mp}, c->warm_gain * kWarmLambdaScale, (const float*)c->lsuma,
+                     c->wlam);
   timer_end(c, t);
 }
 
@@ -606,6 +607,13 @@ int pbf_warm_model_predict(pbf_warm_model* M, const float* features, uint64_t co
   hipFree(f);
   hipFree(o);
   return rc;
+}
+
+int pbf_set_warm_gain(pbf_context* c, float gain) {
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  if (!(gain >= 0.0f && gain <= 4.0f)) return set_error(PBF_ERR_ARG, "gain must be in [0, 4]");
+  c->warm_gain = gain;
+  return PBF_OK;
 }
 
 int pbf_set_warm_model(pbf_context* c, pbf_warm_model* M) {

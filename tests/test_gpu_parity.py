@@ -357,3 +357,44 @@ def test_serial_mode_difference(pbf, scenes):
         rms_gpu, rms_fp = np.sqrt((a ** 2).mean()), np.sqrt((b ** 2).mean())
         print(f"substep {k}: GPU vs serial RMS {rms_gpu:.3e} h, float vs double RMS {rms_fp:.3e} h", flush=True)
         assert rms_gpu <= 3.0 * rms_fp, (k, rms_gpu, rms_fp)
+
+
+def test_surface_tension_equals_oracle(pbf, scenes):
+    """apply_surface_tension (particles.h:400-431) with gamma != 0, per pass and
+    inside the fused step, against the oracle bit for bit (the colour field is
+    finished before any velocity changes, so the pass has no ordering issue)."""
+    p, pos, vel = _small_scene(scenes, seed=13, gamma=0.5)
+    assert p.gamma > 0
+    g, o = _pair(pbf, p, pos, vel)
+    g.step(2)
+    o.step(2)
+    _compare_state(g, o, "fused step, gamma 0.5")
+    g.apply_external_forces(); o.pass_(O.FORCES)
+    g.predict_positions(); o.pass_(O.PREDICT)
+    g.find_neighbors(); o.pass_(O.FIND)
+    g.compute_densities(); o.pass_(O.DENSITY)
+    g.apply_surface_tension(); o.pass_(O.TENSION)
+    assert_bitwise(g.scratch(), o.state()["scratch"], "colour field")
+    assert_bitwise(g.velocities(), o.state()["v"], "velocities after surface tension")
+
+
+def test_emitter_injection_mid_run(pbf, scenes):
+    """Particles added between steps (the distributor's emitter,
+    distributor.h:357-369) join the next neighbour search; the run continues
+    bit for bit with the oracle given the same appended state."""
+    p, pos, vel = _small_scene(scenes, seed=17, capacity=1920 + 200)
+    g, o = _pair(pbf, p, pos, vel)
+    g.step(4)
+    o.step(4)
+    rng = np.random.default_rng(3)
+    new = (np.array([0.2, 0.6, 0.2], np.float32) + rng.uniform(0, 0.2, size=(200, 3))).astype(np.float32)
+    nv = np.tile(np.array([[1.0, -0.5, 0.0]], np.float32), (200, 1))
+    g.add(new, new, nv)
+    s = o.state()
+    x = np.concatenate([s["x"].astype(np.float32), new])
+    xs = np.concatenate([s["xs"].astype(np.float32), new])
+    v = np.concatenate([s["v"].astype(np.float32), nv])
+    o.set_state(x, xs, v)
+    g.step(4)
+    o.step(4)
+    _compare_state(g, o, "after injection", fields=("x", "v"))

@@ -398,3 +398,29 @@ def test_emitter_injection_mid_run(pbf, scenes):
     g.step(4)
     o.step(4)
     _compare_state(g, o, "after injection", fields=("x", "v"))
+
+
+def test_empty_and_single_particle(pbf, scenes):
+    """Edge sizes: an empty context steps, solves and answers getters; one
+    particle (no neighbours) and particles on the box faces match the oracle."""
+    p, pos, vel = _small_scene(scenes, seed=1)
+    g = pbf.Particles(p)
+    g.step(2)
+    g.step_graph(2)
+    g.solve(0.0, 3)
+    assert g.positions().shape == (0, 3)
+    assert g.stats()["neighbor_overflow"] == 0
+    g.close()
+    one = np.array([[0.7, 0.3, 0.4]], np.float32)
+    g, o = _pair(pbf, p, one, np.zeros_like(one))
+    g.step(3)
+    o.step(3)
+    _compare_state(g, o, "single particle")
+    # particles exactly on the faces and corners of the box (clamp and bin edges)
+    bmax = np.array([p.boundary_max[i] for i in range(3)], np.float32)
+    faces = np.array([[0, 0, 0], bmax, [bmax[0], 0, 0], [0, bmax[1], 0], [0, 0, bmax[2]],
+                      [bmax[0] / 2, bmax[1], bmax[2] / 2], [0.05, 0.05, 0.05], bmax - 0.01], np.float32)
+    g, o = _pair(pbf, p, faces, np.zeros_like(faces))
+    g.step(3)
+    o.step(3)
+    _compare_state(g, o, "box faces")

@@ -309,8 +309,12 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
     return rc;
   }
   const uint64_t N = c->capacity;
-  if ((rc = dalloc(&c->x, N)) || (rc = dalloc(&c->xs, N)) || (rc = dalloc(&c->v, N)) || (rc = dalloc(&c->xa, N)) ||
-      (rc = dalloc(&c->xsa, N)) || (rc = dalloc(&c->va, N)) || (rc = dalloc(&c->xt, N)) ||
+  // position buffers carry kRowPad rows of padding: the neighbour walk loads
+  // candidates 8 at a time past the end of a cell's range
+  const uint64_t NP = N + kRowPad;
+  if ((rc = dalloc(&c->x, NP)) || (rc = dalloc(&c->xs, NP)) || (rc = dalloc(&c->v, NP)) ||
+      (rc = dalloc(&c->xa, NP)) || (rc = dalloc(&c->xsa, NP)) || (rc = dalloc(&c->va, NP)) ||
+      (rc = dalloc(&c->xt, NP)) ||
       (rc = dalloc(&c->scr, N)) || (rc = dalloc(&c->scra, N)) || (rc = dalloc(&c->rho, N)) ||
       (rc = dalloc(&c->lam, N)) || (rc = dalloc(&c->rhoa, N)) || (rc = dalloc(&c->lama, N)) ||
       (rc = dalloc(&c->oid, N)) || (rc = dalloc(&c->oida, N)) || (rc = dalloc(&c->key, N)) ||

@@ -271,8 +271,9 @@ void launch_reorder(pbf_context* c) {
 // whose box is at least s away from the walking particle: float subtraction,
 // squaring and addition are monotone, so every candidate's d^2 (computed with
 // the same operations) is at least the box bound and none could be accepted.
-// The lists are unchanged.  The wavefront builder, which handles the particles
-// next to crowded cells, tests cells of more than kBoxMin particles.
+// The lists are unchanged.  Only the wavefront builder tests cells (of more
+// than kBoxMin particles): in the one-lane walk a skip saves time only when
+// all 64 lanes skip, which is rare.
 constexpr uint32_t kBoxMin = 32;
 
 __device__ __forceinline__ uint32_t ord_enc(float f) {
@@ -435,6 +436,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
         if (cell >= K.ncells) continue;
         const uint32_t b = start[cell], e = start[cell + 1];
         // candidates of a cell are contiguous: load 8 at a time, test in order
+        // (position buffers carry 8 rows of padding, so the loads need no clamp)
         for (uint32_t q0 = b; q0 < e; q0 += 8) {
           scanned += 8;
           if (scanned > K.heavy_budget && !K.count_overflow) {
@@ -443,7 +445,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
           }
           float4 t[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) t[u] = xs[min(q0 + (uint32_t)u, e - 1u)];
+          for (int u = 0; u < 8; ++u) t[u] = xs[q0 + (uint32_t)u];
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const uint32_t q = q0 + (uint32_t)u;

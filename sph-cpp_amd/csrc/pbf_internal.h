@@ -90,6 +90,7 @@ enum Counter {
 };
 
 constexpr uint32_t kMaxSolveSlots = PBF_MAX_SOLVE_ITERATIONS;
+constexpr uint64_t kRowPad = 8;  // padding rows of the float4 particle buffers
 constexpr float kWarmLambdaScale = PBF_WARM_LAMBDA_SCALE;
 
 // Per-launch control of the Jacobi loop kernels.  A plain fixed-iteration

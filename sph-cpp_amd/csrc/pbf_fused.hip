@@ -83,15 +83,17 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
+    // branch-free: every entry is stored, an accepted one in row nc and any
+    // other in the spare row cap (never read)
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const float d2 = magsq3(sub3(xp, xyz(t[u])));
-      if (k0 + u >= cnt || hack_at >= 0 || !within_h(d2, K)) continue;
-      if (d2 < K.T_hack) {  // particles.h:304: handled by the plain walk below
-        hack_at = k0 + u;
-        continue;
-      }
-      s_list[(nc++) * kStageBlock + lane] = (int32_t)q[u];
+      const bool w = (k0 + u < cnt) & (hack_at < 0) & within_h(d2, K);
+      const bool hk = w & (d2 < K.T_hack);  // particles.h:304: handled by the plain walk below
+      const bool acc = w & !hk;
+      s_list[(acc ? nc : K.cap) * kStageBlock + lane] = (int32_t)q[u];
+      nc += acc ? 1 : 0;
+      hack_at = hk ? k0 + u : hack_at;
     }
   }
   cnb_count[p] = nc;
@@ -212,7 +214,7 @@ __global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, in
                                                                   uint32_t* __restrict__ moved,
                                                                   unsigned long long* __restrict__ counters) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
-  extern __shared__ int32_t s_list[];  // [cap][kStageBlock]
+  extern __shared__ int32_t s_list[];  // [cap + 1][kStageBlock]
   const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   float e = 0.0f;
@@ -496,7 +498,7 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
                                                             const int32_t* __restrict__ nbr_count,
                                                             int32_t* __restrict__ cnb,
                                                             int32_t* __restrict__ cnb_count) {
-  extern __shared__ int32_t s_list[];  // [cap][kStageBlock]
+  extern __shared__ int32_t s_list[];  // [cap + 1][kStageBlock]
   const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -513,7 +515,9 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const float d2 = magsq3(sub3(xp, xyz(t[u])));
-      if (k0 + u < cnt && within_h(d2, K)) s_list[(nc++) * kStageBlock + lane] = (int32_t)q[u];
+      const bool acc = (k0 + u < cnt) & within_h(d2, K);
+      s_list[(acc ? nc : K.cap) * kStageBlock + lane] = (int32_t)q[u];  // spare row cap when rejected
+      nc += acc ? 1 : 0;
     }
   }
   cnb_count[p] = nc;
@@ -624,7 +628,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     if (!n_) return;                                                                                        \
     KernelTimer* t_ = timer_begin((c), name);                                                               \
     hipLaunchKernelGGL(kern, dim3((n_ + kStageBlock - 1) / kStageBlock), dim3(kStageBlock),                 \
-                       (uint32_t)(kStageBlock * (c)->K.cap * sizeof(int32_t)), (c)->stream, n_, __VA_ARGS__); \
+                       (uint32_t)(kStageBlock * ((c)->K.cap + 1) * sizeof(int32_t)), (c)->stream, n_, __VA_ARGS__); \
     timer_end((c), t_);                                                                                     \
   } while (0)
 

@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
                                                                int32_t* __restrict__ nbr_count,
                                                                uint32_t* __restrict__ heavy,
                                                                unsigned long long* __restrict__ counters) {
-  extern __shared__ int32_t s_list[];  // [cap][kFillBlock]
+  extern __shared__ int32_t s_list[];  // [cap + 1][kFillBlock]
   const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -446,28 +446,30 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
           float4 t[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) t[u] = xs[q0 + (uint32_t)u];
+          // branch-free acceptance: every candidate is stored, an accepted one
+          // in row min(count, cap) and a rejected one in the spare row cap;
+          // rows >= cap are never read
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const uint32_t q = q0 + (uint32_t)u;
-            if (q >= e || q == p) continue;
             const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
-            if (d2 < K.ss) {
-              if (count < cap) {
-                s_list[count * kFillBlock + lane] = (int32_t)q;
-                ++count;
-                // a full list cannot change any more; only the overflow count could
-                if (count == cap && !K.count_overflow) goto done;
-              } else {
-                ++ovf;
-              }
-            }
+            const bool acc = (q < e) & (q != p) & (d2 < K.ss);
+            const int row = acc ? min(count, cap) : cap;
+            s_list[row * kFillBlock + lane] = (int32_t)q;
+            count += acc ? 1 : 0;
           }
+          // a full list cannot change any more; only the overflow count could
+          if (count >= cap && !K.count_overflow) goto done;
         }
       }
 done:
+  if (count > cap) {
+    ovf = (uint32_t)(count - cap);
+    count = cap;
+  }
   nbr_count[p] = count;
   for (int k = 0; k < count; ++k) nbr[(uint64_t)k * n + p] = s_list[k * kFillBlock + lane];
-  if (ovf) atomicAdd(&counters[C_OVERFLOW], (unsigned long long)ovf);
+  if (ovf && K.count_overflow) atomicAdd(&counters[C_OVERFLOW], (unsigned long long)ovf);
 }
 
 // One wavefront per heavy particle; waves stride over the heavy list, whose
@@ -523,7 +525,7 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
   if (!n) return;
   KernelTimer* t = timer_begin(c, "fill_neighbors");
   hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n, kFillBlock)), dim3(kFillBlock),
-                     (uint32_t)(kFillBlock * c->K.cap * sizeof(int32_t)), c->stream, n, (uint32_t)fb,
+                     (uint32_t)(kFillBlock * (c->K.cap + 1) * sizeof(int32_t)), c->stream, n, (uint32_t)fb,
                      (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
                      (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count, c->heavy, c->counters);
   timer_end(c, t);

@@ -274,7 +274,14 @@ void launch_reorder(pbf_context* c) {
 // The lists are unchanged.  Only the wavefront builder tests cells (of more
 // than kBoxMin particles): in the one-lane walk a skip saves time only when
 // all 64 lanes skip, which is rare.
+//
+// The same pass writes the box of every aligned run of 64 slots (a chunk).
+// Slots inside a cell are in original-index order, not spatial order, so a
+// crowded cell (thousands of particles after a collapse) can be near a
+// particle while most of its chunks are not; the wavefront builder tests 64
+// chunk boxes at once and gathers only the chunks that can hold a neighbour.
 constexpr uint32_t kBoxMin = 32;
+constexpr uint32_t kChunkMin = 128;  // cells above this size are walked chunk by chunk
 
 __device__ __forceinline__ uint32_t ord_enc(float f) {
   const uint32_t u = __float_as_uint(f);
@@ -292,7 +299,8 @@ __global__ void k_box_init(uint32_t* __restrict__ box, uint64_t count) {
 // Slots are in cell order: each wavefront reduces its runs of equal keys and
 // the first lane of a run updates the cell's box.
 __global__ void __launch_bounds__(kBlock) k_cell_box(uint32_t n, uint32_t ncells, const uint32_t* __restrict__ key,
-                                                     const float4* __restrict__ xs, uint32_t* __restrict__ box) {
+                                                     const float4* __restrict__ xs, uint32_t* __restrict__ box,
+                                                     float4* __restrict__ chbox) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool valid = p < n;
@@ -300,6 +308,30 @@ __global__ void __launch_bounds__(kBlock) k_cell_box(uint32_t n, uint32_t ncells
   const float4 t = valid ? xs[p] : make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t mn[3] = {ord_enc(t.x), ord_enc(t.y), ord_enc(t.z)};
   uint32_t mx[3] = {mn[0], mn[1], mn[2]};
+  // chunk box over the wavefront's 64 slots (particles outside the grid are
+  // never walked and left out)
+  {
+    const bool in = valid && k < ncells;
+    uint32_t cmn[3], cmx[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      cmn[a] = in ? mn[a] : 0xFFFFFFFFu;
+      cmx[a] = in ? mx[a] : 0u;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        cmn[a] = min(cmn[a], (uint32_t)__shfl_xor(cmn[a], d, 64));
+        cmx[a] = max(cmx[a], (uint32_t)__shfl_xor(cmx[a], d, 64));
+      }
+    }
+    if (lane == 0 && p < n) {
+      const uint32_t ch = p >> 6;
+      chbox[2 * ch] = make_float4(ord_dec(cmn[0]), ord_dec(cmn[1]), ord_dec(cmn[2]), 0.f);
+      chbox[2 * ch + 1] = make_float4(ord_dec(cmx[0]), ord_dec(cmx[1]), ord_dec(cmx[2]), 0.f);
+    }
+  }
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const uint32_t kk = __shfl_down(k, d, 64);
@@ -335,19 +367,20 @@ void launch_cell_box(pbf_context* c) {
   const uint64_t count = 6ull * c->K.ncells;
   hipLaunchKernelGGL(k_box_init, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, c->cbox, count);
   hipLaunchKernelGGL(k_cell_box, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, c->K.ncells,
-                     (const uint32_t*)c->key, (const float4*)c->xs, c->cbox);
+                     (const uint32_t*)c->key, (const float4*)c->xs, c->cbox, c->chbox);
   timer_end(c, t);
 }
 
-// true when no particle of the cell can lie within s of ps (see above)
-__device__ __forceinline__ bool cell_far(const uint32_t* __restrict__ box, uint32_t cell, f3 ps, float ss) {
-  const uint32_t* b = box + 6ull * cell;
-  const float lx = ord_dec(b[0]), ly = ord_dec(b[1]), lz = ord_dec(b[2]);
-  const float hx = ord_dec(b[3]), hy = ord_dec(b[4]), hz = ord_dec(b[5]);
+// true when no point of the box [l, h] can lie within s of ps (see above)
+__device__ __forceinline__ bool box_far(float lx, float ly, float lz, float hx, float hy, float hz, f3 ps, float ss) {
   const float dx = ps.x < lx ? lx - ps.x : (ps.x > hx ? ps.x - hx : 0.0f);
   const float dy = ps.y < ly ? ly - ps.y : (ps.y > hy ? ps.y - hy : 0.0f);
   const float dz = ps.z < lz ? lz - ps.z : (ps.z > hz ? ps.z - hz : 0.0f);
   return (dx * dx + dy * dy) + dz * dz >= ss;
+}
+__device__ __forceinline__ bool cell_far(const uint32_t* __restrict__ box, uint32_t cell, f3 ps, float ss) {
+  const uint32_t* b = box + 6ull * cell;
+  return box_far(ord_dec(b[0]), ord_dec(b[1]), ord_dec(b[2]), ord_dec(b[3]), ord_dec(b[4]), ord_dec(b[5]), ps, ss);
 }
 
 // ---------------------------------------------------------------------------
@@ -476,7 +509,8 @@ done:
 // length is read once per wave (every wave exits after the last entry).
 __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4* __restrict__ xs,
                                                        const uint32_t* __restrict__ start,
-                                                       const uint32_t* __restrict__ box, KParams K,
+                                                       const uint32_t* __restrict__ box,
+                                                       const float4* __restrict__ chbox, KParams K,
                                                        int32_t* __restrict__ nbr, int32_t* __restrict__ nbr_count,
                                                        const uint32_t* __restrict__ heavy,
                                                        unsigned long long* __restrict__ counters) {
@@ -499,17 +533,45 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
           if (cell >= K.ncells) continue;
           const uint32_t b = start[cell], e = start[cell + 1];
           if (e - b > kBoxMin && cell_far(box, cell, ps, K.ss)) continue;
-          for (uint32_t base = b; base < e; base += 64) {
-            const uint32_t q = base + lane;
-            bool acc = false;
-            if (q < e && q != p) acc = magsq3(sub3(ps, ld3(xs, q))) < K.ss;
-            const uint64_t m = __ballot(acc);
-            if (acc) {
-              const uint32_t pos = count + (uint32_t)__popcll(m & below);
-              if (pos < cap) nbr[(uint64_t)pos * n + p] = (int32_t)q;
+          if (e - b <= kChunkMin) {
+            for (uint32_t base = b; base < e; base += 64) {
+              const uint32_t q = base + lane;
+              bool acc = false;
+              if (q < e && q != p) acc = magsq3(sub3(ps, ld3(xs, q))) < K.ss;
+              const uint64_t m = __ballot(acc);
+              if (acc) {
+                const uint32_t pos = count + (uint32_t)__popcll(m & below);
+                if (pos < cap) nbr[(uint64_t)pos * n + p] = (int32_t)q;
+              }
+              count += (uint32_t)__popcll(m);
+              if (count >= cap && !K.count_overflow) goto done;
             }
-            count += (uint32_t)__popcll(m);
-            if (count >= cap && !K.count_overflow) goto done;
+            continue;
+          }
+          // crowded cell: each lane tests one chunk box, then the wavefront
+          // gathers the chunks that may hold a neighbour, in slot order
+          const uint32_t c_last = (e - 1) >> 6;
+          for (uint32_t c0 = b >> 6; c0 <= c_last; c0 += 64) {
+            const uint32_t ch = c0 + lane;
+            bool near = false;
+            if (ch <= c_last) {
+              const float4 lo = chbox[2 * ch], hi = chbox[2 * ch + 1];
+              near = !box_far(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, ps, K.ss);
+            }
+            uint64_t todo = __ballot(near);
+            while (todo) {
+              const uint32_t q = ((c0 + (uint32_t)__builtin_ctzll(todo)) << 6) + lane;
+              todo &= todo - 1ull;
+              bool acc = false;
+              if (q >= b && q < e && q != p) acc = magsq3(sub3(ps, ld3(xs, q))) < K.ss;
+              const uint64_t m = __ballot(acc);
+              if (acc) {
+                const uint32_t pos = count + (uint32_t)__popcll(m & below);
+                if (pos < cap) nbr[(uint64_t)pos * n + p] = (int32_t)q;
+              }
+              count += (uint32_t)__popcll(m);
+              if (count >= cap && !K.count_overflow) goto done;
+            }
           }
         }
   done:
@@ -533,8 +595,8 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
   t = timer_begin(c, "fill_heavy");
   const uint32_t blocks = std::min<uint32_t>(nblocks((uint64_t)n * 64), c->tune.heavy_blocks);
   hipLaunchKernelGGL(k_fill_heavy, dim3(blocks), dim3(kBlock), 0, c->stream, n, (const float4*)c->xs,
-                     (const uint32_t*)c->cell_start, (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count,
-                     (const uint32_t*)c->heavy, c->counters);
+                     (const uint32_t*)c->cell_start, (const uint32_t*)c->cbox, (const float4*)c->chbox, c->K,
+                     c->nbr, c->nbr_count, (const uint32_t*)c->heavy, c->counters);
   timer_end(c, t);
 }
 

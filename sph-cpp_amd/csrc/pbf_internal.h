@@ -151,6 +151,7 @@ struct pbf_context {
   uint32_t* dirty = nullptr;  // == dirty_stamp: the particle or one of its neighbours moved
   uint32_t* cell_start = nullptr;
   uint32_t* cbox = nullptr;  // [6][cells] bounding box of each cell's particles (encoded)
+  float4* chbox = nullptr;   // [2][n/64] bounding box (min, max) of each aligned run of 64 slots
   uint32_t* scan_partials = nullptr;
   int32_t *nbr = nullptr, *nbr_count = nullptr;
   int32_t *cnb = nullptr, *cnb_count = nullptr;

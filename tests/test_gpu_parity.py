@@ -295,6 +295,37 @@ def test_dense_cluster_heavy_lists_bit_exact(pbf, scenes):
     _compare_state(g, o, "dense cluster step")
 
 
+@pytest.mark.parametrize("budget", [64, 1536])
+def test_crowded_cells_chunk_walk_bit_exact(pbf, scenes, budget):
+    """Crowded cells whose slots are spatially coherent in index order (tight
+    clusters laid out x fastest, as a collapsed lattice leaves them): the
+    wavefront builder skips 64-slot chunks by their bounding boxes, and the
+    lists, their order and the overflow count must still be the reference's."""
+    rng = np.random.default_rng(33)
+    centres = 0.25 + 0.02 * np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(6), indexing="ij"),
+                                     axis=-1)[..., ::-1].reshape(-1, 3)
+    crowd = (centres[:, None, :] + rng.uniform(-0.004, 0.004, size=(centres.shape[0], 40, 3))).reshape(-1, 3)
+    sparse = scenes.lattice((12, 12, 12), origin=(0.025, 0.025, 0.025))
+    pos = np.concatenate([crowd, sparse]).astype(np.float32)
+    for count_overflow in (True, False):
+        p = scenes.pbf_params((1.0, 1.0, 1.0), pos.shape[0], count_overflow=count_overflow)
+        g, o = _pair(pbf, p, pos, np.zeros_like(pos))
+        tune = g.tuning()
+        g.set_tuning(tune["block_size"], budget, tune["heavy_blocks"])
+        g.find_neighbors()
+        o.pass_(O.FIND)
+        nb = o.neighbors()
+        lists, counts = g.neighbor_lists()
+        assert np.array_equal(counts, nb["counts"])
+        mask = np.arange(p.max_neighbors)[None, :] < counts[:, None]
+        assert np.array_equal(np.where(mask, lists, 0), np.where(mask, nb["lists"], 0))
+        if count_overflow:
+            assert g.stats()["neighbor_overflow"] == o.counters()["overflow"] > 0
+    g.step(1)
+    o.step(1)
+    _compare_state(g, o, "crowded cells step")
+
+
 def test_partial_bin_span(pbf, scenes):
     """find_neighbors with a bin span shorter than the particle count (halo-free
     reference usage bins [0, count), neighbors.h:241-248)."""

@@ -10,10 +10,14 @@
 //  * Only about half of a particle's listed neighbours lie within h (the list
 //    radius is the 1.2 h bin spacing, neighbors.h:198); the others contribute
 //    exact zeros.  The density pass and the first XSPH pass walk the whole
-//    list and append the entries within h to a compact row (cnb).  The passes
-//    that see the same positions afterwards -- the Jacobi iteration's dp
-//    pass, and vorticity, vorticity-apply and the second XSPH pass -- walk
-//    only the compact row, in the same order.
+//    list and append the entries within h to a compact row (cnb); the density
+//    pass also sums them as it goes, the XSPH pass (whose terms need two more
+//    gathers each) sums the compact row afterwards.  The passes that see the
+//    same positions afterwards -- the Jacobi iteration's dp pass, and
+//    vorticity, vorticity-apply and the second XSPH pass -- walk only the
+//    compact row, in the same order.
+//  * The density pass writes lambda into the w word of its x* output, so the
+//    dp pass gathers a neighbour's x* and lambda in one 16-byte load.
 //  * The near-coincidence fix of compute_densities (particles.h:303-305) moves
 //    x*_p in the middle of the walk; when it fires, the density pass finishes
 //    that particle with a plain walk and raises the iteration's flag, and the
@@ -59,7 +63,8 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return x 
 
 // ---------------------------------------------------------------------------
 // compute_densities (particles.h:292-313) + compute_pressure_lambdas (:318-342)
-// xs -> xt, rho, lam; compact row of the entries within h.  With FEAT, also
+// xs -> xt (x* with lambda in w), rho, lam; compact row of the entries within
+// h.  With FEAT, also
 // the learned warm start's features of the particle (FeatCtl).
 // Returns the particle's density error max(rho/rho0 - 1, 0).
 template <bool FEAT>
@@ -74,7 +79,14 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
                                                     unsigned long long* __restrict__ counters, const FeatCtl& F) {
   f3 xp = ld3(xs, p);
   const int cnt = nbr_count[p];
-  // phase 1: compact row of the entries within h; stop at a near-coincident one
+  float density = K.mass_W0;
+  float sum_C = 0.0f;
+  f3 sg = {0.f, 0.f, 0.f};
+  uint32_t hacks = 0;
+  float dy_sum = 0.0f, div = 0.0f;
+  const f3 vp = FEAT ? ld3(v, p) : f3{0.f, 0.f, 0.f};
+  // one walk over the list: the entries within h go to the compact row and
+  // into the sums, in list order; the walk stops at a near-coincident entry
   int nc = 0;
   int hack_at = -1;
   for (int k0 = 0; k0 < cnt && hack_at < 0; k0 += kBatch) {
@@ -83,51 +95,35 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
-    // branch-free: every entry is stored, an accepted one in row nc and any
-    // other in the spare row cap (never read)
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      const float d2 = magsq3(sub3(xp, xyz(t[u])));
+      const f3 d = sub3(xp, xyz(t[u]));
+      const float d2 = magsq3(d);
       const bool w = (k0 + u < cnt) & (hack_at < 0) & within_h(d2, K);
       const bool hk = w & (d2 < K.T_hack);  // particles.h:304: handled by the plain walk below
       const bool acc = w & !hk;
+      // branch-free: every entry is stored, an accepted one in row nc and any
+      // other in the spare row cap (never read)
       s_list[(acc ? nc : K.cap) * kStageBlock + lane] = (int32_t)q[u];
       nc += acc ? 1 : 0;
       hack_at = hk ? k0 + u : hack_at;
+      if (acc) {
+        const float r = sqrtf(d2);
+        density += K.mass * poly6_in(r, K);
+        const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
+        dec3(sg, grad);
+        sum_C += magsq3(grad);
+        if (FEAT) {
+          dy_sum -= d.y;
+          const f3 dv = sub3(vp, ld3(v, q[u]));
+          div -= (dv.x * grad.x + dv.y * grad.y) + dv.z * grad.z;
+        }
+      }
     }
   }
   cnb_count[p] = nc;
   flush_compact(s_list, lane, nc, cnb, n, p);
-  // phase 2: kernel arithmetic over the compact row, in list order
-  float density = K.mass_W0;
-  float sum_C = 0.0f;
-  f3 sg = {0.f, 0.f, 0.f};
-  uint32_t hacks = 0;
   int within = nc;
-  float dy_sum = 0.0f, div = 0.0f;
-  const f3 vp = FEAT ? ld3(v, p) : f3{0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < nc; k0 += kBatch) {
-    uint32_t q[kBatch];
-    load_batch_lds(s_list, lane, k0, nc, p, q);
-    float4 t[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      if (k0 + u >= nc) continue;
-      const f3 d = sub3(xp, xyz(t[u]));
-      const float r = sqrtf(magsq3(d));
-      density += K.mass * poly6_in(r, K);
-      const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
-      dec3(sg, grad);
-      sum_C += magsq3(grad);
-      if (FEAT) {
-        dy_sum -= d.y;
-        const f3 dv = sub3(vp, ld3(v, q[u]));
-        div -= (dv.x * grad.x + dv.y * grad.y) + dv.z * grad.z;
-      }
-    }
-  }
   if (hack_at >= 0) {
     // particles.h:304-305: move x*_p, then finish this particle with a plain walk
     for (int k = hack_at; k < cnt; ++k) {
@@ -157,12 +153,12 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
     }
   }
   rho[p] = density;
-  st3(xt, p, xp);
   sum_C += magsq3(sg);
   const float constraint = density / K.rho0 - 1.0f;
   const float C_p = ((double)constraint < 0.0) ? 0.0f : constraint;
   const float lp = -C_p / (sum_C + K.eps);
   lam[p] = lp;
+  xt[p] = make_float4(xp.x, xp.y, xp.z, lp);  // the dp pass gathers x* and lambda in one load
   if (hacks) {
     atomicAdd(&counters[C_HACKS], (unsigned long long)hacks);
     moved[atomicAdd(&counters[C_MOVED_N + (it & 1)], 1ull)] = p;
@@ -281,9 +277,16 @@ __global__ void k_mark_dirty(int it, uint32_t stamp, SolveCtl S, KParams K, cons
 }
 
 // compute_pressure_lambdas (particles.h:318-342) from xt for the dirty
-// particles of this iteration (plain walks).
+// particles of this iteration (plain walks).  The new lambda goes to lam and
+// to the particle's xt.w; positions are read as three words, so no lane reads
+// the w word that another lane writes.
+__device__ __forceinline__ f3 ld3_words(const float4* a, uint32_t i) {
+  const float* f = reinterpret_cast<const float*>(a + i);
+  return {f[0], f[1], f[2]};
+}
+
 __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, uint32_t stamp, SolveCtl S,
-                                                          KParams K, const float4* __restrict__ xt,
+                                                          KParams K, float4* __restrict__ xt,
                                                           const float* __restrict__ rho, float* __restrict__ lam,
                                                           const int32_t* __restrict__ nbr,
                                                           const int32_t* __restrict__ nbr_count,
@@ -294,19 +297,19 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
   const bool all = counters[C_MOVED_ALL + (it & 1)] != 0ull;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n || (!all && dirty[p] != stamp)) return;
-  const f3 xp = ld3(xt, p);
+  const f3 xp = ld3_words(xt, p);
   float sum_C = 0.0f;
   f3 sg = {0.f, 0.f, 0.f};
   const int cnt = nbr_count[p];
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
     load_batch(nbr, n, k0, cnt, p, q);
-    float4 t[kBatch];
+    f3 t[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xt[q[u]];
+    for (int u = 0; u < kBatch; ++u) t[u] = ld3_words(xt, q[u]);
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      const f3 d = sub3(xp, xyz(t[u]));
+      const f3 d = sub3(xp, t[u]);
       const float d2 = magsq3(d);
       if (k0 + u >= cnt || !within_h(d2, K)) continue;
       const float r = sqrtf(d2);
@@ -318,7 +321,9 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
   sum_C += magsq3(sg);
   const float constraint = rho[p] / K.rho0 - 1.0f;
   const float C_p = ((double)constraint < 0.0) ? 0.0f : constraint;
-  lam[p] = -C_p / (sum_C + K.eps);
+  const float lp = -C_p / (sum_C + K.eps);
+  lam[p] = lp;
+  reinterpret_cast<float*>(xt + p)[3] = lp;
 }
 
 __device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) {
@@ -333,7 +338,6 @@ __device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) 
 // x* = xt.  Neither applies an update.
 __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, SolveCtl S,
                                                         KParams K, const float4* __restrict__ xt,
-                                                        const float* __restrict__ lam,
                                                         const float4* __restrict__ x, float4* __restrict__ xs,
                                                         float4* __restrict__ v, const int32_t* __restrict__ nbr,
                                                         const int32_t* __restrict__ nbr_count,
@@ -371,8 +375,10 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
   // a dirty particle (moved by the density fix, or next to one) walks its whole list with the test
   const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull &&
                      (counters[C_MOVED_ALL + (it & 1)] != 0ull || dirty[p] == stamp);
-  const f3 xp = ld3(xt, p);
-  const float lp = lam[p];
+  // x* and lambda of a particle come in one load (xt.w holds lambda)
+  const float4 xp4 = xt[p];
+  const f3 xp = xyz(xp4);
+  const float lp = xp4.w;
   f3 dp = {0.f, 0.f, 0.f};
   const int cnt = plain ? nbr_count[p] : cnb_count[p];
   const int32_t* list = plain ? nbr : cnb;
@@ -380,19 +386,15 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
     uint32_t q[kBatch];
     load_batch(list, n, k0, cnt, p, q);
     float4 t[kBatch];
-    float lq[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      t[u] = xt[q[u]];
-      lq[u] = lam[q[u]];
-    }
+    for (int u = 0; u < kBatch; ++u) t[u] = xt[q[u]];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const f3 d = sub3(xp, xyz(t[u]));
       const float d2 = magsq3(d);
       if (k0 + u >= cnt || (plain && !within_h(d2, K))) continue;
       const float r = sqrtf(d2);
-      acc3(dp, smul3(lp + lq[u], spiky_in(d, r, K)));
+      acc3(dp, smul3(lp + t[u].w, spiky_in(d, r, K)));
     }
   }
   f3 nw = add3(xp, smul3(K.inv_rho0, dp));
@@ -654,14 +656,14 @@ void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S) {
 }
 
 void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S) {
-  PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->dirty_stamp, S, c->K, (const float4*)c->xt,
+  PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->dirty_stamp, S, c->K, c->xt,
              (const float*)c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
              (const uint32_t*)c->dirty, (const unsigned long long*)c->counters);
 }
 
 void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S) {
   PBF_LAUNCH(c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
-             (const float*)c->lam, (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
+             (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
              (const int32_t*)c->nbr_count, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count,
              (const uint32_t*)c->dirty, c->counters);
 }

@@ -208,7 +208,7 @@ static void free_all(pbf_context* c) {
                   c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
                   c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty,
                   c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave, c->esum, c->feat, c->mw1,
-                  c->mw2, c->mp, c->scenes, c->sid, c->sida, c->cbox, c->chbox};
+                  c->mw2, c->mp, c->scenes, c->sid, c->sida, c->cbox, c->chbox, c->wmask};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -326,7 +326,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->sv[1], N + 1)) || (rc = dalloc(&c->rhist, radix_hist_size(N))) ||
       (rc = dalloc(&c->roffs, radix_hist_size(N) + 1)) ||
       (rc = dalloc(&c->rpart, scan_partials_size(radix_hist_size(N) + 1))) || (rc = dalloc(&c->heavy, N)) ||
-      (rc = dalloc(&c->chbox, 2 * ((N + 63) / 64))) ||
+      (rc = dalloc(&c->chbox, 2 * ((N + 63) / 64))) || (rc = dalloc(&c->wmask, 2 * N)) ||
       (rc = dalloc(&c->cnb, N * (uint64_t)c->K.stride)) || (rc = dalloc(&c->cnb_count, N)) ||
       (rc = dalloc(&c->moved, N)) || (rc = dalloc(&c->dirty, N)) ||
       (rc = dalloc(&c->ehist, (uint64_t)kMaxSolveSlots + 1)) || (rc = dalloc(&c->lsum, N)) ||

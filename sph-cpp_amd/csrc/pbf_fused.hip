@@ -9,13 +9,14 @@
 //    issued, then the arithmetic runs in list order.
 //  * Only about half of a particle's listed neighbours lie within h (the list
 //    radius is the 1.2 h bin spacing, neighbors.h:198); the others contribute
-//    exact zeros.  The density pass and the first XSPH pass walk the whole
-//    list and append the entries within h to a compact row (cnb); the density
-//    pass also sums them as it goes, the XSPH pass (whose terms need two more
-//    gathers each) sums the compact row afterwards.  The passes that see the
-//    same positions afterwards -- the Jacobi iteration's dp pass, and
-//    vorticity, vorticity-apply and the second XSPH pass -- walk only the
-//    compact row, in the same order.
+//    exact zeros.  The density pass walks the whole list once, summing the
+//    entries within h as it goes, and records them in a bit mask per particle
+//    (one bit per list entry); the Jacobi iteration's dp pass, which sees the
+//    same positions, walks the list again but gathers and sums only the
+//    masked entries.  The first XSPH pass appends the entries within h of the
+//    final positions to a compact row (cnb), staged in LDS; vorticity,
+//    vorticity-apply and the second XSPH pass walk only that row, in the same
+//    order.
 //  * The density pass writes lambda into the w word of its x* output, so the
 //    dp pass gathers a neighbour's x* and lambda in one 16-byte load.
 //  * The near-coincidence fix of compute_densities (particles.h:303-305) moves
@@ -63,19 +64,18 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return x 
 
 // ---------------------------------------------------------------------------
 // compute_densities (particles.h:292-313) + compute_pressure_lambdas (:318-342)
-// xs -> xt (x* with lambda in w), rho, lam; compact row of the entries within
-// h.  With FEAT, also
-// the learned warm start's features of the particle (FeatCtl).
+// xs -> xt (x* with lambda in w), rho, lam, and the particle's within-h mask
+// (bit k of wmask[p], of wmask[n + p] for k >= 64: list entry k lies within h
+// and is in the sums).  With FEAT, also the learned warm start's features of
+// the particle (FeatCtl).
 // Returns the particle's density error max(rho/rho0 - 1, 0).
 template <bool FEAT>
 __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int it, const KParams& K,
-                                                    int32_t* __restrict__ s_list, uint32_t lane,
                                                     const float4* __restrict__ xs, const float4* __restrict__ v,
                                                     float4* __restrict__ xt, float* __restrict__ rho,
                                                     float* __restrict__ lam, const int32_t* __restrict__ nbr,
                                                     const int32_t* __restrict__ nbr_count,
-                                                    int32_t* __restrict__ cnb, int32_t* __restrict__ cnb_count,
-                                                    uint32_t* __restrict__ moved,
+                                                    uint64_t* __restrict__ wmask, uint32_t* __restrict__ moved,
                                                     unsigned long long* __restrict__ counters, const FeatCtl& F) {
   f3 xp = ld3(xs, p);
   const int cnt = nbr_count[p];
@@ -85,9 +85,9 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   uint32_t hacks = 0;
   float dy_sum = 0.0f, div = 0.0f;
   const f3 vp = FEAT ? ld3(v, p) : f3{0.f, 0.f, 0.f};
-  // one walk over the list: the entries within h go to the compact row and
-  // into the sums, in list order; the walk stops at a near-coincident entry
-  int nc = 0;
+  // one walk over the list: the entries within h go into the sums, in list
+  // order, and into the mask; the walk stops at a near-coincident entry
+  uint64_t m0 = 0, m1 = 0;
   int hack_at = -1;
   for (int k0 = 0; k0 < cnt && hack_at < 0; k0 += kBatch) {
     uint32_t q[kBatch];
@@ -95,6 +95,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
+    uint32_t bits = 0;
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const f3 d = sub3(xp, xyz(t[u]));
@@ -102,10 +103,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
       const bool w = (k0 + u < cnt) & (hack_at < 0) & within_h(d2, K);
       const bool hk = w & (d2 < K.T_hack);  // particles.h:304: handled by the plain walk below
       const bool acc = w & !hk;
-      // branch-free: every entry is stored, an accepted one in row nc and any
-      // other in the spare row cap (never read)
-      s_list[(acc ? nc : K.cap) * kStageBlock + lane] = (int32_t)q[u];
-      nc += acc ? 1 : 0;
+      bits |= (uint32_t)acc << u;
       hack_at = hk ? k0 + u : hack_at;
       if (acc) {
         const float r = sqrtf(d2);
@@ -120,10 +118,15 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
         }
       }
     }
+    // k0 is a multiple of kBatch = 8: a batch never straddles the two words
+    if (k0 < 64)
+      m0 |= (uint64_t)bits << k0;
+    else
+      m1 |= (uint64_t)bits << (k0 - 64);
   }
-  cnb_count[p] = nc;
-  flush_compact(s_list, lane, nc, cnb, n, p);
-  int within = nc;
+  wmask[p] = m0;
+  if (K.cap > 64) wmask[n + p] = m1;
+  int within = __popcll(m0) + __popcll(m1);
   if (hack_at >= 0) {
     // particles.h:304-305: move x*_p, then finish this particle with a plain walk
     for (int k = hack_at; k < cnt; ++k) {
@@ -197,30 +200,24 @@ __device__ __forceinline__ unsigned long long fixed_error(float e) {
 // orders like the float) and its fixed-point sum into S.esum[eslot].  With
 // S.tol >= 0 nothing runs once the solve is done.
 template <bool FEAT>
-__global__ void __launch_bounds__(kStageBlock) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S,
-                                                                  FeatCtl F, KParams K,
-                                                                  const float4* __restrict__ xs,
-                                                                  const float4* __restrict__ v,
-                                                                  float4* __restrict__ xt, float* __restrict__ rho,
-                                                                  float* __restrict__ lam,
-                                                                  const int32_t* __restrict__ nbr,
-                                                                  const int32_t* __restrict__ nbr_count,
-                                                                  int32_t* __restrict__ cnb,
-                                                                  int32_t* __restrict__ cnb_count,
-                                                                  uint32_t* __restrict__ moved,
-                                                                  unsigned long long* __restrict__ counters) {
+__global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S, FeatCtl F,
+                                                             KParams K, const float4* __restrict__ xs,
+                                                             const float4* __restrict__ v, float4* __restrict__ xt,
+                                                             float* __restrict__ rho, float* __restrict__ lam,
+                                                             const int32_t* __restrict__ nbr,
+                                                             const int32_t* __restrict__ nbr_count,
+                                                             uint64_t* __restrict__ wmask,
+                                                             uint32_t* __restrict__ moved,
+                                                             unsigned long long* __restrict__ counters) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
-  extern __shared__ int32_t s_list[];  // [cap + 1][kStageBlock]
-  const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   float e = 0.0f;
   if (p < n)
-    e = density_lambda_one<FEAT>(p, n, it, K, s_list, lane, xs, v, xt, rho, lam, nbr, nbr_count, cnb, cnb_count,
-                                 moved, counters, F);
+    e = density_lambda_one<FEAT>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, moved, counters, F);
   if (S.ehist) {
     const unsigned long long es = wave_sum(fixed_error(e));
     e = wave_max(e);
-    if (lane == 0) {
+    if ((threadIdx.x & 63) == 0) {
       if (e > 0.0f) atomicMax(&S.ehist[eslot], __float_as_uint(e));
       if (es) atomicAdd(&S.esum[eslot], es);
     }
@@ -341,8 +338,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
                                                         const float4* __restrict__ x, float4* __restrict__ xs,
                                                         float4* __restrict__ v, const int32_t* __restrict__ nbr,
                                                         const int32_t* __restrict__ nbr_count,
-                                                        const int32_t* __restrict__ cnb,
-                                                        const int32_t* __restrict__ cnb_count,
+                                                        const uint64_t* __restrict__ wmask,
                                                         const uint32_t* __restrict__ dirty,
                                                         unsigned long long* __restrict__ counters) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
@@ -380,19 +376,23 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
   const f3 xp = xyz(xp4);
   const float lp = xp4.w;
   f3 dp = {0.f, 0.f, 0.f};
-  const int cnt = plain ? nbr_count[p] : cnb_count[p];
-  const int32_t* list = plain ? nbr : cnb;
+  // the entries the density pass summed (its within-h mask), or for a dirty
+  // particle every entry with the within-h test
+  const int cnt = nbr_count[p];
+  const uint64_t m0 = plain ? ~0ull : wmask[p];
+  const uint64_t m1 = (plain || K.cap <= 64) ? ~0ull : wmask[n + p];
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(list, n, k0, cnt, p, q);
+    load_batch(nbr, n, k0, cnt, p, q);
+    const uint32_t bits = (uint32_t)((k0 < 64 ? m0 >> k0 : m1 >> (k0 - 64)) & 0xFFu);
     float4 t[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xt[q[u]];
+    for (int u = 0; u < kBatch; ++u) t[u] = xt[(bits >> u) & 1u ? q[u] : p];  // masked entries read p itself
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const f3 d = sub3(xp, xyz(t[u]));
       const float d2 = magsq3(d);
-      if (k0 + u >= cnt || (plain && !within_h(d2, K))) continue;
+      if (k0 + u >= cnt || !((bits >> u) & 1u) || (plain && !within_h(d2, K))) continue;
       const float r = sqrtf(d2);
       acc3(dp, smul3(lp + t[u].w, spiky_in(d, r, K)));
     }
@@ -636,13 +636,13 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
 
 void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S, const FeatCtl& F) {
   if (F.feat)
-    PBF_LAUNCH_STAGED(c, "density_features", k_density_lambda_c<true>, it, eslot, S, F, c->K,
-                      (const float4*)c->xs, (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr,
-                      (const int32_t*)c->nbr_count, c->cnb, c->cnb_count, c->moved, c->counters);
+    PBF_LAUNCH(c, "density_features", k_density_lambda_c<true>, it, eslot, S, F, c->K, (const float4*)c->xs,
+               (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
+               c->wmask, c->moved, c->counters);
   else
-    PBF_LAUNCH_STAGED(c, "density_lambda", k_density_lambda_c<false>, it, eslot, S, F, c->K,
-                      (const float4*)c->xs, (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr,
-                      (const int32_t*)c->nbr_count, c->cnb, c->cnb_count, c->moved, c->counters);
+    PBF_LAUNCH(c, "density_lambda", k_density_lambda_c<false>, it, eslot, S, F, c->K, (const float4*)c->xs,
+               (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
+               c->wmask, c->moved, c->counters);
 }
 
 void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S) {
@@ -664,8 +664,7 @@ void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S) {
 void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S) {
   PBF_LAUNCH(c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
              (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
-             (const int32_t*)c->nbr_count, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count,
-             (const uint32_t*)c->dirty, c->counters);
+             (const int32_t*)c->nbr_count, (const uint64_t*)c->wmask, (const uint32_t*)c->dirty, c->counters);
 }
 
 void launch_warm_dp(pbf_context* c) {

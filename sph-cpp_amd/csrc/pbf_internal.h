@@ -155,6 +155,7 @@ struct pbf_context {
   uint32_t* scan_partials = nullptr;
   int32_t *nbr = nullptr, *nbr_count = nullptr;
   int32_t *cnb = nullptr, *cnb_count = nullptr;
+  uint64_t* wmask = nullptr;  // [2][n] within-h masks of the last fused density pass
   unsigned long long* counters = nullptr;
   uint32_t* ehist = nullptr;
   unsigned long long* esum = nullptr;

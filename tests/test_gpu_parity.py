@@ -207,6 +207,20 @@ def test_fused_step_equals_per_pass(pbf, scenes):
         assert_bitwise(getattr(a, f)(), getattr(b, f)(), "fused vs per-pass " + f)
 
 
+def test_cap_128_dense_lattice_steps_bit_exact(pbf, scenes):
+    """A lattice at 0.4 h spacing under the largest cap (128): lists run past
+    64 entries, so the density pass's within-h mask needs its second word;
+    three fused steps must equal the oracle bit for bit."""
+    pos = scenes.jitter(scenes.lattice((12, 10, 10), spacing=0.04), seed=17)
+    p = scenes.pbf_params((1.2, 0.8, 0.8), pos.shape[0], cap=128)
+    g, o = _pair(pbf, p, pos, np.zeros_like(pos))
+    g.step(3)
+    o.step(3)
+    counts = g.neighbor_lists()[1]
+    assert counts.max() > 64
+    _compare_state(g, o, "cap 128, 3 steps")
+
+
 def test_c1_one_and_hundred_substeps_bit_exact(pbf, scenes):
     """32,768-particle dam break: 1 and 100 substeps, GPU == oracle (Jacobi) bit for bit."""
     p, pos, vel = scenes.config_c1(seed=0, count_overflow=True)

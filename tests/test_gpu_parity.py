@@ -221,6 +221,26 @@ def test_cap_128_dense_lattice_steps_bit_exact(pbf, scenes):
     _compare_state(g, o, "cap 128, 3 steps")
 
 
+def test_wide_slab_lists_bit_exact(pbf, scenes):
+    """A thin, wide slab (160 x 160 x 5 particles): a cell layer holds more than
+    32,767 particles, so neighbours one layer apart sit that far apart in slot
+    order (no narrower list encoding may be assumed); one fused step must
+    equal the oracle bit for bit."""
+    pos = scenes.jitter(scenes.lattice((160, 160, 5)), seed=19)
+    p = scenes.pbf_params((8.4, 8.4, 1.6), pos.shape[0])
+    g, o = _pair(pbf, p, pos, np.zeros_like(pos))
+    g.find_neighbors()
+    lists, counts = g.neighbor_lists()  # rows and entries are original indices
+    slot = np.empty(pos.shape[0], dtype=np.int64)
+    slot[g.particle_ids().astype(np.int64)] = np.arange(pos.shape[0])
+    mask = np.arange(p.max_neighbors)[None, :] < counts[:, None]
+    far = np.abs(slot[lists.astype(np.int64)] - slot[:, None])
+    assert far[mask].max() > 32767
+    g.step(1)
+    o.step(1)
+    _compare_state(g, o, "wide slab step")
+
+
 def test_c1_one_and_hundred_substeps_bit_exact(pbf, scenes):
     """32,768-particle dam break: 1 and 100 substeps, GPU == oracle (Jacobi) bit for bit."""
     p, pos, vel = scenes.config_c1(seed=0, count_overflow=True)

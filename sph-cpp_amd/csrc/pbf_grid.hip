@@ -441,15 +441,23 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
   const f3 ps = ld3(xs, p);
   const WalkCells w = walk_cells(ps, K.s);
   const GridView G = grid_of(K, p);
+  // the 27 cell ranges first, all loads in flight together (a cell outside
+  // the grid is an empty range); the walk then waits only on candidates
+  uint32_t cb[27], ce[27];
+#pragma unroll
+  for (int c = 0; c < 27; ++c) {
+    const uint32_t cell = cell_key(pick3(w.cx, c / 9), pick3(w.cy, (c / 3) % 3), pick3(w.cz, c % 3), G, K);
+    const bool in = cell < K.ncells;
+    const uint32_t cc = in ? cell : 0u;
+    const uint32_t b = start[cc], e = start[cc + 1];
+    cb[c] = in ? b : 0u;
+    ce[c] = in ? e : 0u;
+  }
   if (K.count_overflow) {
     // every candidate must be seen: route by the total number of candidates
     uint32_t total = 0;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) {
-          const uint32_t cell = cell_key(pick3(w.cx, i), pick3(w.cy, j), pick3(w.cz, k), G, K);
-          if (cell < K.ncells) total += start[cell + 1] - start[cell];
-        }
+#pragma unroll
+    for (int c = 0; c < 27; ++c) total += ce[c] - cb[c];
     if (total > K.heavy_budget) {
       heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
       return;
@@ -462,39 +470,36 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
   int count = 0;
   uint32_t ovf = 0;
   const int cap = K.cap;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      for (int k = 0; k < 3; ++k) {
-        const uint32_t cell = cell_key(pick3(w.cx, i), pick3(w.cy, j), pick3(w.cz, k), G, K);
-        if (cell >= K.ncells) continue;
-        const uint32_t b = start[cell], e = start[cell + 1];
-        // candidates of a cell are contiguous: load 8 at a time, test in order
-        // (position buffers carry 8 rows of padding, so the loads need no clamp)
-        for (uint32_t q0 = b; q0 < e; q0 += 8) {
-          scanned += 8;
-          if (scanned > K.heavy_budget && !K.count_overflow) {
-            heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
-            return;
-          }
-          float4 t[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) t[u] = xs[q0 + (uint32_t)u];
-          // branch-free acceptance: every candidate is stored, an accepted one
-          // in row min(count, cap) and a rejected one in the spare row cap;
-          // rows >= cap are never read
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const uint32_t q = q0 + (uint32_t)u;
-            const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
-            const bool acc = (q < e) & (q != p) & (d2 < K.ss);
-            const int row = acc ? min(count, cap) : cap;
-            s_list[row * kFillBlock + lane] = (int32_t)q;
-            count += acc ? 1 : 0;
-          }
-          // a full list cannot change any more; only the overflow count could
-          if (count >= cap && !K.count_overflow) goto done;
-        }
+  for (int c = 0; c < 27; ++c) {
+    const uint32_t b = cb[c], e = ce[c];
+    // candidates of a cell are contiguous: load 8 at a time, test in order
+    // (position buffers carry 8 rows of padding, so the loads need no clamp)
+    for (uint32_t q0 = b; q0 < e; q0 += 8) {
+      scanned += 8;
+      if (scanned > K.heavy_budget && !K.count_overflow) {
+        heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
+        return;
       }
+      float4 t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = xs[q0 + (uint32_t)u];
+      // branch-free acceptance: every candidate is stored, an accepted one
+      // in row min(count, cap) and a rejected one in the spare row cap;
+      // rows >= cap are never read
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t q = q0 + (uint32_t)u;
+        const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
+        const bool acc = (q < e) & (q != p) & (d2 < K.ss);
+        const int row = acc ? min(count, cap) : cap;
+        s_list[row * kFillBlock + lane] = (int32_t)q;
+        count += acc ? 1 : 0;
+      }
+      // a full list cannot change any more; only the overflow count could
+      if (count >= cap && !K.count_overflow) goto done;
+    }
+  }
 done:
   if (count > cap) {
     ovf = (uint32_t)(count - cap);

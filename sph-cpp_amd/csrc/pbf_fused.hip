@@ -75,8 +75,8 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
                                                     float4* __restrict__ xt, float* __restrict__ rho,
                                                     float* __restrict__ lam, const int32_t* __restrict__ nbr,
                                                     const int32_t* __restrict__ nbr_count,
-                                                    uint64_t* __restrict__ wmask, uint32_t* __restrict__ moved,
-                                                    unsigned long long* __restrict__ counters, const FeatCtl& F) {
+                                                    uint64_t* __restrict__ wmask, const FeatCtl& F,
+                                                    uint32_t& hacks_out) {
   f3 xp = ld3(xs, p);
   const int cnt = nbr_count[p];
   float density = K.mass_W0;
@@ -162,11 +162,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   const float lp = -C_p / (sum_C + K.eps);
   lam[p] = lp;
   xt[p] = make_float4(xp.x, xp.y, xp.z, lp);  // the dp pass gathers x* and lambda in one load
-  if (hacks) {
-    atomicAdd(&counters[C_HACKS], (unsigned long long)hacks);
-    moved[atomicAdd(&counters[C_MOVED_N + (it & 1)], 1ull)] = p;
-    atomicExch(&counters[C_HACK_FLAG + (it & 1)], 1ull);
-  }
+  hacks_out = hacks;
   if (FEAT) {
     const float inv_s = 1.0f / kWarmLambdaScale;
     const float dyn = dy_sum / (float)(within > 0 ? within : 1);
@@ -212,8 +208,23 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   float e = 0.0f;
-  if (p < n)
-    e = density_lambda_one<FEAT>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, moved, counters, F);
+  uint32_t hacks = 0;
+  if (p < n) e = density_lambda_one<FEAT>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, F, hacks);
+  // particles moved by the density fix: one set of counter updates per
+  // wavefront (every lane of the wavefront is here)
+  const uint64_t hm = __ballot(hacks != 0);
+  if (hm) {
+    const uint32_t wl = threadIdx.x & 63;
+    const unsigned long long hsum = wave_sum((unsigned long long)hacks);
+    unsigned long long base = 0;
+    if (wl == 0) {
+      base = atomicAdd(&counters[C_MOVED_N + (it & 1)], (unsigned long long)__popcll(hm));
+      atomicAdd(&counters[C_HACKS], hsum);
+      atomicExch(&counters[C_HACK_FLAG + (it & 1)], 1ull);
+    }
+    base = __shfl(base, 0, 64);
+    if (hacks) moved[base + (unsigned long long)__popcll(hm & ((1ull << wl) - 1ull))] = p;
+  }
   if (S.ehist) {
     const unsigned long long es = wave_sum(fixed_error(e));
     e = wave_max(e);

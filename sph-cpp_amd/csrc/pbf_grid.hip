@@ -349,8 +349,60 @@ __global__ void __launch_bounds__(kBlock) k_cell_box(uint32_t n, uint32_t ncells
       }
     }
   }
+  // a wavefront inside one cell (all 64 keys equal) leaves the cell box to
+  // k_cell_box_chunks, which merges such chunks before touching the cell
+  const bool inside = __shfl(k, 0, 64) == __shfl(k, 63, 64) && __shfl(k, 63, 64) < ncells;
   const uint32_t prev = __shfl_up(k, 1, 64);
-  if (valid && k < ncells && (lane == 0 || prev != k)) {
+  if (!inside && valid && k < ncells && (lane == 0 || prev != k)) {
+    uint32_t* b = box + 6ull * k;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&b[a], mn[a]);
+      atomicMax(&b[3 + a], mx[a]);
+    }
+  }
+}
+
+// The chunks that lie inside one cell (a crowded cell holds thousands): one
+// thread per chunk, runs of equal cells reduced per wavefront, one update of
+// the cell box per run -- instead of one per chunk on the same six words.
+__global__ void __launch_bounds__(kBlock) k_cell_box_chunks(uint32_t n, uint32_t ncells,
+                                                            const uint32_t* __restrict__ key,
+                                                            const float4* __restrict__ chbox,
+                                                            uint32_t* __restrict__ box) {
+  const uint32_t ch = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const uint32_t nch = (n + 63) / 64;
+  uint32_t k = 0xFFFFFFFFu;
+  uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+  if (ch < nch && 64ull * ch + 63 < n) {
+    const uint32_t k0 = key[64 * ch], k1 = key[64 * ch + 63];
+    if (k0 == k1 && k1 < ncells) {
+      k = k0;
+      const float4 lo = chbox[2 * ch], hi = chbox[2 * ch + 1];
+      mn[0] = ord_enc(lo.x), mn[1] = ord_enc(lo.y), mn[2] = ord_enc(lo.z);
+      mx[0] = ord_enc(hi.x), mx[1] = ord_enc(hi.y), mx[2] = ord_enc(hi.z);
+    }
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t kk = __shfl_down(k, d, 64);
+    uint32_t omn[3], omx[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      omn[a] = __shfl_down(mn[a], d, 64);
+      omx[a] = __shfl_down(mx[a], d, 64);
+    }
+    if (lane + d < 64 && kk == k) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = min(mn[a], omn[a]);
+        mx[a] = max(mx[a], omx[a]);
+      }
+    }
+  }
+  const uint32_t prev = __shfl_up(k, 1, 64);
+  if (k < ncells && (lane == 0 || prev != k)) {
     uint32_t* b = box + 6ull * k;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -368,6 +420,8 @@ void launch_cell_box(pbf_context* c) {
   hipLaunchKernelGGL(k_box_init, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, c->cbox, count);
   hipLaunchKernelGGL(k_cell_box, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, c->K.ncells,
                      (const uint32_t*)c->key, (const float4*)c->xs, c->cbox, c->chbox);
+  hipLaunchKernelGGL(k_cell_box_chunks, dim3(nblocks((n + 63) / 64)), dim3(kBlock), 0, c->stream, n, c->K.ncells,
+                     (const uint32_t*)c->key, (const float4*)c->chbox, c->cbox);
   timer_end(c, t);
 }
 
@@ -420,24 +474,13 @@ __device__ __forceinline__ WalkCells walk_cells(f3 ps, float s) {
 // consecutive slots instead of 64 scattered 4-byte writes.
 constexpr int kFillBlock = 64;
 
-__global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint32_t fb, uint32_t fe,
-                                                               const float4* __restrict__ xs,
-                                                               const uint32_t* __restrict__ oid,
-                                                               const uint32_t* __restrict__ start,
-                                                               const uint32_t* __restrict__ box, KParams K,
-                                                               int32_t* __restrict__ nbr,
-                                                               int32_t* __restrict__ nbr_count,
-                                                               uint32_t* __restrict__ heavy,
-                                                               unsigned long long* __restrict__ counters) {
-  extern __shared__ int32_t s_list[];  // [cap + 1][kFillBlock]
-  const uint32_t lane = threadIdx.x;
-  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t o = oid[p];
-  if (o < fb || o >= fe) {
-    nbr_count[p] = 0;
-    return;
-  }
+// The lane walk of particle p: writes its list and count and returns true,
+// or returns false when the particle goes to the wavefront builder.  ovf
+// receives the candidates dropped by the cap.
+__device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane, int32_t* __restrict__ s_list,
+                                          const float4* __restrict__ xs, const uint32_t* __restrict__ start,
+                                          const KParams& K, int32_t* __restrict__ nbr,
+                                          int32_t* __restrict__ nbr_count, uint32_t& ovf) {
   const f3 ps = ld3(xs, p);
   const WalkCells w = walk_cells(ps, K.s);
   const GridView G = grid_of(K, p);
@@ -458,17 +501,13 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
     uint32_t total = 0;
 #pragma unroll
     for (int c = 0; c < 27; ++c) total += ce[c] - cb[c];
-    if (total > K.heavy_budget) {
-      heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
-      return;
-    }
+    if (total > K.heavy_budget) return false;
   }
   // otherwise the walk ends when the list is full; a lane that has scanned
   // K.heavy_budget without filling its list hands the particle over (the
   // wavefront builder starts again and writes the same entries)
   uint32_t scanned = 0;
   int count = 0;
-  uint32_t ovf = 0;
   const int cap = K.cap;
 #pragma unroll
   for (int c = 0; c < 27; ++c) {
@@ -477,10 +516,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
     // (position buffers carry 8 rows of padding, so the loads need no clamp)
     for (uint32_t q0 = b; q0 < e; q0 += 8) {
       scanned += 8;
-      if (scanned > K.heavy_budget && !K.count_overflow) {
-        heavy[atomicAdd(&counters[C_HEAVY], 1ull)] = p;
-        return;
-      }
+      if (scanned > K.heavy_budget && !K.count_overflow) return false;
       float4 t[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) t[u] = xs[q0 + (uint32_t)u];
@@ -507,7 +543,47 @@ done:
   }
   nbr_count[p] = count;
   for (int k = 0; k < count; ++k) nbr[(uint64_t)k * n + p] = s_list[k * kFillBlock + lane];
-  if (ovf && K.count_overflow) atomicAdd(&counters[C_OVERFLOW], (unsigned long long)ovf);
+  return true;
+}
+
+// Hand-offs and overflow counts go to the shared counters once per
+// wavefront (a same-address atomic per lane serialises in L2 when many
+// particles of a collapsed region hand off together).
+__global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint32_t fb, uint32_t fe,
+                                                               const float4* __restrict__ xs,
+                                                               const uint32_t* __restrict__ oid,
+                                                               const uint32_t* __restrict__ start,
+                                                               const uint32_t* __restrict__ box, KParams K,
+                                                               int32_t* __restrict__ nbr,
+                                                               int32_t* __restrict__ nbr_count,
+                                                               uint32_t* __restrict__ heavy,
+                                                               unsigned long long* __restrict__ counters) {
+  extern __shared__ int32_t s_list[];  // [cap + 1][kFillBlock]
+  const uint32_t lane = threadIdx.x;
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  bool to_heavy = false;
+  uint32_t ovf = 0;
+  if (p < n) {
+    const uint32_t o = oid[p];
+    if (o < fb || o >= fe)
+      nbr_count[p] = 0;
+    else
+      to_heavy = !walk_list(p, n, lane, s_list, xs, start, K, nbr, nbr_count, ovf);
+  }
+  // every lane of the wavefront (one per block) is here
+  const uint64_t hm = __ballot(to_heavy);
+  if (hm) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&counters[C_HEAVY], (unsigned long long)__popcll(hm));
+    base = __shfl(base, 0, 64);
+    if (to_heavy) heavy[base + (unsigned long long)__popcll(hm & ((1ull << lane) - 1ull))] = p;
+  }
+  if (K.count_overflow) {
+    unsigned long long o = ovf;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) o += __shfl_xor(o, d, 64);
+    if (lane == 0 && o) atomicAdd(&counters[C_OVERFLOW], o);
+  }
 }
 
 // One wavefront per heavy particle; waves stride over the heavy list, whose

@@ -586,6 +586,23 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
   }
 }
 
+// A group of 64 candidates of the wavefront builder, one per lane (valid:
+// the lane holds a candidate other than p; t: its position): the accepted
+// ones are appended in slot order through a ballot.  Returns true when the
+// list is full and the walk may end.
+__device__ __forceinline__ bool take_group(uint32_t q, bool valid, float4 t, f3 ps, const KParams& K, uint32_t n,
+                                           uint32_t p, uint64_t below, uint32_t cap, int32_t* __restrict__ nbr,
+                                           uint32_t& count) {
+  const bool acc = valid && magsq3(sub3(ps, f3{t.x, t.y, t.z})) < K.ss;
+  const uint64_t m = __ballot(acc);
+  if (acc) {
+    const uint32_t pos = count + (uint32_t)__popcll(m & below);
+    if (pos < cap) nbr[(uint64_t)pos * n + p] = (int32_t)q;
+  }
+  count += (uint32_t)__popcll(m);
+  return count >= cap && !K.count_overflow;
+}
+
 // One wavefront per heavy particle; waves stride over the heavy list, whose
 // length is read once per wave (every wave exits after the last entry).
 __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4* __restrict__ xs,
@@ -614,18 +631,18 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
           if (cell >= K.ncells) continue;
           const uint32_t b = start[cell], e = start[cell + 1];
           if (e - b > kBoxMin && cell_far(box, cell, ps, K.ss)) continue;
+          // candidates come in groups of 64 (one per lane); two groups are
+          // loaded before the first is tested, so that two loads of a
+          // wavefront are in flight instead of one
           if (e - b <= kChunkMin) {
-            for (uint32_t base = b; base < e; base += 64) {
-              const uint32_t q = base + lane;
-              bool acc = false;
-              if (q < e && q != p) acc = magsq3(sub3(ps, ld3(xs, q))) < K.ss;
-              const uint64_t m = __ballot(acc);
-              if (acc) {
-                const uint32_t pos = count + (uint32_t)__popcll(m & below);
-                if (pos < cap) nbr[(uint64_t)pos * n + p] = (int32_t)q;
-              }
-              count += (uint32_t)__popcll(m);
-              if (count >= cap && !K.count_overflow) goto done;
+            for (uint32_t base = b; base < e; base += 128) {
+              const uint32_t qa = base + lane, qb = base + 64 + lane;
+              const bool va = qa < e && qa != p, vb = qb < e && qb != p;
+              float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = ta;
+              if (va) ta = xs[qa];
+              if (vb) tb = xs[qb];
+              if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count)) goto done;
+              if (base + 64 < e && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count)) goto done;
             }
             continue;
           }
@@ -641,17 +658,18 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
             }
             uint64_t todo = __ballot(near);
             while (todo) {
-              const uint32_t q = ((c0 + (uint32_t)__builtin_ctzll(todo)) << 6) + lane;
+              const uint32_t ca = (uint32_t)__builtin_ctzll(todo);
               todo &= todo - 1ull;
-              bool acc = false;
-              if (q >= b && q < e && q != p) acc = magsq3(sub3(ps, ld3(xs, q))) < K.ss;
-              const uint64_t m = __ballot(acc);
-              if (acc) {
-                const uint32_t pos = count + (uint32_t)__popcll(m & below);
-                if (pos < cap) nbr[(uint64_t)pos * n + p] = (int32_t)q;
-              }
-              count += (uint32_t)__popcll(m);
-              if (count >= cap && !K.count_overflow) goto done;
+              const bool two = todo != 0ull;
+              const uint32_t cb = two ? (uint32_t)__builtin_ctzll(todo) : ca;
+              if (two) todo &= todo - 1ull;
+              const uint32_t qa = ((c0 + ca) << 6) + lane, qb = ((c0 + cb) << 6) + lane;
+              const bool va = qa >= b && qa < e && qa != p, vb = two && qb >= b && qb < e && qb != p;
+              float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = ta;
+              if (va) ta = xs[qa];
+              if (vb) tb = xs[qb];
+              if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count)) goto done;
+              if (two && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count)) goto done;
             }
           }
         }

@@ -208,7 +208,7 @@ static void free_all(pbf_context* c) {
                   c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
                   c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty,
                   c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave, c->esum, c->feat, c->mw1,
-                  c->mw2, c->mp, c->scenes, c->sid, c->sida, c->cbox, c->chbox, c->wmask};
+                  c->mw2, c->mp, c->scenes, c->sid, c->sida, c->cbox, c->chbox, c->wmask, c->mflag};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -328,7 +328,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->rpart, scan_partials_size(radix_hist_size(N) + 1))) || (rc = dalloc(&c->heavy, N)) ||
       (rc = dalloc(&c->chbox, 2 * ((N + 63) / 64))) || (rc = dalloc(&c->wmask, 2 * N)) ||
       (rc = dalloc(&c->cnb, N * (uint64_t)c->K.stride)) || (rc = dalloc(&c->cnb_count, N)) ||
-      (rc = dalloc(&c->moved, N)) || (rc = dalloc(&c->dirty, N)) ||
+      (rc = dalloc(&c->moved, N)) || (rc = dalloc(&c->dirty, N)) || (rc = dalloc(&c->mflag, N)) ||
       (rc = dalloc(&c->ehist, (uint64_t)kMaxSolveSlots + 1)) || (rc = dalloc(&c->lsum, N)) ||
       (rc = dalloc(&c->lsuma, N)) || (rc = dalloc(&c->wlam, N)) || (rc = dalloc(&c->wgiven, N)) ||
       (rc = dalloc(&c->xsave, N)) || (rc = dalloc(&c->esum, (uint64_t)kMaxSolveSlots + 1)) ||
@@ -347,6 +347,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
   launch_fill_u32((uint32_t*)c->nbr_count, 0u, N, c->stream);
   launch_fill_u32(c->dirty, 0u, N, c->stream);
+  launch_fill_u32(c->mflag, 0u, N, c->stream);
   launch_fill_u32((uint32_t*)c->lsum, 0u, N, c->stream);
   launch_fill_u32(c->ehist, 0u, (uint64_t)kMaxSolveSlots + 1, c->stream);
   launch_fill_u64(c->esum, 0ull, (uint64_t)kMaxSolveSlots + 1, c->stream);
@@ -689,7 +690,10 @@ static void step_body(pbf_context* c) {
   {
     // the multiplier sums move with the reorder inside find_neighbors
     launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
-    if (c->capturing) launch_fill_u32(c->dirty, 0u, n, c->stream);  // stamps repeat on every replay
+    if (c->capturing) {  // stamps repeat on every replay
+      launch_fill_u32(c->dirty, 0u, n, c->stream);
+      launch_fill_u32(c->mflag, 0u, n, c->stream);
+    }
     launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
     find_neighbors_impl(c, n, 0, n);
     if (c->track_lsum) launch_fill_u32((uint32_t*)c->lsum, 0u, n, c->stream);

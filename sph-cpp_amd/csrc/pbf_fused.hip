@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
                                                              const int32_t* __restrict__ nbr,
                                                              const int32_t* __restrict__ nbr_count,
                                                              uint64_t* __restrict__ wmask,
-                                                             uint32_t* __restrict__ moved,
+                                                             uint32_t* __restrict__ moved, uint32_t stamp,
+                                                             uint32_t* __restrict__ mflag,
                                                              unsigned long long* __restrict__ counters) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
@@ -223,7 +224,10 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
       atomicExch(&counters[C_HACK_FLAG + (it & 1)], 1ull);
     }
     base = __shfl(base, 0, 64);
-    if (hacks) moved[base + (unsigned long long)__popcll(hm & ((1ull << wl) - 1ull))] = p;
+    if (hacks) {
+      moved[base + (unsigned long long)__popcll(hm & ((1ull << wl) - 1ull))] = p;
+      mflag[p] = stamp;
+    }
   }
   if (S.ehist) {
     const unsigned long long es = wave_sum(fixed_error(e));
@@ -236,58 +240,74 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
 }
 
 // A particle is dirty when it or a particle in its list was moved by the
-// density fix: its fused lambda and its compact list may be stale.  A list
+// density fix: its fused lambda and its within-h mask may be stale.  A list
 // walk visits cells computed from floating-point coordinates (x + i*s,
 // neighbors.h:177-179), which lie within two cells of the particle's own cell
 // on every axis, so every particle that can list q sits within two cells of
-// q's cell: all particles of those 5x5x5 cells are marked.  Many moved
-// particles mark everybody instead.
+// q's cell: all particles of those 5x5x5 cells are marked, one wavefront per
+// moved particle (lanes 0..24 find the 25 rows of cells, then the wavefront
+// stamps each row's slot range).  Many moved particles, or a crowded
+// neighbourhood, leave the work to lambda_repair's exact scan instead.
 constexpr uint32_t kMaxMovedMarked = 4096;
 constexpr uint32_t kMaxMarkedPerMoved = 8192;
+constexpr uint32_t kMarkBlocks = 64;
 
-__global__ void k_mark_dirty(int it, uint32_t stamp, SolveCtl S, KParams K, const uint32_t* __restrict__ key,
-                             const uint32_t* __restrict__ start, const uint32_t* __restrict__ moved,
-                             uint32_t* __restrict__ dirty, unsigned long long* __restrict__ counters) {
+__global__ void __launch_bounds__(kBlock) k_mark_dirty(int it, uint32_t stamp, SolveCtl S, KParams K,
+                                                       const uint32_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ start,
+                                                       const uint32_t* __restrict__ moved,
+                                                       uint32_t* __restrict__ dirty,
+                                                       unsigned long long* __restrict__ counters) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   const uint32_t nmoved = (uint32_t)counters[C_MOVED_N + (it & 1)];
   if (nmoved > kMaxMovedMarked) {
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_MOVED_ALL + (it & 1)] = 1ull;
     return;
   }
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nmoved; i += gridDim.x * blockDim.x) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t i = wave; i < nmoved; i += nwaves) {
     const uint32_t q = moved[i];
     const uint32_t kq = key[q];
     if (kq >= K.ncells) {  // outside the grid: no cell to start from
-      counters[C_MOVED_ALL + (it & 1)] = 1ull;
+      if (lane == 0) counters[C_MOVED_ALL + (it & 1)] = 1ull;
       continue;
     }
     // cell coordinates in the moved particle's own grid (its scene's)
     const GridView G = grid_of(K, q);
     const uint32_t lq = kq - G.off;
     const int cz = (int)(lq / G.dxy), cy = (int)((lq % G.dxy) / G.dimx), cx = (int)(lq % G.dimx);
-    uint32_t total = 0;
-    for (int z = max(cz - 2, 0); z <= min(cz + 2, (int)G.dimz - 1); ++z)
-      for (int y = max(cy - 2, 0); y <= min(cy + 2, (int)G.dimy - 1); ++y) {
+    uint32_t rb = 0, re = 0;
+    if (lane < 25) {
+      const int z = cz - 2 + (int)lane / 5, y = cy - 2 + (int)lane % 5;
+      if (z >= 0 && z < (int)G.dimz && y >= 0 && y < (int)G.dimy) {
         const uint64_t row = G.off + (uint64_t)z * G.dxy + (uint64_t)y * G.dimx;
-        total += start[row + min(cx + 2, (int)G.dimx - 1) + 1] - start[row + max(cx - 2, 0)];
+        rb = start[row + max(cx - 2, 0)];
+        re = start[row + min(cx + 2, (int)G.dimx - 1) + 1];
       }
-    if (total > kMaxMarkedPerMoved) {  // crowded neighbourhood: cheaper to treat everybody as dirty
-      counters[C_MOVED_ALL + (it & 1)] = 1ull;
+    }
+    uint32_t total = re - rb;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) total += __shfl_xor(total, d, 64);
+    if (total > kMaxMarkedPerMoved) {  // crowded neighbourhood: leave it to the exact scan
+      if (lane == 0) counters[C_MOVED_ALL + (it & 1)] = 1ull;
       continue;
     }
-    for (int z = max(cz - 2, 0); z <= min(cz + 2, (int)G.dimz - 1); ++z)
-      for (int y = max(cy - 2, 0); y <= min(cy + 2, (int)G.dimy - 1); ++y) {
-        const uint64_t row = G.off + (uint64_t)z * G.dxy + (uint64_t)y * G.dimx;
-        const uint32_t b = start[row + max(cx - 2, 0)], e = start[row + min(cx + 2, (int)G.dimx - 1) + 1];
-        for (uint32_t p = b; p < e; ++p) dirty[p] = stamp;
-      }
+    for (int r = 0; r < 25; ++r) {
+      const uint32_t b = __shfl(rb, r, 64), e = __shfl(re, r, 64);
+      for (uint32_t t = b + lane; t < e; t += 64) dirty[t] = stamp;
+    }
   }
 }
 
 // compute_pressure_lambdas (particles.h:318-342) from xt for the dirty
 // particles of this iteration (plain walks).  The new lambda goes to lam and
 // to the particle's xt.w; positions are read as three words, so no lane reads
-// the w word that another lane writes.
+// the w word that another lane writes.  When k_mark_dirty left the marking
+// (many moved particles or a crowded neighbourhood), a particle is dirty
+// exactly when it or an entry of its list carries this iteration's moved
+// stamp; this pass finds those and stamps them in dirty for the dp pass.
 __device__ __forceinline__ f3 ld3_words(const float4* a, uint32_t i) {
   const float* f = reinterpret_cast<const float*>(a + i);
   return {f[0], f[1], f[2]};
@@ -298,17 +318,31 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
                                                           const float* __restrict__ rho, float* __restrict__ lam,
                                                           const int32_t* __restrict__ nbr,
                                                           const int32_t* __restrict__ nbr_count,
-                                                          const uint32_t* __restrict__ dirty,
+                                                          const uint32_t* __restrict__ mflag,
+                                                          uint32_t* __restrict__ dirty,
                                                           const unsigned long long* __restrict__ counters) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   if (counters[C_HACK_FLAG + (it & 1)] == 0ull) return;
-  const bool all = counters[C_MOVED_ALL + (it & 1)] != 0ull;
+  const bool scan = counters[C_MOVED_ALL + (it & 1)] != 0ull;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= n || (!all && dirty[p] != stamp)) return;
+  if (p >= n) return;
+  const int cnt = nbr_count[p];
+  if (scan) {
+    bool hit = mflag[p] == stamp;
+    for (int k0 = 0; k0 < cnt && !hit; k0 += kBatch) {
+      uint32_t q[kBatch];
+      load_batch(nbr, n, k0, cnt, p, q);
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) hit |= (k0 + u < cnt) & (mflag[q[u]] == stamp);
+    }
+    if (!hit) return;
+    dirty[p] = stamp;
+  } else if (dirty[p] != stamp) {
+    return;
+  }
   const f3 xp = ld3_words(xt, p);
   float sum_C = 0.0f;
   f3 sg = {0.f, 0.f, 0.f};
-  const int cnt = nbr_count[p];
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
     load_batch(nbr, n, k0, cnt, p, q);
@@ -380,8 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
   }
   if (p >= n) return;
   // a dirty particle (moved by the density fix, or next to one) walks its whole list with the test
-  const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull &&
-                     (counters[C_MOVED_ALL + (it & 1)] != 0ull || dirty[p] == stamp);
+  const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull && dirty[p] == stamp;
   // x* and lambda of a particle come in one load (xt.w holds lambda)
   const float4 xp4 = xt[p];
   const f3 xp = xyz(xp4);
@@ -649,18 +682,18 @@ void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const Solve
   if (F.feat)
     PBF_LAUNCH(c, "density_features", k_density_lambda_c<true>, it, eslot, S, F, c->K, (const float4*)c->xs,
                (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
-               c->wmask, c->moved, c->counters);
+               c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters);
   else
     PBF_LAUNCH(c, "density_lambda", k_density_lambda_c<false>, it, eslot, S, F, c->K, (const float4*)c->xs,
                (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
-               c->wmask, c->moved, c->counters);
+               c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters);
 }
 
 void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
   KernelTimer* t = timer_begin(c, "mark_dirty");
-  hipLaunchKernelGGL(k_mark_dirty, dim3(16), dim3(kBlock), 0, c->stream, it, c->dirty_stamp, S, c->K,
+  hipLaunchKernelGGL(k_mark_dirty, dim3(kMarkBlocks), dim3(kBlock), 0, c->stream, it, c->dirty_stamp, S, c->K,
                      (const uint32_t*)c->key, (const uint32_t*)c->cell_start, (const uint32_t*)c->moved, c->dirty,
                      c->counters);
   timer_end(c, t);
@@ -669,7 +702,7 @@ void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S) {
 void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S) {
   PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->dirty_stamp, S, c->K, c->xt,
              (const float*)c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
-             (const uint32_t*)c->dirty, (const unsigned long long*)c->counters);
+             (const uint32_t*)c->mflag, c->dirty, (const unsigned long long*)c->counters);
 }
 
 void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S) {

@@ -149,6 +149,7 @@ struct pbf_context {
   uint32_t* heavy = nullptr;
   uint32_t* moved = nullptr;  // particles moved by the density fix in the current iteration
   uint32_t* dirty = nullptr;  // == dirty_stamp: the particle or one of its neighbours moved
+  uint32_t* mflag = nullptr;  // == dirty_stamp: the density fix moved the particle in that iteration
   uint32_t* cell_start = nullptr;
   uint32_t* cbox = nullptr;  // [6][cells] bounding box of each cell's particles (encoded)
   float4* chbox = nullptr;   // [2][n/64] bounding box (min, max) of each aligned run of 64 slots

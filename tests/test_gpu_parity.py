@@ -269,6 +269,40 @@ def test_density_fix_and_lambda_repair(pbf, scenes):
     _compare_state(g, o, "step with density fix")
 
 
+@pytest.mark.parametrize("case", ["crowded", "many"])
+def test_density_fix_exact_scan(pbf, scenes, case):
+    """The density fix (particles.h:303-305) where marking by neighbourhood
+    gives way to the exact scan of the lists: coincident pairs inside a cell of
+    thousands of particles ("crowded"), and more than 4,096 moved particles at
+    once ("many": every lattice point doubled).  Two steps must equal the
+    oracle bit for bit."""
+    rng = np.random.default_rng(41)
+    if case == "crowded":
+        centres = 0.25 + 0.02 * np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(6), indexing="ij"),
+                                         axis=-1)[..., ::-1].reshape(-1, 3)
+        crowd = (centres[:, None, :] + rng.uniform(-0.004, 0.004, size=(centres.shape[0], 40, 3))).reshape(-1, 3)
+        # coincident pairs first in index order, so that they lie early in
+        # the crowded cell's slots and in each other's capped lists
+        pairs = np.repeat(crowd[:200], 2, axis=0)
+        pos = np.concatenate([pairs, crowd[200:], scenes.lattice((12, 12, 12), origin=(0.025, 0.025, 0.025))])
+        box = (1.0, 1.0, 1.0)
+    else:
+        lat = scenes.jitter(scenes.lattice((16, 16, 16)), seed=43)
+        pos = np.concatenate([lat, lat])
+        box = (1.6, 1.6, 1.6)
+    pos = pos.astype(np.float32)
+    p = scenes.pbf_params(box, pos.shape[0])
+    g, o = _pair(pbf, p, pos, np.zeros_like(pos))
+    hacks = 0
+    for step in range(2):
+        g.step(1)
+        o.step(1)
+        hacks += g.stats()["density_hacks"]  # per step; the oracle's count is cumulative
+        _compare_state(g, o, f"{case} density fix, step {step + 1}")
+    print(f"{case}: {hacks} density fixes in 2 steps", flush=True)
+    assert hacks == o.counters()["hacks"] > 0
+
+
 def test_spans_and_remove(pbf, scenes):
     """Sub-spans leave other particles untouched; remove() drops the last particles."""
     p, pos, vel = _small_scene(scenes, seed=17, counts=(8, 8, 8), box=(0.8, 0.8, 0.8))

@@ -33,8 +33,9 @@ int check_hip(hipError_t e, const char* what) {
 
 // Events of finished launches are folded into the totals once a timer holds
 // kMaxPendingEvents pairs, so a long run keeps a bounded number of events
-// (the oldest launches have long finished: waiting on them costs nothing).
-constexpr size_t kMaxPendingEvents = 16;
+// (the oldest launches have long finished: waiting on them costs nothing);
+// their events are kept for later launches.
+constexpr size_t kMaxPendingEvents = 64;
 
 static void fold_oldest(KernelTimer& t, size_t count) {
   for (size_t i = 0; i < count && i < t.pending.size(); ++i) {
@@ -44,8 +45,8 @@ static void fold_oldest(KernelTimer& t, size_t count) {
     hipEventElapsedTime(&ms, pr.first, pr.second);
     t.total_ms += ms;
     t.launches++;
-    hipEventDestroy(pr.first);
-    hipEventDestroy(pr.second);
+    t.spare.push_back(pr.first);
+    t.spare.push_back(pr.second);
   }
   t.pending.erase(t.pending.begin(), t.pending.begin() + std::min(count, t.pending.size()));
 }
@@ -55,9 +56,16 @@ KernelTimer* timer_begin(pbf_context* c, const char* name) {
   for (auto& t : c->timers) {
     if (t.name == name) {
       if (t.pending.size() >= kMaxPendingEvents) fold_oldest(t, kMaxPendingEvents / 2);
-      hipEvent_t a, b;
-      hipEventCreate(&a);
-      hipEventCreate(&b);
+      hipEvent_t ev[2];
+      for (hipEvent_t& e : ev) {
+        if (t.spare.empty()) {
+          hipEventCreate(&e);
+        } else {
+          e = t.spare.back();
+          t.spare.pop_back();
+        }
+      }
+      const hipEvent_t a = ev[0], b = ev[1];
       hipEventRecord(a, c->stream);
       t.pending.push_back({a, b});
       return &t;
@@ -72,6 +80,15 @@ void timer_end(pbf_context* c, KernelTimer* t) {
 
 static void timers_resolve(pbf_context* c) {
   for (auto& t : c->timers) fold_oldest(t, t.pending.size());
+}
+
+// resolve, then release every event the timers hold
+static void timers_release(pbf_context* c) {
+  timers_resolve(c);
+  for (auto& t : c->timers) {
+    for (hipEvent_t e : t.spare) hipEventDestroy(e);
+    t.spare.clear();
+  }
 }
 
 // smallest float t with sqrtf(t) > h (sqrt with round-to-nearest is monotone)
@@ -359,7 +376,7 @@ int pbf_destroy(pbf_context* c) {
   if (!c) return PBF_OK;
   hipStreamSynchronize(c->stream);
   drop_graph(c);
-  timers_resolve(c);
+  timers_release(c);
   free_all(c);
   if (c->done_host) hipHostFree(c->done_host);
   if (c->region_a) hipEventDestroy(c->region_a);
@@ -1135,7 +1152,7 @@ int pbf_memcpy_device(void* dst, const void* src, uint64_t bytes) {
 int pbf_profile_kernels(pbf_context* c, const char* names) {
   if (!c || !names) return set_error(PBF_ERR_ARG, "null argument");
   hipStreamSynchronize(c->stream);
-  timers_resolve(c);
+  timers_release(c);
   c->timers.clear();
   ++c->version;
   std::string all(names);

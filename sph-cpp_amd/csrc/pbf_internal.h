@@ -116,6 +116,7 @@ struct KernelTimer {
   uint64_t launches = 0;
   double total_ms = 0.0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<hipEvent_t> spare;  // events of folded launches, reused
 };
 
 }  // namespace pbf

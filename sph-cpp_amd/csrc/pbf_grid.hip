@@ -624,55 +624,63 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
     const WalkCells w = walk_cells(ps, K.s);
     const GridView G = grid_of(K, p);
     uint32_t count = 0;  // accepted so far, uncapped (uniform across the wave)
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) {
-          const uint32_t cell = cell_key(pick3(w.cx, i), pick3(w.cy, j), pick3(w.cz, k), G, K);
-          if (cell >= K.ncells) continue;
-          const uint32_t b = start[cell], e = start[cell + 1];
-          if (e - b > kBoxMin && cell_far(box, cell, ps, K.ss)) continue;
-          // candidates come in groups of 64 (one per lane); two groups are
-          // loaded before the first is tested, so that two loads of a
-          // wavefront are in flight instead of one
-          if (e - b <= kChunkMin) {
-            for (uint32_t base = b; base < e; base += 128) {
-              const uint32_t qa = base + lane, qb = base + 64 + lane;
-              const bool va = qa < e && qa != p, vb = qb < e && qb != p;
-              float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = ta;
-              if (va) ta = xs[qa];
-              if (vb) tb = xs[qb];
-              if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count)) goto done;
-              if (base + 64 < e && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count)) goto done;
-            }
-            continue;
-          }
-          // crowded cell: each lane tests one chunk box, then the wavefront
-          // gathers the chunks that may hold a neighbour, in slot order
-          const uint32_t c_last = (e - 1) >> 6;
-          for (uint32_t c0 = b >> 6; c0 <= c_last; c0 += 64) {
-            const uint32_t ch = c0 + lane;
-            bool near = false;
-            if (ch <= c_last) {
-              const float4 lo = chbox[2 * ch], hi = chbox[2 * ch + 1];
-              near = !box_far(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, ps, K.ss);
-            }
-            uint64_t todo = __ballot(near);
-            while (todo) {
-              const uint32_t ca = (uint32_t)__builtin_ctzll(todo);
-              todo &= todo - 1ull;
-              const bool two = todo != 0ull;
-              const uint32_t cb = two ? (uint32_t)__builtin_ctzll(todo) : ca;
-              if (two) todo &= todo - 1ull;
-              const uint32_t qa = ((c0 + ca) << 6) + lane, qb = ((c0 + cb) << 6) + lane;
-              const bool va = qa >= b && qa < e && qa != p, vb = two && qb >= b && qb < e && qb != p;
-              float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = ta;
-              if (va) ta = xs[qa];
-              if (vb) tb = xs[qb];
-              if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count)) goto done;
-              if (two && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count)) goto done;
-            }
-          }
+    // lane c < 27 fetches cell c's range and box test, all in flight at once;
+    // the wavefront then walks the cells in order from registers
+    uint32_t lb = 0, le = 0;
+    if (lane < 27) {
+      const uint32_t cell =
+          cell_key(pick3(w.cx, (int)lane / 9), pick3(w.cy, ((int)lane / 3) % 3), pick3(w.cz, (int)lane % 3), G, K);
+      if (cell < K.ncells) {
+        lb = start[cell];
+        le = start[cell + 1];
+        if (le - lb > kBoxMin && cell_far(box, cell, ps, K.ss)) le = lb;  // far: walk nothing
+      }
+    }
+    for (int c = 0; c < 27; ++c) {
+      const uint32_t b = (uint32_t)__shfl((int)lb, c, 64), e = (uint32_t)__shfl((int)le, c, 64);
+      if (b == e) continue;
+      // candidates come in groups of 64 (one per lane); two groups are
+      // loaded before the first is tested, so that two loads of a
+      // wavefront are in flight instead of one
+      if (e - b <= kChunkMin) {
+        for (uint32_t base = b; base < e; base += 128) {
+          const uint32_t qa = base + lane, qb = base + 64 + lane;
+          const bool va = qa < e && qa != p, vb = qb < e && qb != p;
+          float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = ta;
+          if (va) ta = xs[qa];
+          if (vb) tb = xs[qb];
+          if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count)) goto done;
+          if (base + 64 < e && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count)) goto done;
         }
+        continue;
+      }
+      // crowded cell: each lane tests one chunk box, then the wavefront
+      // gathers the chunks that may hold a neighbour, in slot order
+      const uint32_t c_last = (e - 1) >> 6;
+      for (uint32_t c0 = b >> 6; c0 <= c_last; c0 += 64) {
+        const uint32_t ch = c0 + lane;
+        bool near = false;
+        if (ch <= c_last) {
+          const float4 lo = chbox[2 * ch], hi = chbox[2 * ch + 1];
+          near = !box_far(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, ps, K.ss);
+        }
+        uint64_t todo = __ballot(near);
+        while (todo) {
+          const uint32_t ca = (uint32_t)__builtin_ctzll(todo);
+          todo &= todo - 1ull;
+          const bool two = todo != 0ull;
+          const uint32_t cb = two ? (uint32_t)__builtin_ctzll(todo) : ca;
+          if (two) todo &= todo - 1ull;
+          const uint32_t qa = ((c0 + ca) << 6) + lane, qb = ((c0 + cb) << 6) + lane;
+          const bool va = qa >= b && qa < e && qa != p, vb = two && qb >= b && qb < e && qb != p;
+          float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = ta;
+          if (va) ta = xs[qa];
+          if (vb) tb = xs[qb];
+          if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count)) goto done;
+          if (two && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count)) goto done;
+        }
+      }
+    }
   done:
     if (lane == 0) {
       nbr_count[p] = (int32_t)(count < cap ? count : cap);

@@ -432,7 +432,7 @@ class Particles:
         _check(lib().pbf_set_warm_model(self._ctx, model._h))
         _check(lib().pbf_set_warm_gain(self._ctx, float(gain)))
 
-    def set_tuning(self, block_size: int = 256, heavy_budget: int = 1536, heavy_blocks: int = 2048) -> None:
+    def set_tuning(self, block_size: int = 256, heavy_budget: int = 768, heavy_blocks: int = 8192) -> None:
         t = Tuning(block_size, heavy_budget, heavy_blocks)
         _check(lib().pbf_set_tuning(self._ctx, C.byref(t)))
 

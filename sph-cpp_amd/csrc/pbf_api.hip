@@ -175,7 +175,7 @@ static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
   K.batched = 0;
   K.scenes = nullptr;
   K.sid = nullptr;
-  K.heavy_budget = 1536u;
+  K.heavy_budget = 768u;
   return PBF_OK;
 }
 
@@ -985,8 +985,8 @@ int pbf_autotune(pbf_context* c, uint32_t steps, pbf_tuning* best, double* best_
   if (!rc) rc = check_hip(hipEventCreate(&a), "hipEventCreate");
   if (!rc) rc = check_hip(hipEventCreate(&b), "hipEventCreate");
   const uint32_t blocks[] = {64u, 128u, 256u};
-  const uint32_t budgets[] = {768u, 1536u, 3072u};
-  const uint32_t hblocks[] = {1024u, 2048u, 8192u};
+  const uint32_t budgets[] = {384u, 768u, 1536u};
+  const uint32_t hblocks[] = {2048u, 8192u, 16384u};
   for (uint32_t bs : blocks)
     for (uint32_t hb : budgets)
       for (uint32_t hk : hblocks) {

@@ -496,16 +496,17 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
     cb[c] = in ? b : 0u;
     ce[c] = in ? e : 0u;
   }
-  if (K.count_overflow) {
-    // every candidate must be seen: route by the total number of candidates
-    uint32_t total = 0;
+  uint32_t total = 0;
 #pragma unroll
-    for (int c = 0; c < 27; ++c) total += ce[c] - cb[c];
-    if (total > K.heavy_budget) return false;
-  }
-  // otherwise the walk ends when the list is full; a lane that has scanned
-  // K.heavy_budget without filling its list hands the particle over (the
-  // wavefront builder starts again and writes the same entries)
+  for (int c = 0; c < 27; ++c) total += ce[c] - cb[c];
+  // every candidate must be seen to count the overflow: route by the total
+  if (K.count_overflow && total > K.heavy_budget) return false;
+  // otherwise the walk ends when the list is full; a lane whose 27 cells hold
+  // more than twice K.heavy_budget candidates and that has scanned
+  // K.heavy_budget of them without filling its list hands the particle over
+  // (the wavefront builder starts again and writes the same entries); lanes
+  // with fewer candidates walk them all
+  const uint32_t budget = total > 2u * K.heavy_budget ? K.heavy_budget : 0xFFFFFFFFu;
   uint32_t scanned = 0;
   int count = 0;
   const int cap = K.cap;
@@ -516,7 +517,7 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
     // (position buffers carry 8 rows of padding, so the loads need no clamp)
     for (uint32_t q0 = b; q0 < e; q0 += 8) {
       scanned += 8;
-      if (scanned > K.heavy_budget && !K.count_overflow) return false;
+      if (scanned > budget && !K.count_overflow) return false;
       float4 t[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) t[u] = xs[q0 + (uint32_t)u];

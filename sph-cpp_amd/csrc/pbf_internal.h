@@ -67,7 +67,7 @@ struct KParams {
   int cap;
   int stride;                    // list row length: cap rounded up to a multiple of 8
   int count_overflow;            // PBF_OPT_COUNT_OVERFLOW
-  uint32_t heavy_budget;         // candidates one lane scans before a wavefront takes over
+  uint32_t heavy_budget;         // candidates one lane scans before a wavefront takes over (see walk_list)
   // batched scenes: ncells is the total over the scenes, sid the scene of each slot
   int batched;
   const SceneGeo* scenes;
@@ -178,7 +178,7 @@ struct pbf_context {
   std::vector<void*> graph_sig;  // buffer pointers when the graph was captured
   uint64_t version = 0;  // bumped by anything that changes the launches of a step
   bool capturing = false;
-  pbf_tuning tune{256u, 1536u, 2048u};
+  pbf_tuning tune{256u, 768u, 8192u};
   unsigned long long* done_host = nullptr;  // pinned copy of C_DONE, polled between slot chunks
   float *lsum = nullptr, *lsuma = nullptr, *wlam = nullptr, *wgiven = nullptr;
   float4* xsave = nullptr;

@@ -9,7 +9,11 @@ was kept, the features of that solve's first density pass become samples,
 with target the change of the multiplier sum over the substep,
 (Lambda - Lambda_prev) / S: the network learns to extrapolate the previous
 sums to the current substep.  It is trained with the library's own MFMA
-forward/backward and Adam kernels (pbf_warm_model_train).
+forward/backward and Adam kernels (pbf_warm_model_train).  Afterwards the
+output layer is scaled by the least-squares gain of the output on samples of
+held-out scenes at the largest box scale (clamped to [0, 1]), so that only as
+much of the correction is applied as carries over to large scenes the network
+has not seen.
 
     python sph-cpp_amd/train_warm.py [--pairs 2000000] [--epochs 40] [--out sph-cpp_amd/warm_model.npy]
 
@@ -33,10 +37,11 @@ pbf = importlib.import_module("sph-cpp_amd")
 scenes = importlib.import_module("sph-cpp_amd.scenes")
 
 MODEL_PATH = os.path.join(HERE, "warm_model.npy")
+PBF_HIDDEN = pbf.PBF_WARM_HIDDEN
 
 
-def random_scene(rng, cap=60):
-    scale = int(rng.choice([1, 2, 4]))
+def random_scene(rng, cap=60, scale=None):
+    scale = int(scale or rng.choice([1, 2, 4]))
     box = (3.2 * scale, 1.6 * scale, 1.6 * scale)
     counts = (int(rng.integers(16, 49)) * scale, int(rng.integers(16, 33)) * scale, int(rng.integers(12, 33)) * scale)
     amp = float(rng.uniform(0.0, 0.05 * scenes.H))
@@ -47,17 +52,18 @@ def random_scene(rng, cap=60):
 
 
 def generate(model, pairs, seed=0, steps=200, cap=60, tol=0.01, max_slots=128, per_sample=10_000,
-             scenes_min=8, log=print):
-    """Each scene contributes at most pairs / scenes_min samples, each sampled
-    substep at most per_sample (a stride over the cell order), and the
-    sampled substeps are spread over the whole trajectory."""
+             scenes_min=8, log=print, budget=None, scale=None):
+    """Add samples until the model holds `pairs`.  Each scene contributes at
+    most `budget` samples (default: pairs / scenes_min), each sampled substep
+    at most per_sample (a stride over the cell order), and the sampled
+    substeps are spread over the whole trajectory."""
     rng = np.random.default_rng(seed)
     k = 0
     t0 = time.time()
-    budget = pairs // scenes_min + 1
+    budget = budget or pairs // scenes_min + 1
     while model.sample_count() < pairs:
         start = model.sample_count()
-        params, pos, vel, info = random_scene(rng, cap)
+        params, pos, vel, info = random_scene(rng, cap, scale)
         t = pbf.Particles(params)
         t.add(pos, pos, vel)
         log(json.dumps({"scene": k, **info, "particles": len(pos)}))
@@ -78,21 +84,59 @@ def generate(model, pairs, seed=0, steps=200, cap=60, tol=0.01, max_slots=128, p
         k += 1
 
 
+VALIDATION_PAIRS = 80_000
+VALIDATION_SCENES = 4
+VALIDATION_SCALE = 4  # held-out scenes at the largest box scale (the million-particle scenes the start serves)
+
+
+def output_gain(model, first, count):
+    """Least-squares gain g of the network output on samples [first, first +
+    count): the loss of g * net is a parabola in g, fitted from g = 0, 1/2, 1
+    (the output layer w3, b3 is the last 65 parameters, so scaling it scales
+    the output).  Returns g clamped to [0, 1] and the three losses."""
+    p = model.params()
+    losses = []
+    for g in (0.0, 0.5, 1.0):
+        q = p.copy()
+        q[-(PBF_HIDDEN + 1):] *= np.float32(g)
+        model.set_params(q)
+        losses.append(model.gradient(first, count)[1])
+    model.set_params(p)
+    c, lh, l1 = losses
+    a = 2.0 * (l1 - c) - 4.0 * (lh - c)   # L(g) = a g^2 - 2 b g + c
+    b = 0.5 * (a - (l1 - c))
+    g = b / a if a > 0 else 0.0
+    return float(min(max(g, 0.0), 1.0)), losses
+
+
 def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print):
-    model = pbf.WarmModel(pairs + 2 * 40_000)
+    model = pbf.WarmModel(pairs + 2 * VALIDATION_PAIRS)
     t0 = time.time()
     generate(model, pairs, seed, log=log)
     t1 = time.time()
     model.init(seed)
     n = model.sample_count()
-    _, l0 = model.gradient(0, min(n, 200_000))
+    _, l0 = model.gradient(0, n)
     half = epochs // 2
     m1 = model.train(half, batch, lr, seed + 1)
     m2 = model.train(epochs - half, batch, lr / 3, seed + 2)
-    _, l1 = model.gradient(0, min(n, 200_000))
+    _, l1 = model.gradient(0, n)
     t2 = time.time()
+    # held-out scenes (appended after the trained range): the output gain that
+    # minimises their loss is folded into the output layer, so a correction
+    # that does not carry over to new scenes shrinks towards zero and the warm
+    # start towards the previous multiplier sums it corrects
+    generate(model, n + VALIDATION_PAIRS, seed + 1000, log=log, budget=VALIDATION_PAIRS // VALIDATION_SCENES + 1,
+             scale=VALIDATION_SCALE)
+    nv = model.sample_count() - n
+    gain, vl = output_gain(model, n, nv)
+    p = model.params()
+    p[-(PBF_HIDDEN + 1):] *= np.float32(gain)
+    model.set_params(p)
     summary = {"samples": n, "epochs": epochs, "batch": batch, "loss_before": l0, "loss_after": l1,
-               "mean_batch_loss": [m1, m2], "data_seconds": round(t1 - t0, 1), "train_seconds": round(t2 - t1, 1)}
+               "mean_batch_loss": [m1, m2], "validation_samples": nv,
+               "validation_loss_at_gain_0_half_1": vl, "output_gain": gain,
+               "data_seconds": round(t1 - t0, 1), "train_seconds": round(t2 - t1, 1)}
     return model, summary
 
 

@@ -336,7 +336,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->lam, N)) || (rc = dalloc(&c->rhoa, N)) || (rc = dalloc(&c->lama, N)) ||
       (rc = dalloc(&c->oid, N)) || (rc = dalloc(&c->oida, N)) || (rc = dalloc(&c->key, N)) ||
       (rc = dalloc(&c->keya, N)) || (rc = dalloc(&c->slot, N)) || (rc = dalloc(&c->tmp, N + 1)) ||
-      (rc = dalloc(&c->tmpo, N + 1)) || (rc = dalloc(&c->nbr, N * (uint64_t)c->K.stride)) ||
+      (rc = dalloc(&c->tmpo, N + 1)) || (rc = dalloc(&c->nbr, (N + 63) / 64 * 64 * (uint64_t)c->K.stride)) ||
       (rc = dalloc(&c->nbr_count, N)) || (rc = dalloc(&c->counters, (uint64_t)C_NUM)) ||
       (rc = dalloc(&c->stage, 4 * N)) || (rc = dalloc(&c->stage64, N + 1)) || (rc = alloc_grid(c)) ||
       (rc = dalloc(&c->sk[0], N + 1)) || (rc = dalloc(&c->sk[1], N + 1)) || (rc = dalloc(&c->sv[0], N + 1)) ||
@@ -344,7 +344,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->roffs, radix_hist_size(N) + 1)) ||
       (rc = dalloc(&c->rpart, scan_partials_size(radix_hist_size(N) + 1))) || (rc = dalloc(&c->heavy, N)) ||
       (rc = dalloc(&c->chbox, 2 * ((N + 63) / 64))) || (rc = dalloc(&c->wmask, 2 * N)) ||
-      (rc = dalloc(&c->cnb, N * (uint64_t)c->K.stride)) || (rc = dalloc(&c->cnb_count, N)) ||
+      (rc = dalloc(&c->cnb, (N + 63) / 64 * 64 * (uint64_t)c->K.stride)) || (rc = dalloc(&c->cnb_count, N)) ||
       (rc = dalloc(&c->moved, N)) || (rc = dalloc(&c->dirty, N)) || (rc = dalloc(&c->mflag, N)) ||
       (rc = dalloc(&c->ehist, (uint64_t)kMaxSolveSlots + 1)) || (rc = dalloc(&c->lsum, N)) ||
       (rc = dalloc(&c->lsuma, N)) || (rc = dalloc(&c->wlam, N)) || (rc = dalloc(&c->wgiven, N)) ||
@@ -1100,13 +1100,13 @@ int pbf_get_neighbor_lists(pbf_context* c, uint64_t* out, int32_t* counts) {
   std::vector<uint32_t> o, cnt, lists;
   if (int rc = download_u32(c, c->oid, n, o)) return rc;
   if (int rc = download_u32(c, (const uint32_t*)c->nbr_count, n, cnt)) return rc;
-  if (int rc = download_u32(c, (const uint32_t*)c->nbr, n * K, lists)) return rc;
+  if (int rc = download_u32(c, (const uint32_t*)c->nbr, (n + 63) / 64 * 64 * c->K.stride, lists)) return rc;
   std::memset(out, 0, n * K * sizeof(uint64_t));
   for (uint64_t p = 0; p < n; ++p) {
     const uint64_t op = o[p];
     const int32_t cp = (int32_t)cnt[p];
     counts[op] = cp;
-    for (int32_t k = 0; k < cp; ++k) out[op * K + k] = o[lists[(uint64_t)k * n + p]];
+    for (int32_t k = 0; k < cp; ++k) out[op * K + k] = o[lists[list_at((uint32_t)k, (uint32_t)p, c->K.stride)]];
   }
   return PBF_OK;
 }

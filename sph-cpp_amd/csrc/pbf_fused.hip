@@ -33,13 +33,13 @@ static inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) 
 
 __device__ __forceinline__ f3 xyz(float4 t) { return {t.x, t.y, t.z}; }
 
-// Entries k0 .. k0+7 of particle p's column-major list (entry k at list[k*n + p]):
-// for each entry, the 64 lanes of a wavefront read one contiguous segment.
-// Entries at or past cnt are replaced by p, which keeps every gather in bounds.
-__device__ __forceinline__ void load_batch(const int32_t* __restrict__ list, uint32_t n, int k0, int cnt,
+// Entries k0 .. k0+7 of particle p's list (tiled, list_at): for each entry,
+// the 64 lanes of a wavefront read one contiguous segment.  Entries at or
+// past cnt are replaced by p, which keeps every gather in bounds.
+__device__ __forceinline__ void load_batch(const int32_t* __restrict__ list, uint32_t stride, int k0, int cnt,
                                            uint32_t p, uint32_t (&q)[kBatch]) {
 #pragma unroll
-  for (int u = 0; u < kBatch; ++u) q[u] = (k0 + u < cnt) ? (uint32_t)list[(uint64_t)(k0 + u) * n + p] : p;
+  for (int u = 0; u < kBatch; ++u) q[u] = (k0 + u < cnt) ? (uint32_t)list[list_at(k0 + u, p, stride)] : p;
 }
 
 // The passes that build compact lists run one wavefront per block and stage
@@ -55,8 +55,8 @@ __device__ __forceinline__ void load_batch_lds(const int32_t* __restrict__ s_lis
 }
 
 __device__ __forceinline__ void flush_compact(const int32_t* __restrict__ s_list, uint32_t lane, int nc,
-                                              int32_t* __restrict__ cnb, uint32_t n, uint32_t p) {
-  for (int k = 0; k < nc; ++k) cnb[(uint64_t)k * n + p] = s_list[k * kStageBlock + lane];
+                                              int32_t* __restrict__ cnb, uint32_t stride, uint32_t p) {
+  for (int k = 0; k < nc; ++k) cnb[list_at(k, p, stride)] = s_list[k * kStageBlock + lane];
 }
 
 // x limited to [lo, hi] (NaN gives lo)
@@ -91,7 +91,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   int hack_at = -1;
   for (int k0 = 0; k0 < cnt && hack_at < 0; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(nbr, n, k0, cnt, p, q);
+    load_batch(nbr, K.stride, k0, cnt, p, q);
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
@@ -130,7 +130,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   if (hack_at >= 0) {
     // particles.h:304-305: move x*_p, then finish this particle with a plain walk
     for (int k = hack_at; k < cnt; ++k) {
-      const uint32_t qk = (uint32_t)nbr[(uint64_t)k * n + p];
+      const uint32_t qk = (uint32_t)nbr[list_at(k, p, K.stride)];
       const f3 xq = ld3(xs, qk);
       f3 d = sub3(xp, xq);
       float d2 = magsq3(d);
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
     bool hit = mflag[p] == stamp;
     for (int k0 = 0; k0 < cnt && !hit; k0 += kBatch) {
       uint32_t q[kBatch];
-      load_batch(nbr, n, k0, cnt, p, q);
+      load_batch(nbr, K.stride, k0, cnt, p, q);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) hit |= (k0 + u < cnt) & (mflag[q[u]] == stamp);
     }
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
   f3 sg = {0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(nbr, n, k0, cnt, p, q);
+    load_batch(nbr, K.stride, k0, cnt, p, q);
     f3 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = ld3_words(xt, q[u]);
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
   const uint64_t m1 = (plain || K.cap <= 64) ? ~0ull : wmask[n + p];
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(nbr, n, k0, cnt, p, q);
+    load_batch(nbr, K.stride, k0, cnt, p, q);
     const uint32_t bits = (uint32_t)((k0 < 64 ? m0 >> k0 : m1 >> (k0 - 64)) & 0xFFu);
     float4 t[kBatch];
 #pragma unroll
@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_dp(uint32_t n, KParams K, const
   const int cnt = nbr_count[p];
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(nbr, n, k0, cnt, p, q);
+    load_batch(nbr, K.stride, k0, cnt, p, q);
     float4 t[kBatch];
     float lq[kBatch];
 #pragma unroll
@@ -514,7 +514,7 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
     if (FROM_LDS)
       load_batch_lds(list, lane, k0, nc, p, q);
     else
-      load_batch(list, n, k0, nc, p, q);
+      load_batch(list, K.stride, k0, nc, p, q);
     float4 t[kBatch], vq[kBatch];
     float rq[kBatch];
 #pragma unroll
@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
   int nc = 0;
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(nbr, n, k0, cnt, p, q);
+    load_batch(nbr, K.stride, k0, cnt, p, q);
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
     }
   }
   cnb_count[p] = nc;
-  flush_compact(s_list, lane, nc, cnb, n, p);
+  flush_compact(s_list, lane, nc, cnb, K.stride, p);
   const f3 dv = xsph_sum<true>(s_list, n, lane, nc, p, xp, vp, K, xs, rho, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, c
   f3 w = {0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(cnb, n, k0, cnt, p, q);
+    load_batch(cnb, K.stride, k0, cnt, p, q);
     float4 t[kBatch], vq[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -635,7 +635,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
   f3 eta = {0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(cnb, n, k0, cnt, p, q);
+    load_batch(cnb, K.stride, k0, cnt, p, q);
     float4 t[kBatch], wq[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {

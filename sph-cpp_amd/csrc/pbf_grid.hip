@@ -543,7 +543,7 @@ done:
     count = cap;
   }
   nbr_count[p] = count;
-  for (int k = 0; k < count; ++k) nbr[(uint64_t)k * n + p] = s_list[k * kFillBlock + lane];
+  for (int k = 0; k < count; ++k) nbr[list_at(k, p, K.stride)] = s_list[k * kFillBlock + lane];
   return true;
 }
 
@@ -598,7 +598,7 @@ __device__ __forceinline__ bool take_group(uint32_t q, bool valid, float4 t, f3 
   const uint64_t m = __ballot(acc);
   if (acc) {
     const uint32_t pos = count + (uint32_t)__popcll(m & below);
-    if (pos < cap) nbr[(uint64_t)pos * n + p] = (int32_t)q;
+    if (pos < cap) nbr[list_at(pos, p, K.stride)] = (int32_t)q;
   }
   count += (uint32_t)__popcll(m);
   return count >= cap && !K.count_overflow;

@@ -15,11 +15,12 @@
 //   cell_start       uint32[C+2] first sorted slot of every cell (C = the outside bucket)
 //   heavy            uint32[n]   particles whose 27 cells hold too many candidates
 //                                for one lane (built by a whole wavefront instead)
-//   nbr              int32[K][n] neighbour lists, column major: nbr[k*n + p] (entry k of
-//                                a wavefront's 64 particles is one contiguous 256-byte read)
+//   nbr              int32[n/64][K][64] neighbour lists tiled by groups of 64 particles:
+//                                nbr[list_at(k, p, K)] (entry k of a wavefront's 64 particles
+//                                is one contiguous 256-byte read, its whole list one block)
 //   nbr_count        int32[n]
-//   cnb, cnb_count   int32[K][n], int32[n]  compact lists of the entries within h, column
-//                                major (built by the density and first XSPH passes)
+//   cnb, cnb_count   int32[n/64][K][64], int32[n]  compact lists of the entries within h
+//                                (same tiling; built by the first XSPH pass)
 //   counters         u64[16]     overflow, outside, hacks, hack flags, solve state, ...
 //   ehist            uint32[kMaxSolveSlots+1]  max density error per solve slot (float bits)
 //   lsum, lsuma      float[n]    multipliers summed over the updates of the last solve
@@ -110,6 +111,14 @@ struct FeatCtl {
   float* feat;          // [n][8] (null: no features)
   const float* lprev;   // previous multiplier sums
 };
+
+// Neighbour lists are tiled by groups of 64 particles: entry k of particle p
+// sits at ((p / 64) * stride + k) * 64 + p % 64.  Entry k of a wavefront's 64
+// particles is one 256-byte read, and a wavefront's whole list is one
+// contiguous block of stride * 256 bytes.
+__host__ __device__ __forceinline__ uint64_t list_at(uint32_t k, uint32_t p, uint32_t stride) {
+  return ((uint64_t)(p >> 6) * stride + k) * 64u + (p & 63u);
+}
 
 struct KernelTimer {
   std::string name;

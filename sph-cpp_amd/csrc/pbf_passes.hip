@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(kBlock) k_density(uint32_t n, uint32_t b, uint
   uint32_t hacks = 0;
   const int cnt = nbr_count[p];
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t q = (uint32_t)nbr[(uint64_t)k * n + p];
+    const uint32_t q = (uint32_t)nbr[list_at(k, p, K.stride)];
     const f3 xq = ld3(xs, q);
     f3 d = sub3(xp, xq);
     float d2 = magsq3(d);
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(kBlock) k_lambda(uint32_t n, uint32_t b, uint3
   f3 sg = {0.f, 0.f, 0.f};
   const int cnt = nbr_count[p];
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t q = (uint32_t)nbr[(uint64_t)k * n + p];
+    const uint32_t q = (uint32_t)nbr[list_at(k, p, K.stride)];
     const f3 d = sub3(xp, ld3(pos, q));
     const float d2 = magsq3(d);
     if (!within_h(d2, K)) continue;
@@ -145,7 +145,7 @@ __device__ __forceinline__ f3 pressure_dp(uint32_t p, uint32_t n, const f3 xp, c
                                           const int32_t* __restrict__ nbr, int cnt) {
   f3 dp = {0.f, 0.f, 0.f};
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t q = (uint32_t)nbr[(uint64_t)k * n + p];
+    const uint32_t q = (uint32_t)nbr[list_at(k, p, K.stride)];
     const f3 d = sub3(xp, ld3(pos, q));
     const float d2 = magsq3(d);
     if (!within_h(d2, K)) continue;
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(kBlock) k_viscosity(uint32_t n, uint32_t b, ui
   f3 dv = {0.f, 0.f, 0.f};
   const int cnt = nbr_count[p];
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t q = (uint32_t)nbr[(uint64_t)k * n + p];
+    const uint32_t q = (uint32_t)nbr[list_at(k, p, K.stride)];
     const f3 d = sub3(xp, ld3(xs, q));
     const float d2 = magsq3(d);
     if (!within_h(d2, K)) continue;
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) k_vorticity(uint32_t n, uint32_t b, ui
   f3 w = {0.f, 0.f, 0.f};
   const int cnt = nbr_count[p];
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t q = (uint32_t)nbr[(uint64_t)k * n + p];
+    const uint32_t q = (uint32_t)nbr[list_at(k, p, K.stride)];
     const f3 d = sub3(xp, ld3(xs, q));
     const float d2 = magsq3(d);
     if (!within_h(d2, K)) continue;
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity(uint32_t n, uint32_t
   f3 eta = {0.f, 0.f, 0.f};
   const int cnt = nbr_count[p];
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t q = (uint32_t)nbr[(uint64_t)k * n + p];
+    const uint32_t q = (uint32_t)nbr[list_at(k, p, K.stride)];
     const f3 d = sub3(xp, ld3(xs, q));
     const float d2 = magsq3(d);
     if (!within_h(d2, K)) continue;
@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(kBlock) k_color(uint32_t n, uint32_t b, uint32
   f3 col = {0.f, 0.f, 0.f};
   const int cnt = nbr_count[p];
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t q = (uint32_t)nbr[(uint64_t)k * n + p];
+    const uint32_t q = (uint32_t)nbr[list_at(k, p, K.stride)];
     const f3 d = sub3(xp, ld3(xs, q));
     const float d2 = magsq3(d);
     if (!within_h(d2, K)) continue;
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(kBlock) k_tension(uint32_t n, uint32_t b, uint
   f3 f = {0.f, 0.f, 0.f};
   const int cnt = nbr_count[p];
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t q = (uint32_t)nbr[(uint64_t)k * n + p];
+    const uint32_t q = (uint32_t)nbr[list_at(k, p, K.stride)];
     const f3 r = sub3(xp, ld3(xs, q));
     float r_mag = sqrtf(magsq3(r));
     if ((double)r_mag < K.st_rmin) r_mag = (float)K.st_rmin;

@@ -62,10 +62,41 @@ __device__ __forceinline__ float poly6(float r, const KParams& K) {
   if (r > K.h) return 0.0f;
   return poly6_in(r, K);
 }
+// Three correctly rounded quotients a/b with one divisor.  A correctly
+// rounded divide on gfx950 is a Markstein division (P. Markstein, IBM J. Res.
+// Dev. 34(1), 1990): reciprocal estimate, one Newton step
+// r1 = r + r (1 - b r), then q = a r1 corrected twice with the exact residual
+// a - b q (FMA); range scaling and a fix-up step only change anything for
+// zero, infinite or NaN operands and for quotients or operands near the ends
+// of the exponent range.  Inside 2^-90 <= |a| <= 2^90 (or a == 0) and
+// 2^-100 <= b <= 2^100 this computes exactly those steps, sharing the
+// divisor's reciprocal and its Newton step between the three numerators;
+// anything else takes the plain divide, so every result equals a / b.
+__device__ __forceinline__ bool div_in_range(float a) {
+  const float m = fabsf(a);
+  return (a == 0.0f) | ((m >= 0x1p-90f) & (m <= 0x1p90f));
+}
+__device__ __forceinline__ float div_markstein(float a, float b, float r1) {
+  const float q = a * r1;
+  const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q, a), r1, q);
+  const float q2 = __builtin_fmaf(__builtin_fmaf(-b, q1, a), r1, q1);
+  return a == 0.0f ? a : q2;  // b > 0 here: the quotient of +-0 is +-0
+}
+__device__ __forceinline__ f3 div3_shared(f3 a, float b) {
+  // one condition, evaluated without branches
+  const bool fast = (b >= 0x1p-100f) & (b <= 0x1p100f) & div_in_range(a.x) & div_in_range(a.y) & div_in_range(a.z);
+  if (__builtin_expect(fast, 1)) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float r1 = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+    return {div_markstein(a.x, b, r1), div_markstein(a.y, b, r1), div_markstein(a.z, b, r1)};
+  }
+  return div3(a, b);
+}
+
 // Del_Spikey<float,3>::operator() for r <= h, with r = |d| (kernels.h:171)
 __device__ __forceinline__ f3 spiky_in(f3 d, float r, const KParams& K) {
   const float s = K.spiky_norm * (K.h - r) * (K.h - r);
-  return div3(smul3(s, d), r + K.r_eps);
+  return div3_shared(smul3(s, d), r + K.r_eps);
 }
 // C_Spline<float,3>::operator() (kernels.h:220-228); the first branch is in double
 __device__ __forceinline__ float cspline(float r, const KParams& K) {

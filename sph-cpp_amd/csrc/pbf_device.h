@@ -93,6 +93,23 @@ __device__ __forceinline__ f3 div3_shared(f3 a, float b) {
   return div3(a, b);
 }
 
+// Correctly rounded square root.  On gfx950 it is the hardware estimate s
+// corrected to its neighbour s -+ 1 ulp by the sign of the exact residual
+// x - s' s (FMA), wrapped in a rescaling for x < 2^-96 and a pass-through of
+// +-0 and +inf.  For 2^-96 <= x <= 2^96 this computes exactly those steps
+// without the wrapper; other x take the plain sqrtf, so every result equals
+// sqrtf(x).
+__device__ __forceinline__ float sqrt_cr(float x) {
+  if (__builtin_expect((x >= 0x1p-96f) & (x <= 0x1p96f), 1)) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __int_as_float(__float_as_int(s) - 1);
+    const float sp = __int_as_float(__float_as_int(s) + 1);
+    const float r = __builtin_fmaf(-sm, s, x) <= 0.0f ? sm : s;
+    return __builtin_fmaf(-sp, s, x) > 0.0f ? sp : r;
+  }
+  return sqrtf(x);
+}
+
 // Del_Spikey<float,3>::operator() for r <= h, with r = |d| (kernels.h:171)
 __device__ __forceinline__ f3 spiky_in(f3 d, float r, const KParams& K) {
   const float s = K.spiky_norm * (K.h - r) * (K.h - r);

@@ -106,7 +106,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
       bits |= (uint32_t)acc << u;
       hack_at = hk ? k0 + u : hack_at;
       if (acc) {
-        const float r = sqrtf(d2);
+        const float r = sqrt_cr(d2);
         density += K.mass * poly6_in(r, K);
         const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
         dec3(sg, grad);
@@ -142,7 +142,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
         d2 = magsq3(d);
         if (!within_h(d2, K)) continue;
       }
-      const float r = sqrtf(d2);
+      const float r = sqrt_cr(d2);
       density += K.mass * poly6_in(r, K);
       const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
       dec3(sg, grad);
@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
       const f3 d = sub3(xp, t[u]);
       const float d2 = magsq3(d);
       if (k0 + u >= cnt || !within_h(d2, K)) continue;
-      const float r = sqrtf(d2);
+      const float r = sqrt_cr(d2);
       const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
       dec3(sg, grad);
       sum_C += magsq3(grad);
@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
       const f3 d = sub3(xp, xyz(t[u]));
       const float d2 = magsq3(d);
       if (k0 + u >= cnt || !((bits >> u) & 1u) || (plain && !within_h(d2, K))) continue;
-      const float r = sqrtf(d2);
+      const float r = sqrt_cr(d2);
       acc3(dp, smul3(lp + t[u].w, spiky_in(d, r, K)));
     }
   }
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_dp(uint32_t n, KParams K, const
       const f3 d = sub3(xp, xyz(t[u]));
       const float d2 = magsq3(d);
       if (k0 + u >= cnt || !within_h(d2, K)) continue;
-      const float r = sqrtf(d2);
+      const float r = sqrt_cr(d2);
       acc3(dp, smul3(lp + lq[u], spiky_in(d, r, K)));
     }
   }
@@ -526,7 +526,7 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= nc) continue;
-      const float r = sqrtf(magsq3(sub3(xp, xyz(t[u]))));
+      const float r = sqrt_cr(magsq3(sub3(xp, xyz(t[u]))));
       acc3(dv, div3(mul3(sub3(xyz(vq[u]), vp), poly6_in(r, K)), rq[u]));
     }
   }

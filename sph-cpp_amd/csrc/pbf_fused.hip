@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
     for (int u = 0; u < kBatch; ++u) {
       const float d2 = magsq3(sub3(xp, xyz(t[u])));
       const bool acc = (k0 + u < cnt) & within_h(d2, K);
-      s_list[(acc ? nc : K.cap) * kStageBlock + lane] = (int32_t)q[u];  // spare row cap when rejected
+      s_list[nc * kStageBlock + lane] = (int32_t)q[u];  // a rejected entry is overwritten by the next accepted one
       nc += acc ? 1 : 0;
     }
   }

@@ -521,16 +521,16 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
       float4 t[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) t[u] = xs[q0 + (uint32_t)u];
-      // branch-free acceptance: every candidate is stored, an accepted one
-      // in row min(count, cap) and a rejected one in the spare row cap;
-      // rows >= cap are never read
+      // branch-free acceptance: every candidate is stored in row
+      // min(count, cap) and only an accepted one advances count, so a
+      // rejected one is overwritten by the next accepted one (rows >= the
+      // final count, and the spare row cap, are never read)
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const uint32_t q = q0 + (uint32_t)u;
         const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
         const bool acc = (q < e) & (q != p) & (d2 < K.ss);
-        const int row = acc ? min(count, cap) : cap;
-        s_list[row * kFillBlock + lane] = (int32_t)q;
+        s_list[min(count, cap) * kFillBlock + lane] = (int32_t)q;
         count += acc ? 1 : 0;
       }
       // a full list cannot change any more; only the overflow count could

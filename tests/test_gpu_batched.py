@@ -93,3 +93,28 @@ def test_graph_replay_equals_eager(pbf, scenes, batched):
     assert np.array_equal(a.positions(), b.positions())
     a.close()
     b.close()
+
+
+def test_graph_replay_with_density_fixes(pbf, scenes):
+    """Replays of a scene where the near-coincidence fix (particles.h:303-305)
+    fires thousands of times per step (every lattice point doubled): the fix's
+    moved and dirty stamps repeat from replay to replay and are cleared inside
+    the graph; replays must still equal eager steps bit for bit."""
+    lat = scenes.jitter(scenes.lattice((16, 16, 16)), seed=43)
+    pos = np.concatenate([lat, lat]).astype(np.float32)
+    params = scenes.pbf_params((1.6, 1.6, 1.6), pos.shape[0])
+    a = pbf.Particles(params)
+    b = pbf.Particles(params)
+    a.add(pos, pos, np.zeros_like(pos))
+    b.add(pos, pos, np.zeros_like(pos))
+    hacks = 0
+    for n in (2, 4, 2):
+        a.step_graph(n)
+        b.step(n)
+        a.sync()
+        hacks += b.stats()["density_hacks"]
+        assert np.array_equal(a.positions(), b.positions()), f"after {n} more steps"
+        assert np.array_equal(a.velocities(), b.velocities())
+    assert hacks > 0
+    a.close()
+    b.close()

@@ -337,6 +337,17 @@ int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes);
 int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes);
 int pbf_memcpy_device(void* dst, const void* src, uint64_t bytes);  /* device to device */
 
+/* ---- self-check (no counterpart in the reference) ----
+ * The neighbour passes divide and take square roots with the library's own
+ * sequences, built to give the bits of the correctly rounded operations for
+ * every input.  Compares them with the plain operations on `samples`
+ * pseudo-random operand sets (raw bit patterns, operands shaped like the
+ * passes' own, log-uniform magnitudes over 2^-48..2^48) and writes out[0] =
+ * divide mismatches, out[1] = square-root mismatches, out[2] / out[3] = samples
+ * that took the shared-reciprocal divide / the unwrapped square root, out[4] =
+ * divide mismatches among the operands shaped like the passes' own. */
+int pbf_check_arithmetic(uint64_t samples, uint64_t seed, uint64_t* out);
+
 /* ---- output ----
  * Append one positions record to `path`, with the content AdiosWriter::write_particles
  * writes per call (adios_writer.h:78-114, adios_writer.xml:4-14): int64 global_bytes,

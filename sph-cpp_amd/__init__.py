@@ -189,6 +189,7 @@ def lib():
     _sig(L, "pbf_warm_model_adam_step", [_P, _F, C.c_float])
     _sig(L, "pbf_warm_model_train", [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _F])
     _sig(L, "pbf_warm_model_predict", [_P, _F, C.c_uint64, _F])
+    _sig(L, "pbf_check_arithmetic", [C.c_uint64, C.c_uint64, _U64])
     _sig(L, "pbf_sync", [_P])
     _sig(L, "pbf_get_stats", [_P, C.POINTER(Stats)])
     _sig(L, "pbf_grid_dims", [_P, _U64])
@@ -550,6 +551,15 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def check_arithmetic(samples: int, seed: int = 0) -> dict:
+    """pbf_check_arithmetic: the library's divide and square root against the
+    plain operations on the device."""
+    out = np.zeros(5, dtype=np.uint64)
+    _check(lib().pbf_check_arithmetic(int(samples), int(seed), out.ctypes.data_as(_U64)))
+    return {"divide_mismatches": int(out[0]), "sqrt_mismatches": int(out[1]),
+            "divide_fast": int(out[2]), "sqrt_fast": int(out[3]), "divide_mismatches_shaped": int(out[4])}
 
 
 def sort_by_key_u32(keys: np.ndarray, values: np.ndarray, key_bits: int = 32):

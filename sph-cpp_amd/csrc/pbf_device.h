@@ -67,14 +67,15 @@ __device__ __forceinline__ float poly6(float r, const KParams& K) {
 // Dev. 34(1), 1990): reciprocal estimate, one Newton step
 // r1 = r + r (1 - b r), then q = a r1 corrected twice with the exact residual
 // a - b q (FMA); range scaling and a fix-up step only change anything for
-// zero, infinite or NaN operands and for quotients or operands near the ends
-// of the exponent range.  Inside 2^-90 <= |a| <= 2^90 (or a == 0) and
-// 2^-100 <= b <= 2^100 this computes exactly those steps, sharing the
-// divisor's reciprocal and its Newton step between the three numerators;
-// anything else takes the plain divide, so every result equals a / b.
+// zero, infinite or NaN operands, denormal operands or quotients, and
+// operands 96 or more binades apart.  For b and |a| in [2^-40, 2^40] (or
+// a == 0) none of that happens and this computes exactly those steps,
+// sharing the divisor's reciprocal and its Newton step between the three
+// numerators; anything else takes the plain divide, so every result equals
+// a / b (pbf_check_arithmetic compares the two on the device).
 __device__ __forceinline__ bool div_in_range(float a) {
   const float m = fabsf(a);
-  return (a == 0.0f) | ((m >= 0x1p-90f) & (m <= 0x1p90f));
+  return (a == 0.0f) | ((m >= 0x1p-40f) & (m <= 0x1p40f));
 }
 __device__ __forceinline__ float div_markstein(float a, float b, float r1) {
   const float q = a * r1;
@@ -82,10 +83,11 @@ __device__ __forceinline__ float div_markstein(float a, float b, float r1) {
   const float q2 = __builtin_fmaf(__builtin_fmaf(-b, q1, a), r1, q1);
   return a == 0.0f ? a : q2;  // b > 0 here: the quotient of +-0 is +-0
 }
+__device__ __forceinline__ bool div3_fast(f3 a, float b) {  // one condition, evaluated without branches
+  return (b >= 0x1p-40f) & (b <= 0x1p40f) & div_in_range(a.x) & div_in_range(a.y) & div_in_range(a.z);
+}
 __device__ __forceinline__ f3 div3_shared(f3 a, float b) {
-  // one condition, evaluated without branches
-  const bool fast = (b >= 0x1p-100f) & (b <= 0x1p100f) & div_in_range(a.x) & div_in_range(a.y) & div_in_range(a.z);
-  if (__builtin_expect(fast, 1)) {
+  if (__builtin_expect(div3_fast(a, b), 1)) {
     const float r = __builtin_amdgcn_rcpf(b);
     const float r1 = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
     return {div_markstein(a.x, b, r1), div_markstein(a.y, b, r1), div_markstein(a.z, b, r1)};
@@ -99,8 +101,9 @@ __device__ __forceinline__ f3 div3_shared(f3 a, float b) {
 // +-0 and +inf.  For 2^-96 <= x <= 2^96 this computes exactly those steps
 // without the wrapper; other x take the plain sqrtf, so every result equals
 // sqrtf(x).
+__device__ __forceinline__ bool sqrt_fast(float x) { return (x >= 0x1p-96f) & (x <= 0x1p96f); }
 __device__ __forceinline__ float sqrt_cr(float x) {
-  if (__builtin_expect((x >= 0x1p-96f) & (x <= 0x1p96f), 1)) {
+  if (__builtin_expect(sqrt_fast(x), 1)) {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float sm = __int_as_float(__float_as_int(s) - 1);
     const float sp = __int_as_float(__float_as_int(s) + 1);

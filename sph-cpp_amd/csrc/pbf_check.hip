@@ -58,7 +58,7 @@ __global__ void k_check_arithmetic(uint64_t samples, uint64_t seed, unsigned lon
     const f3 q = div3_shared(a, b);
     const f3 ref = div3(a, b);
     fast_div += div3_fast(a, b) ? 1 : 0;
-    const bool bad = !(same_bits(q.x, ref.x) & same_bits(q.y, ref.y) & same_bits(q.z, ref.z));
+    const bool bad = !(same_bits(q.x, ref.x) && same_bits(q.y, ref.y) && same_bits(q.z, ref.z));
     bad_div += bad ? 1 : 0;
     bad_shaped += (bad && kind == 0) ? 1 : 0;
     fast_sqrt += sqrt_fast(x) ? 1 : 0;

@@ -84,7 +84,9 @@ __device__ __forceinline__ float div_markstein(float a, float b, float r1) {
   return a == 0.0f ? a : q2;  // b > 0 here: the quotient of +-0 is +-0
 }
 __device__ __forceinline__ bool div3_fast(f3 a, float b) {  // one condition, evaluated without branches
-  return (b >= 0x1p-40f) & (b <= 0x1p40f) & div_in_range(a.x) & div_in_range(a.y) & div_in_range(a.z);
+  const unsigned ok = (unsigned)(b >= 0x1p-40f) & (unsigned)(b <= 0x1p40f) & (unsigned)div_in_range(a.x) &
+                      (unsigned)div_in_range(a.y) & (unsigned)div_in_range(a.z);
+  return ok != 0u;
 }
 __device__ __forceinline__ f3 div3_shared(f3 a, float b) {
   if (__builtin_expect(div3_fast(a, b), 1)) {

@@ -303,6 +303,32 @@ def test_density_fix_exact_scan(pbf, scenes, case):
     assert hacks == o.counters()["hacks"] > 0
 
 
+def test_mover_moved_mid_run(pbf, scenes):
+    """The interactive mover (apply_boundary_conditions, particles.h:510-526,
+    updated from the renderer through the parameters, Simulation/Source/main.cpp:62-63)
+    inside the fluid, moved between steps: the GPU after pbf_set_params equals
+    an oracle that continues from the same state with the new centre."""
+    p, pos, vel = _small_scene(scenes, seed=29)
+    p.mover_center[0], p.mover_center[1], p.mover_center[2] = 0.4, 0.2, 0.25
+    g, o = _pair(pbf, p, pos, vel)
+    g.step(3)
+    o.step(3)
+    _compare_state(g, o, "mover at (0.4, 0.2, 0.25)")
+    p2 = p.copy()
+    p2.mover_center[0] = 0.55
+    g.set_params(p2)
+    s = o.state()
+    o2 = O.Oracle(O.params_from_pbf(p2), mode=O.JACOBI, threads=THREADS)
+    o2.set_state(s["x"], s["xs"], s["v"])
+    g.step(3)
+    o2.step(3)
+    _compare_state(g, o2, "mover moved to (0.55, 0.2, 0.25)")
+    # the sphere is inside the block: particles were pushed out of it
+    x = g.positions()
+    d = np.linalg.norm(x - np.array([0.55, 0.2, 0.25], dtype=np.float32), axis=1)
+    assert (d < 0.2 * 1.01).sum() < (np.linalg.norm(pos - np.array([0.55, 0.2, 0.25]), axis=1) < 0.2).sum()
+
+
 def test_spans_and_remove(pbf, scenes):
     """Sub-spans leave other particles untouched; remove() drops the last particles."""
     p, pos, vel = _small_scene(scenes, seed=17, counts=(8, 8, 8), box=(0.8, 0.8, 0.8))

@@ -118,3 +118,40 @@ def test_graph_replay_with_density_fixes(pbf, scenes):
     assert hacks > 0
     a.close()
     b.close()
+
+
+def test_batched_density_fixes_equal_independent(pbf, scenes):
+    """The near-coincidence fix (particles.h:303-305) inside a batched context:
+    scene 0 has one coincident pair (neighbourhood marking through the
+    scene's own grid), scene 1 doubles every lattice point (more than 4,096
+    moved particles: the exact scan).  Each scene must evolve exactly as in a
+    context of its own."""
+    a = scenes.jitter(scenes.lattice((12, 12, 12)), seed=51)
+    a[10] = a[11]
+    lat = scenes.jitter(scenes.lattice((16, 16, 16)), seed=52)
+    b = np.concatenate([lat, lat]).astype(np.float32)
+    boxes = np.array([[1.2, 1.2, 1.2], [1.6, 1.6, 1.6]], dtype=np.float32)
+    parts = ((a, boxes[0]), (b, boxes[1]))
+    params = scenes.pbf_params((1.6, 1.6, 1.6), a.shape[0] + b.shape[0])
+    g = pbf.Particles(params, boxes=boxes)
+    for k, (pos, _) in enumerate(parts):
+        g.add_to_scene(k, pos, pos, np.zeros_like(pos))
+    g.step(3)
+    x, v = g.positions(), g.velocities()
+    first = 0
+    for k, (pos, box) in enumerate(parts):
+        p = params.copy()
+        p.max_particles_local = pos.shape[0]
+        for i in range(3):
+            p.boundary_max[i] = float(box[i])
+        h = pbf.Particles(p)
+        h.add(pos, pos, np.zeros_like(pos))
+        h.step(3)
+        n = pos.shape[0]
+        print(f"scene {k}: {n} particles, last-step density fixes {h.stats()['density_hacks']}", flush=True)
+        assert h.stats()["density_hacks"] > 0 or k == 0
+        assert np.array_equal(h.positions(), x[first:first + n]), f"scene {k} positions"
+        assert np.array_equal(h.velocities(), v[first:first + n]), f"scene {k} velocities"
+        first += n
+        h.close()
+    g.close()

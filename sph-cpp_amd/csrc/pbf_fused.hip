@@ -42,10 +42,11 @@ __device__ __forceinline__ void load_batch(const int32_t* __restrict__ list, uin
   for (int u = 0; u < kBatch; ++u) q[u] = (k0 + u < cnt) ? (uint32_t)list[list_at(k0 + u, p, stride)] : p;
 }
 
-// The passes that build compact lists run one wavefront per block and stage
-// each lane's entries in its own LDS column (entry k of the 64 lanes
-// contiguous): phase 2 reads them from LDS, and the global copy is written at
-// the end with entry k of the 64 lanes in one contiguous store.
+// The pass that builds the compact lists (the first XSPH pass) runs one
+// wavefront per block and stages each lane's entries in its own LDS column
+// (entry k of the 64 lanes contiguous): its second loop reads them from LDS,
+// and the global copy is written at the end with entry k of the 64 lanes in
+// one contiguous store.
 constexpr int kStageBlock = 64;
 
 __device__ __forceinline__ void load_batch_lds(const int32_t* __restrict__ s_list, uint32_t lane, int k0, int cnt,
@@ -503,7 +504,7 @@ __global__ void k_add_warm(uint32_t n, const float* __restrict__ wlam, float* __
 }
 
 // XSPH sum (particles.h:441-444) over a compact row, from global memory
-// (FROM_LDS = false: cnb column-major) or from the lane's LDS column
+// (FROM_LDS = false: cnb, tiled like the lists) or from the lane's LDS column
 template <bool FROM_LDS>
 __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_t n, uint32_t lane, int nc,
                                        uint32_t p, f3 xp, f3 vp, const KParams& K, const float4* __restrict__ xs,

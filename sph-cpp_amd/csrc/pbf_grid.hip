@@ -16,11 +16,12 @@
 // exactly the (key, original index) order the reference produces.
 //
 // Lists.  One lane per particle walks the 27 cells in the reference's order
-// and stages its entries in LDS, then writes them column-major (entry k of
-// every particle contiguous).  A particle whose candidate count passes a
-// budget is handed to a list of heavy particles that one wavefront each
-// finishes with ballot-compacted appends, so that collapsed cells do not
-// serialise a wavefront on one lane.
+// and stages its entries in LDS, then writes them tiled by groups of 64
+// particles (list_at: entry k of a wavefront's 64 particles contiguous).  A
+// particle with many candidates whose walk passes a budget is handed to a
+// list of heavy particles that one wavefront each finishes with
+// ballot-compacted appends, so that collapsed cells do not serialise a
+// wavefront on one lane.
 #include "pbf_device.h"
 
 namespace pbf {
@@ -441,12 +442,14 @@ __device__ __forceinline__ bool cell_far(const uint32_t* __restrict__ box, uint3
 // fill_neighbors (neighbors.h:197-230).  A particle walks its 27 cells in the
 // reference's order (x offset outer, z inner; cell ids from the FP
 // coordinates x + i*s, neighbors.h:171-185), skips itself and accepts
-// d^2 < s^2 while the list has room.  Lists hold sorted slots, column major.
+// d^2 < s^2 while the list has room.  Lists hold sorted slots (list_at).
 //
-// Two tiers with identical results: one lane per particle when its 27 cells
-// hold at most K.heavy_budget particles; otherwise the particle goes to a
-// list that k_fill_heavy walks with a whole wavefront (64 candidates per
-// step, accepted ones appended in order through a ballot).
+// Two tiers with identical results: one lane per particle, unless its 27
+// cells hold more than twice K.heavy_budget candidates and its walk scans
+// K.heavy_budget of them without filling the list (with the overflow count
+// on: more than K.heavy_budget candidates); such a particle goes to a list
+// that k_fill_heavy walks with a whole wavefront (64 candidates per step,
+// accepted ones appended in order through a ballot).
 
 struct WalkCells {
   float cx[3], cy[3], cz[3];

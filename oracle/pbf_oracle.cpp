@@ -52,8 +52,15 @@
 //   apply vorticity     uniform |omega| field leaves v unchanged; gradient sign
 //   whole step          isolated particles follow the closed-form ballistic
 //                       trajectory; float vs double spread
-//   surface tension     C_Spline zeros (kernels_test.cpp:209-239); gamma = 0
-//                       is a no-op; beyond that "parity unpinned"
+//   C_Spline            zeros (kernels_test.cpp:209-239); Akinci et al. 2013
+//                       eq. 2: C(h/2) = -C(0) = 1/(2 pi h^3), branches meet,
+//                       closed-form root, rise then fall
+//   warm-start features lattice closed forms: velocity divergence of an
+//                       affine field by cubic symmetry, mean y offset and
+//                       count inside and on the top layer
+//   surface tension     gamma = 0 is a no-op; two- and three-particle closed
+//                       forms (Akinci et al. 2013 eq. 1-6); momentum of the
+//                       pairwise forces is conserved
 #include <algorithm>
 #include <cmath>
 #include <cstdint>

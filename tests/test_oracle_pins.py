@@ -181,6 +181,28 @@ def test_poly6_textbook_closed_form():
         assert o.poly6(r) == pytest.approx(315.0 / (64 * math.pi * h ** 9) * (h * h - r * r) ** 3, rel=1e-12)
 
 
+def test_cohesion_spline_closed_form_properties():
+    """Akinci et al. 2013 eq. 2 (cohesion spline): C = 32/(pi h^9) (h - r)^3 r^3 for h/2 < r <= h and
+    32/(pi h^9) (2 (h - r)^3 r^3 - h^6/64) for r <= h/2.  What the definition fixes:
+    C(h/2) = 1/(2 pi h^3) = -C(0) (a wrong norm power or a dropped or sign-flipped h^6/64 term fails);
+    the two branches meet at h/2 (a wrong factor 2 or swapped branches fail); the one root below h/2 is
+    at r/h = (1 - sqrt(1 - 2^(-1/3)))/2, where (h - r) r = 2^(-7/3) h^2; C rises on [0, h/2] and
+    falls on [h/2, h]."""
+    for h in (f32(0.1), f32(0.05)):
+        o = O.Oracle(mk(h=h), double=True)
+        top = 1.0 / (2.0 * math.pi * h ** 3)
+        assert o.cspline(0.5 * h) == pytest.approx(top, rel=1e-12)
+        assert o.cspline(0.0) == pytest.approx(-top, rel=1e-12)
+        assert o.cspline(0.5 * h * (1 + 1e-9)) == pytest.approx(top, rel=1e-7)
+        root = 0.5 * (1.0 - math.sqrt(1.0 - 2.0 ** (-1.0 / 3.0))) * h
+        assert o.cspline(root * (1 - 1e-6)) < 0.0 < o.cspline(root * (1 + 1e-6))
+        assert abs(o.cspline(root)) <= 1e-9 * top
+        rs = np.linspace(0.0, h, 401)
+        c = np.array([o.cspline(r) for r in rs])
+        assert np.all(np.diff(c[:201]) > 0) and np.all(np.diff(c[200:]) < 0)
+        assert c[-1] == pytest.approx(0.0, abs=1e-12 * top)
+
+
 def test_spiky_gradient_is_gradient_of_spiky_kernel():
     """Del_Spikey(p, q) = grad_p W_spiky(|p - q|), W_spiky = 15/(pi h^6) (h - r)^3
     (Mueller et al. 2003); central differences in double.  Catches sign and norm errors."""
@@ -451,6 +473,44 @@ def test_apply_vorticity_direction():
     np.testing.assert_allclose(vc, expect, rtol=1e-6, atol=1e-4 * np.linalg.norm(expect))
 
 
+def test_warm_features_on_a_lattice_closed_forms():
+    """Features 4-7 of the learned warm start on a cubic lattice (spacing s, h = 2.2 s) in an affine
+    velocity field v = A x.  With gradW(x_p - x_q) = -(45/(pi h^6)) (h - r)^2 (x_p - x_q)/r and
+    v_q - v_p = A d (d = x_q - x_p), cubic symmetry (sum f(r) d d^T = (1/3) sum f(r) r^2 I) gives
+    feature 7 = m dt sum (v_q - v_p) . grad_q C = -(45/(pi h^6 rho0)) m dt (tr A / 3) sum (h - r)^2 r
+    for an interior particle (an expanding field lowers the density; the rotation part of A adds
+    nothing).  Feature 6 = mean (y_q - y_p)/h is 0 inside and, on the top layer, s/h times the mean j
+    of the integer offsets (i, j, k), j <= 0, i^2 + j^2 + k^2 <= 4; feature 4 = count/32 (32 inside).
+    Catches a reversed velocity difference, a dropped rho0, m or dt, the sign of the offset and a
+    count over the whole list.  The solve predicts x* = x + v dt, so particles start at lattice - v dt."""
+    s, h, rho0, m, dt = 0.05, f32(0.11), 1000.0, f32(0.125), f32(1 / 120)
+    nside = 9
+    lat = lattice_points(nside, nside, nside, s, origin=(0.3, 0.3, 0.3)).astype(np.float64)
+    A = np.array([[0.9, 0.4, -0.2], [-0.4, 1.3, 0.5], [0.2, -0.5, 0.8]])
+    centre = np.array([0.5, 0.5, 0.5])
+    vel = (lat - centre) @ A.T
+    x0 = lat - vel * dt
+    o = O.Oracle(mk(h=h, g=0.0, dt=dt, rho0=rho0, mass=m, box=(1, 1, 1), cap=200), double=True)
+    o.set_state(x0, x0, vel)
+    n = lat.shape[0]
+    feat = o.solve(0.0, 1, warm=np.zeros(n), features=True)[4]
+
+    off = np.array([(i, j, k) for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)
+                    if 0 < i * i + j * j + k * k <= 4], dtype=np.float64)
+    assert off.shape[0] == 32
+    r = np.linalg.norm(off, axis=1) * s
+    inner = (4 * nside + 4) * nside + 4          # lattice index (4, 4, 4)
+    top = (4 * nside + (nside - 1)) * nside + 4  # (4, 8, 4)
+    f7 = -45.0 / (math.pi * h ** 6 * rho0) * m * dt * (np.trace(A) / 3.0) * ((h - r) ** 2 * r).sum()
+    assert feat[inner, 7] < 0
+    assert feat[inner, 7] == pytest.approx(f7, rel=1e-5)
+    assert feat[inner, 4] == 1.0 and abs(feat[inner, 6]) <= 1e-6
+    below = off[off[:, 1] <= 0]
+    assert feat[top, 4] == pytest.approx(below.shape[0] / 32.0, rel=1e-7)
+    assert feat[top, 6] == pytest.approx(s * below[:, 1].mean() / h, rel=1e-5)
+    assert feat[inner, 5] == pytest.approx(vel[inner, 1] * dt / h, rel=1e-5, abs=1e-7)
+
+
 def test_isolated_particles_follow_ballistic_trajectory():
     """Whole step with no neighbours: symplectic Euler free fall, y_n = y0 + g dt^2 n(n+1)/2."""
     dt, g = f32(1 / 120), f32(-9.8)
@@ -462,6 +522,90 @@ def test_isolated_particles_follow_ballistic_trajectory():
     st = o.state()
     np.testing.assert_allclose(st["x"][:, 1], pts[:, 1] + g * dt * dt * n * (n + 1) / 2, rtol=1e-12)
     np.testing.assert_allclose(st["v"][:, 1], g * dt * n, rtol=1e-9)
+
+
+def _tension_pair(o, d, rho, vel):
+    pts = np.array([[0.5, 0.5, 0.5], [0.5 + d, 0.5, 0.5]])
+    o.set_state(pts, pts, vel)
+    o.set_aux(rho=rho)
+    o.pass_(O.FIND)
+    o.pass_(O.TENSION)
+    return pts, o.state()["v"]
+
+
+def test_surface_tension_pair_closed_form():
+    """Akinci et al. 2013 eq. 1-6 for two particles: normals n_i = h gradW(x_i - x_j) / rho_j, force
+    on i K_ij (-gamma C(r) (x_i - x_j)/r - gamma (n_i - n_j)) with K_ij = 2 rho0 / (rho_i + rho_j), and
+    dv_i = F_i / rho_i dt (particles.h:400-431).  The three separations put r in the repulsive and
+    attractive parts of the inner spline branch and in the outer branch; the densities differ so that a
+    swapped rho_i / rho_j fails.  The spline is written out here from the paper, the spiky gradient is the
+    one pinned above."""
+    h, gamma, rho0, dt = f32(0.1), f32(0.7), 1000.0, f32(1 / 120)
+    rho = np.array([950.0, 1040.0])
+    vel = np.array([[0.1, 0.0, 0.0], [0.0, -0.2, 0.0]])
+
+    def spline(r):
+        c = 32.0 / (math.pi * h ** 9)
+        if r <= 0.5 * h:
+            return c * (2.0 * (h - r) ** 3 * r ** 3 - h ** 6 / 64.0)
+        return c * (h - r) ** 3 * r ** 3
+
+    for d in (0.02, 0.04, 0.07):
+        o = O.Oracle(mk(h=h, gamma=gamma, rho0=rho0, dt=dt, box=(1, 1, 1)), double=True)
+        pts, v = _tension_pair(o, d, rho, vel)
+        n = [h * o.spiky(pts[0], pts[1]) / rho[1], h * o.spiky(pts[1], pts[0]) / rho[0]]
+        K = 2.0 * rho0 / (rho[0] + rho[1])
+        for i, j in ((0, 1), (1, 0)):
+            rij = pts[i] - pts[j]
+            r = float(np.linalg.norm(rij))
+            F = K * (-gamma * spline(r) * rij / r - gamma * (n[i] - n[j]))
+            np.testing.assert_allclose(v[i], vel[i] + F / rho[i] * dt, rtol=1e-12, atol=1e-15)
+        # the cohesion part alone pulls particle 0 towards particle 1 (+x) above the spline's root
+        # (0.273 h) and pushes it away below
+        coh0 = -gamma * spline(d) * (pts[0] - pts[1]) / d
+        assert (coh0[0] > 0) == (d > 0.273 * h)
+
+    # three particles with three densities: a pair cannot tell n_i = h sum gradW / rho_j from a normal
+    # divided by rho_i (n_i - n_j is the same), a bent chain can
+    pts = np.array([[0.5, 0.5, 0.5], [0.55, 0.52, 0.5], [0.58, 0.47, 0.53]])
+    rho3 = np.array([950.0, 1040.0, 1010.0])
+    vel3 = np.zeros((3, 3))
+    o = O.Oracle(mk(h=h, gamma=gamma, rho0=rho0, dt=dt, box=(1, 1, 1)), double=True)
+    o.set_state(pts, pts, vel3)
+    o.set_aux(rho=rho3)
+    o.pass_(O.FIND)
+    o.pass_(O.TENSION)
+    v = o.state()["v"]
+    dist = np.linalg.norm(pts[:, None] - pts[None], axis=2)
+    nb = [[j for j in range(3) if j != i and dist[i, j] <= h] for i in range(3)]
+    assert all(len(x) == 2 for x in nb)
+    n = [h * sum(o.spiky(pts[i], pts[j]) / rho3[j] for j in nb[i]) for i in range(3)]
+    for i in range(3):
+        F = np.zeros(3)
+        for j in nb[i]:
+            rij = pts[i] - pts[j]
+            K = 2.0 * rho0 / (rho3[i] + rho3[j])
+            F += K * (-gamma * spline(dist[i, j]) * rij / dist[i, j] - gamma * (n[i] - n[j]))
+        np.testing.assert_allclose(v[i], F / rho3[i] * dt, rtol=1e-12, atol=1e-15)
+
+
+def test_surface_tension_conserves_momentum():
+    """Each pair's force is antisymmetric (K_ij symmetric, n_i - n_j and the cohesion term odd), so over
+    symmetric lists sum_i rho_i dv_i = 0 with any densities.  Catches n_i + n_j, K built from one
+    particle's density, and dividing the force by rho_j."""
+    pts = _compressed_cluster(seed=7)
+    n = pts.shape[0]
+    rng = np.random.default_rng(3)
+    rho = rng.uniform(900.0, 1100.0, n)
+    o = O.Oracle(mk(h=f32(0.1), gamma=f32(2.0), box=(1, 1, 1), cap=400), double=True)
+    o.set_state(pts, pts, np.zeros((n, 3)))
+    o.set_aux(rho=rho)
+    o.pass_(O.FIND)
+    assert int(o.counters()["overflow"]) == 0
+    o.pass_(O.TENSION)
+    mom = rho[:, None] * o.state()["v"]
+    assert np.abs(mom).max() > 0
+    assert np.abs(mom.sum(0)).max() <= 1e-12 * np.abs(mom).sum()
 
 
 def test_surface_tension_gamma_zero_is_noop_and_float_double_spread():

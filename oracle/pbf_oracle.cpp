@@ -127,6 +127,12 @@ struct Sim {
   R h, s, rho0, mass, g, gamma, eps, visc, dt, max_speed, vort;
   V3<R> bmin, bmax, mover;
   int iters, cap, mode, threads;
+  // stop_at_cap: a walk ends when its list is full.  Lists and counts are the
+  // ones of the full walk (the reference's walk goes on past a full list to no
+  // effect, neighbors.h:213-225; the full walk here counts what did not fit),
+  // and the overflow counter reads 0.  A checker shortcut for states whose
+  // crowded cells make the full scan slow.
+  int stop_at_cap = 0;
   // kernel constants (kernels.h:62, :160-161, :218)
   R poly6_norm, spiky_norm, cspline_norm, r_eps;
   // neighbour grid (neighbors.h:67-71)
@@ -300,7 +306,9 @@ struct Sim {
           } else if (d2 < valid) {
             ++o;
           }
+          if (stop_at_cap && count == cap) break;
         }
+        if (stop_at_cap && count == cap) break;
       }
       counts[p] = count;
       return o;
@@ -769,6 +777,11 @@ void orc_set_threads(void* h, int threads) {
 void orc_set_mode(void* h, int mode) {
   Handle* H = (Handle*)h;
   H->f.mode = H->d.mode = mode;
+}
+
+void orc_set_stop_at_cap(void* h, int on) {
+  Handle* H = (Handle*)h;
+  H->f.stop_at_cap = H->d.stop_at_cap = on != 0;
 }
 
 int orc_set_state(void* h, size_t n, const float* x, const float* xs, const float* v) {

@@ -48,6 +48,7 @@ def lib():
         L.orc_destroy.argtypes = [P]
         L.orc_set_threads.argtypes = [P, C.c_int]
         L.orc_set_mode.argtypes = [P, C.c_int]
+        L.orc_set_stop_at_cap.argtypes = [P, C.c_int]
         L.orc_set_state.argtypes = [P, C.c_size_t, F, F, F]
         L.orc_set_state_f64.argtypes = [P, C.c_size_t, D, D, D]
         L.orc_get_state.argtypes = [P, D, D, D, D, D, D]
@@ -126,6 +127,10 @@ class Oracle:
 
     def set_mode(self, m):
         self.L.orc_set_mode(self.h, int(m))
+
+    def set_stop_at_cap(self, on=True):
+        """End each list walk once the list is full (same lists and counts, overflow not counted)."""
+        self.L.orc_set_stop_at_cap(self.h, int(bool(on)))
 
     def set_state(self, x, xs=None, v=None):
         x = np.ascontiguousarray(x).reshape(-1, 3)

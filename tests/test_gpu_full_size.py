@@ -1,0 +1,106 @@
+"""GPU parity at the full benchmark sizes, through the C ABI.
+
+Two workloads against the CPU oracle (Jacobi mode, the device's semantics):
+the 1,048,576-particle dam break (128^3 lattice, 200 substeps) and the
+8,388,608-particle double dam break (two 128x256x128 blocks, cap 64 with the
+overflow counted, 100 substeps).
+
+* The first substeps from the initial lattice: every field bit for bit.
+* The state the whole run reaches, where the cap-60 collapse has piled
+  particles into cells of hundreds of thousands: the neighbour lists the
+  device builds for that state (the wavefront builder's far skips, the lane
+  walk's adaptive budget) bit for bit, and for the dam break one more substep
+  from it, which applies the near-coincidence fix tens of thousands of times.
+
+For the collapsed states the oracle ends each list walk once its list is full
+(Oracle.set_stop_at_cap): the same lists and counts as the full walk, which on
+those cells takes minutes per pass.  The overflow count is compared where the
+oracle walks in full (the first substep of the double dam break).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from test_gpu_parity import _compare_state
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+ROWS = 1 << 20  # rows of the lists compared at a time (bounds the host copies)
+
+
+def _oracle(params, x, xs, v, stop_at_cap=False):
+    o = O.Oracle(O.params_from_pbf(params), mode=O.JACOBI, threads=THREADS)
+    o.set_stop_at_cap(stop_at_cap)
+    o.set_state(x, xs, v)
+    return o
+
+
+def _assert_lists_equal(g, o, cap, what):
+    nb = o.neighbors()
+    b, e = g.bin_bounds()
+    assert np.array_equal(g.particle_ids(), nb["ids"]), what + ": particle order"
+    assert np.array_equal(b, nb["begin"]) and np.array_equal(e, nb["end"]), what + ": cell bounds"
+    lists, counts = g.neighbor_lists()
+    assert np.array_equal(counts, nb["counts"]), what + ": counts"
+    cols = np.arange(cap)[None, :]
+    for r0 in range(0, counts.shape[0], ROWS):
+        r1 = min(counts.shape[0], r0 + ROWS)
+        mask = cols < counts[r0:r1, None]
+        same = np.array_equal(np.where(mask, lists[r0:r1], 0), np.where(mask, nb["lists"][r0:r1], 0))
+        assert same, f"{what}: lists of particles [{r0}, {r1})"
+    return (e - b).max()
+
+
+def test_dam_break_1m_first_substeps_bit_exact(pbf, scenes):
+    p, pos, vel = scenes.config_c2(seed=0)
+    g = pbf.Particles(p)
+    g.add(pos, pos, vel)
+    o = _oracle(p, pos, pos, vel)
+    g.step(2)
+    o.step(2)
+    _compare_state(g, o, "1M dam break, substep 2")
+
+
+def test_dam_break_1m_collapsed_state_lists_and_next_substep(pbf, scenes):
+    p, pos, vel = scenes.config_c2(seed=0)
+    g = pbf.Particles(p)
+    g.add(pos, pos, vel)
+    g.step(200)
+    x, xs, v = g.positions(), g.position_stars(), g.velocities()
+    g.find_neighbors()
+    o = _oracle(p, x, xs, v, stop_at_cap=True)
+    o.pass_(O.FIND)
+    biggest = _assert_lists_equal(g, o, p.max_neighbors, "1M dam break after 200 substeps")
+    assert biggest > 100_000, f"largest cell holds {biggest} particles: the run did not collapse"
+    o.set_state(x, x, v)
+    g.step(1)
+    o.step(1)
+    assert o.counters()["hacks"] > 10_000
+    _compare_state(g, o, "1M dam break, substep 201")
+
+
+def test_double_dam_break_8m_first_substep_bit_exact(pbf, scenes):
+    p, pos, vel = scenes.double_dam_break(seed=0, count_overflow=True)
+    assert pos.shape[0] == 8_388_608 and p.max_neighbors == 64
+    g = pbf.Particles(p)
+    g.add(pos, pos, vel)
+    o = _oracle(p, pos, pos, vel)
+    g.step(1)
+    o.step(1)
+    _compare_state(g, o, "8M double dam break, substep 1")
+    assert g.stats()["neighbor_overflow"] == o.counters()["overflow"]
+
+
+def test_double_dam_break_8m_full_run_lists_bit_exact(pbf, scenes):
+    p, pos, vel = scenes.double_dam_break(seed=0, count_overflow=True)
+    g = pbf.Particles(p)
+    g.add(pos, pos, vel)
+    g.step(100)
+    x, xs, v = g.positions(), g.position_stars(), g.velocities()
+    g.find_neighbors()
+    o = _oracle(p, x, xs, v, stop_at_cap=True)
+    o.pass_(O.FIND)
+    _assert_lists_equal(g, o, p.max_neighbors, "8M double dam break after 100 substeps")

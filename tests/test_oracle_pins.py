@@ -142,6 +142,26 @@ def test_neighbor_lists_brute_force_and_order():
     assert o2.counters()["overflow"] == int(np.maximum(full["counts"] - 20, 0).sum())
 
 
+def test_stop_at_cap_keeps_lists_and_steps():
+    """The checker shortcut that ends a walk once its list is full (for the full-size GPU tests'
+    collapsed states) keeps every list, count and step result; only the overflow count is lost."""
+    pts = _scene(seed=2)
+    res = []
+    for stop in (False, True):
+        o = O.Oracle(mk(box=(1.0, 0.8, 0.8), cap=20))
+        o.set_stop_at_cap(stop)
+        o.set_state(pts)
+        o.pass_(O.FIND)
+        res.append((o.neighbors(), o.counters()["overflow"]))
+        o.step(3)
+        res[-1] += (o.state(),)
+    assert res[0][1] > 0 and res[1][1] == 0
+    for k in ("counts", "lists", "ids", "begin", "end"):
+        assert np.array_equal(res[0][0][k], res[1][0][k]), k
+    for k in ("x", "v", "rho", "lam"):
+        assert np.array_equal(res[0][2][k], res[1][2][k]), k
+
+
 # ---------------------------------------------------------------- kernels
 def test_kernel_zeros_known_answers():
     """kernels_test.cpp: Poly6 and C_Spline are 0 at r = h and beyond; Del_Spikey 0 beyond h."""

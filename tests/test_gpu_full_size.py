@@ -1,9 +1,10 @@
 """GPU parity at the full benchmark sizes, through the C ABI.
 
-Two workloads against the CPU oracle (Jacobi mode, the device's semantics):
-the 1,048,576-particle dam break (128^3 lattice, 200 substeps) and the
+Three workloads against the CPU oracle (Jacobi mode, the device's semantics):
+the 1,048,576-particle dam break (128^3 lattice, 200 substeps), the
 8,388,608-particle double dam break (two 128x256x128 blocks, cap 64 with the
-overflow counted, 100 substeps).
+overflow counted, 100 substeps) and the batch of 256 dam breaks of
+4,096..65,536 particles each, stepped by graph replay.
 
 * The first substeps from the initial lattice: every field bit for bit.
 * The state the whole run reaches, where the cap-60 collapse has piled
@@ -92,6 +93,37 @@ def test_double_dam_break_8m_first_substep_bit_exact(pbf, scenes):
     o.step(1)
     _compare_state(g, o, "8M double dam break, substep 1")
     assert g.stats()["neighbor_overflow"] == o.counters()["overflow"]
+
+
+def test_batched_256_scenes_graph_replay_bit_exact(pbf, scenes):
+    """The whole 256-scene batch (4,096..65,536 particles per scene, log-uniform, about 5.7M in all),
+    autotuned and stepped by graph replay as the benchmark does; six scenes (the smallest, the
+    largest and four more) against the oracle run on their own, bit for bit."""
+    params, boxes, sc = scenes.batched_dam_breaks(num_scenes=256, seed=0)
+    g = pbf.Particles(params, boxes=boxes)
+    for k, (pos, vel) in enumerate(sc):
+        g.add_to_scene(k, pos, pos, vel)
+    g.autotune(steps=2)  # restores the state it started from
+    steps = 6
+    g.step_graph(steps)
+    x, v = g.positions(), g.velocities()
+    sizes = np.array([pos.shape[0] for pos, _ in sc])
+    first = np.concatenate([[0], np.cumsum(sizes)])
+    assert first[-1] == x.shape[0] > 5_000_000
+    picks = sorted({int(sizes.argmin()), int(sizes.argmax()), *np.random.default_rng(1).choice(256, 4).tolist()})
+    for k in picks:
+        pos, vel = sc[k]
+        p = params.copy()
+        p.max_particles_local = pos.shape[0]
+        for i in range(3):
+            p.boundary_max[i] = float(boxes[k, i])
+        o = _oracle(p, pos, pos, vel)
+        o.step(steps)
+        st = o.state()
+        a, b = first[k], first[k + 1]
+        assert np.array_equal(x[a:b], st["x"].astype(np.float32)), f"scene {k} ({b - a} particles) positions"
+        assert np.array_equal(v[a:b], st["v"].astype(np.float32)), f"scene {k} ({b - a} particles) velocities"
+    g.close()
 
 
 def test_double_dam_break_8m_full_run_lists_bit_exact(pbf, scenes):

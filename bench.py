@@ -299,6 +299,39 @@ def batched_run(pbf, scenes, steps, seed, num_scenes=256):
     return out
 
 
+def host_buffer_run(pbf, params, pos, vel, steps, tuning=None):
+    """The substep with host buffers crossing PCIe, through the C ABI's plain
+    pointers (pageable numpy arrays): "round_trip" uploads x and v, steps once
+    and downloads x and v every substep (a caller that keeps the particles in
+    host memory); "positions_per_substep" only downloads positions after each
+    substep (the reference's renderer sync with frames_per_update = 1,
+    Simulation/Source/main.cpp:121-123)."""
+    n = pos.shape[0]
+    out = {"steps": steps}
+    for name in ("round_trip", "positions_per_substep"):
+        g = pbf.Particles(params)
+        g.add(pos, pos, vel)
+        if tuning:
+            g.set_tuning(tuning["block_size"], tuning["heavy_budget"], tuning["heavy_blocks"])
+        x, v = pos.copy(), vel.copy()
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if name == "round_trip":
+                g.set_positions(x)
+                g.set_velocities(v)
+            g.step(1)
+            x = g.positions()
+            if name == "round_trip":
+                v = g.velocities()
+        g.sync()
+        wall = time.perf_counter() - t0
+        out[name] = {"value": n * steps / wall, "ms_per_step": wall * 1e3 / steps}
+        g.close()
+        print(f"host buffers: {name} done", file=sys.stderr, flush=True)
+    return out
+
+
 def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=128):
     """Plain vs warm-started iterate-to-tolerance solves on the same scene:
     each mode runs its own trajectory of `steps` substeps, every substep
@@ -470,6 +503,7 @@ def main(argv=None):
     if args.variants:
         out["variants"]["batched_256_scenes"] = batched_run(pbf, scenes, 100, args.seed)
         out["variants"]["double_dam_break_8M"] = double_dam_break_run(pbf, scenes, 100, args.seed)
+        out["variants"]["host_buffers"] = host_buffer_run(pbf, params, pos, vel, args.steps, tuning=tuning)
     if args.warm:
         out["warm_start"] = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps)
     if not args.no_cpu:

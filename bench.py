@@ -332,13 +332,13 @@ def host_buffer_run(pbf, params, pos, vel, steps, tuning=None):
     return out
 
 
-def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=128):
+def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=128, model_path=WARM_MODEL):
     """Plain vs warm-started iterate-to-tolerance solves on the same scene:
     each mode runs its own trajectory of `steps` substeps, every substep
     iterating until the mean density error is <= tol (at most max_slots)."""
-    if not os.path.exists(WARM_MODEL):
+    if not os.path.exists(model_path):
         return {"skipped": "no trained warm-start model (sph-cpp_amd/train_warm.py)"}
-    weights = np.load(WARM_MODEL, allow_pickle=False)
+    weights = np.load(model_path, allow_pickle=False)
     model = pbf.WarmModel(16)
     model.set_params(weights)
     out = {"criterion": f"mean density error <= {tol} per substep (at most {max_slots} Jacobi slots)",
@@ -393,6 +393,8 @@ def main(argv=None):
     ap.add_argument("--no-warm", dest="warm", action="store_false",
                     help="skip the plain vs warm-start comparison at a density-error tolerance")
     ap.add_argument("--warm-steps", type=int, default=200)
+    ap.add_argument("--warm-model", default=WARM_MODEL, help="network weights for the learned warm start")
+    ap.add_argument("--warm-only", action="store_true", help="only the warm-start comparison")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch settings instead of running pbf_autotune first")
     ap.add_argument("--summarize", default="", metavar="DIR",
@@ -422,6 +424,10 @@ def main(argv=None):
     counts = tuple(int(c) for c in args.lattice.lower().split("x"))
     params, pos, vel = scenes.config_c2(seed=args.seed, lattice_counts=counts, cap=args.cap)
     n = pos.shape[0]
+    if args.warm_only:
+        ws = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps, model_path=args.warm_model)
+        print(json.dumps({"warm_start": ws, "model": args.warm_model}), flush=True)
+        return 0
 
     # warm-up on a throwaway context (module load, first launches); the
     # autotuner then picks the launch settings halfway through the run
@@ -505,7 +511,8 @@ def main(argv=None):
         out["variants"]["double_dam_break_8M"] = double_dam_break_run(pbf, scenes, 100, args.seed)
         out["variants"]["host_buffers"] = host_buffer_run(pbf, params, pos, vel, args.steps, tuning=tuning)
     if args.warm:
-        out["warm_start"] = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps)
+        out["warm_start"] = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps,
+                                          model_path=args.warm_model)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(params, pos, vel)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]

@@ -109,10 +109,10 @@ def output_gain(model, first, count):
     return float(min(max(g, 0.0), 1.0)), losses
 
 
-def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print):
+def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print, scale=None, scenes_min=8):
     model = pbf.WarmModel(pairs + 2 * VALIDATION_PAIRS)
     t0 = time.time()
-    generate(model, pairs, seed, log=log)
+    generate(model, pairs, seed, log=log, scale=scale, scenes_min=scenes_min)
     t1 = time.time()
     model.init(seed)
     n = model.sample_count()
@@ -147,9 +147,13 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--scale", type=int, default=0, choices=(0, 1, 2, 4),
+                    help="box scale of the training scenes (0: 1, 2 and 4 at random)")
+    ap.add_argument("--scenes", type=int, default=8, help="training scenes (each gives pairs / scenes samples)")
     ap.add_argument("--out", default=MODEL_PATH)
     a = ap.parse_args(argv)
-    model, summary = train(a.pairs, a.epochs, a.batch, a.lr, a.seed, log=lambda s: print(s, flush=True))
+    model, summary = train(a.pairs, a.epochs, a.batch, a.lr, a.seed, log=lambda s: print(s, flush=True),
+                           scale=a.scale or None, scenes_min=a.scenes)
     np.save(a.out, model.params())
     print(json.dumps({"out": a.out, **summary}), flush=True)
     model.close()

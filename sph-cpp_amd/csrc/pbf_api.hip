@@ -40,11 +40,13 @@ constexpr size_t kMaxPendingEvents = 64;
 static void fold_oldest(KernelTimer& t, size_t count) {
   for (size_t i = 0; i < count && i < t.pending.size(); ++i) {
     auto& pr = t.pending[i];
-    hipEventSynchronize(pr.second);
     float ms = 0.f;
-    hipEventElapsedTime(&ms, pr.first, pr.second);
-    t.total_ms += ms;
-    t.launches++;
+    // a pair whose time cannot be read is left out of the totals (the launch
+    // count then shows the gap)
+    if (hipEventSynchronize(pr.second) == hipSuccess && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+      t.total_ms += ms;
+      t.launches++;
+    }
     t.spare.push_back(pr.first);
     t.spare.push_back(pr.second);
   }
@@ -56,17 +58,22 @@ KernelTimer* timer_begin(pbf_context* c, const char* name) {
   for (auto& t : c->timers) {
     if (t.name == name) {
       if (t.pending.size() >= kMaxPendingEvents) fold_oldest(t, kMaxPendingEvents / 2);
-      hipEvent_t ev[2];
+      hipEvent_t ev[2] = {nullptr, nullptr};
       for (hipEvent_t& e : ev) {
         if (t.spare.empty()) {
-          hipEventCreate(&e);
+          if (hipEventCreate(&e) != hipSuccess) e = nullptr;
         } else {
           e = t.spare.back();
           t.spare.pop_back();
         }
       }
       const hipEvent_t a = ev[0], b = ev[1];
-      hipEventRecord(a, c->stream);
+      if (!a || !b || hipEventRecord(a, c->stream) != hipSuccess) {
+        // no events for this launch: it runs untimed
+        for (hipEvent_t e : ev)
+          if (e) t.spare.push_back(e);
+        return nullptr;
+      }
       t.pending.push_back({a, b});
       return &t;
     }

@@ -31,6 +31,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s @1M, <=1% density err: plain vs learned warm-start"
 HBM_PEAK_GBS = 8000.0
+# non-packed FP32 vector issue of the chip: 256 CUs x 4 SIMDs, one wave64
+# instruction per SIMD every 4 cycles at 2.4 GHz = 3.93e13 lane-operations/s
+VALU_SIMDS = 1024
+VALU_CLOCK_GHZ = 2.4
+VALU_LANE_OPS_PEAK = VALU_SIMDS * 16 * VALU_CLOCK_GHZ * 1e9
+# algorithmic FP32 operations per particle-substep at the rest lattice: 9 per
+# candidate (K = 373 candidates in the 27 cells), 324 per particle, and per
+# neighbour within the list (56) 90 per Jacobi iteration plus 125 for the
+# velocity passes
+F_ALG_K, F_ALG_NBR = 373, 56
 
 # Algorithmic bytes per particle for each kernel launch: the compulsory
 # per-particle streams of that pass (positions and velocities as 12-byte
@@ -150,6 +160,8 @@ PROFILE_COMMANDS = {
     "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/trace -o c2 -- " + PROFILED,
     "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o c2 -- " + PROFILED,
     "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o c2 -- " + PROFILED,
+    "sq": "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES "
+          "--output-format csv -d gpurun_out/pmc_sq -o c2 -- " + PROFILED,
     "cal_fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/cal_fetch -o cp -- "
                  "python3 bench.py --calibrate-copy 512",
     "cal_write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/cal_write -o cp -- "
@@ -243,6 +255,37 @@ def summarize(d: str, workload: str) -> None:
     with open(os.path.join(prof, "pmc_traffic.json"), "w") as f:
         json.dump({"workload": workload, "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
                    **traffic}, f, indent=1)
+
+    sq_path = os.path.join(d, "pmc_sq", "c2_counter_collection.csv")
+    if os.path.exists(sq_path):
+        sq = {c: _pmc(sq_path, c) for c in ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS")}
+        lines = [f"# VALU issue per launch: {workload}", "",
+                 "One counter run of the benchmark command (six SQ counters):", "",
+                 "    " + PROFILE_COMMANDS["sq"], "",
+                 "Means over the timed run's launches.  `VALU issue` = wave64 VALU instructions per SIMD per",
+                 f"cycle: VALU instructions / ({VALU_SIMDS} SIMDs x the kernel's mean duration from the kernel trace",
+                 f"x {VALU_CLOCK_GHZ} GHz).  A back-to-back stream of wave64 instructions that each take 4 cycles on a",
+                 "16-lane SIMD issues 0.25; the density, pressure-update and wavefront list-builder kernels",
+                 "measure 0.26-0.33, so gfx950 runs part of their FP32 instructions in fewer than 4 cycles and",
+                 "those kernels are bound by VALU issue.  The lane list builder and the first XSPH pass, at",
+                 "0.10-0.13, wait on gathers.", "",
+                 "| kernel | launches | waves | VALU instr / wave | vector loads / wave | LDS instr / wave | "
+                 "mean us | VALU issue |",
+                 "|---|---:|---:|---:|---:|---:|---:|---:|"]
+        for k in sorted(per, key=lambda k: -sum(per[k])):
+            if k not in sq["SQ_INSTS_VALU"] or not sq["SQ_WAVES"].get(k):
+                continue
+            waves = statistics.mean(sq["SQ_WAVES"][k])
+            valu = statistics.mean(sq["SQ_INSTS_VALU"][k])
+            vmem = statistics.mean(sq["SQ_INSTS_VMEM_RD"].get(k, [0.0]))
+            lds = statistics.mean(sq["SQ_INSTS_LDS"].get(k, [0.0]))
+            us = statistics.mean(per[k])
+            rate = valu / (VALU_SIMDS * us * 1e-6 * VALU_CLOCK_GHZ * 1e9) if us > 0 else 0.0
+            if waves <= 0:
+                continue
+            lines.append(f"| `{k}` | {len(sq['SQ_INSTS_VALU'][k])} | {waves:.0f} | {valu / waves:.0f} | "
+                         f"{vmem / waves:.1f} | {lds / waves:.1f} | {us:.1f} | {rate:.3f} |")
+        open(os.path.join(prof, f"{workload}_valu.md"), "w").write("\n".join(lines) + "\n")
 
 
 WARM_MODEL = os.path.join(ROOT, "sph-cpp_amd", "warm_model.npy")
@@ -465,9 +508,13 @@ def main(argv=None):
     # whole-substep bookkeeping: B_alg = 260 + 44 I + 8 C/N bytes per particle-substep
     I = params.solve_step_count
     b_alg = 260 + 44 * I + 8.0 * res["cells"] / n
+    f_alg = 9 * F_ALG_K + 324 + F_ALG_NBR * (90 * I + 125)
     step_roof = {"alg_bytes_per_particle_substep": round(b_alg, 2),
                  "achieved_GBs": round(b_alg * value / 1e9, 2),
-                 "frac_of_hbm_peak": round(b_alg * value / 1e9 / HBM_PEAK_GBS, 5)}
+                 "frac_of_hbm_peak": round(b_alg * value / 1e9 / HBM_PEAK_GBS, 5),
+                 "alg_flops_per_particle_substep": f_alg,
+                 "achieved_Tlane_ops": round(f_alg * value / 1e12, 2),
+                 "frac_of_fp32_lane_peak": round(f_alg * value / VALU_LANE_OPS_PEAK, 4)}
 
     out = {
         "metric": METRIC,

@@ -106,6 +106,16 @@ def _profile_traffic(kernel: str):
         return None
 
 
+def _profile_valu(kernel: str):
+    """VALU issue of the kernel from the committed SQ counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "pmc_valu.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
 KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "cell_box", "fill_neighbors", "fill_heavy",
            "density_lambda",
            "mark_dirty", "lambda_repair", "dp_update", "viscosity", "vorticity", "apply_vorticity",
@@ -272,6 +282,7 @@ def summarize(d: str, workload: str) -> None:
                  "| kernel | launches | waves | VALU instr / wave | vector loads / wave | LDS instr / wave | "
                  "mean us | VALU issue |",
                  "|---|---:|---:|---:|---:|---:|---:|---:|"]
+        issue = {}
         for k in sorted(per, key=lambda k: -sum(per[k])):
             if k not in sq["SQ_INSTS_VALU"] or not sq["SQ_WAVES"].get(k):
                 continue
@@ -285,7 +296,12 @@ def summarize(d: str, workload: str) -> None:
                 continue
             lines.append(f"| `{k}` | {len(sq['SQ_INSTS_VALU'][k])} | {waves:.0f} | {valu / waves:.0f} | "
                          f"{vmem / waves:.1f} | {lds / waves:.1f} | {us:.1f} | {rate:.3f} |")
+            issue[k] = {"valu_issue_per_simd_cycle": round(rate, 4), "valu_instr_per_wave": round(valu / waves),
+                        "vector_loads_per_wave": round(vmem / waves, 1)}
         open(os.path.join(prof, f"{workload}_valu.md"), "w").write("\n".join(lines) + "\n")
+        with open(os.path.join(prof, "pmc_valu.json"), "w") as f:
+            json.dump({"workload": workload, "note": "wave64 VALU instructions per SIMD per cycle; 0.25 is one "
+                       "4-cycle instruction after another", **issue}, f, indent=1)
 
 
 WARM_MODEL = os.path.join(ROOT, "sph-cpp_amd", "warm_model.npy")
@@ -501,6 +517,7 @@ def main(argv=None):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": _profile_traffic(HIP_KERNEL.get(dom, dom)),
+        "valu": _profile_valu(HIP_KERNEL.get(dom, dom)),
         "alg_bytes_per_launch": alg,
         "avg_launch_ms": round(avg_launch_ms, 5),
         "launches": kd["launches"],

@@ -35,6 +35,16 @@ def test_library_exports_every_declared_symbol(pbf):
     assert not missing, missing
 
 
+def test_missing_library_fails_loudly(pbf, monkeypatch, tmp_path):
+    """With no built library the package raises instead of falling back to anything else."""
+    monkeypatch.setattr(pbf, "_lib", None)
+    monkeypatch.setattr(pbf, "LIB_PATH", str(tmp_path / "libpbf.so"))
+    with pytest.raises(RuntimeError, match="missing"):
+        pbf.lib()
+    with pytest.raises(RuntimeError, match="missing"):
+        pbf.derive(pbf.Params())
+
+
 def test_params_derive_defaults(pbf):
     p = pbf.Params()
     p.particle_rest_spacing = 0.05

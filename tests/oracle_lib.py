@@ -11,7 +11,10 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+# ORACLE_SO=oracle/liboracle_asan.so (make -C oracle asan) runs the tests
+# against the oracle built with the address and undefined-behaviour
+# sanitizers (the interpreter then needs libasan and libubsan in LD_PRELOAD)
+ORACLE_SO = os.environ.get("ORACLE_SO") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 SERIAL = 0
 JACOBI = 1

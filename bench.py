@@ -106,6 +106,24 @@ def _profile_traffic(kernel: str):
         return None
 
 
+def kernels_roofline(prof, n):
+    """Every timed kernel of the profiled replay: algorithmic bytes per launch
+    (ALG_BYTES x n), mean launch time (HIP events), the HBM fraction they give,
+    and the VALU issue from the committed SQ summary."""
+    out = {}
+    for k, d in prof["kernels"].items():
+        if not d["launches"]:
+            continue
+        ms = d["total_ms"] / d["launches"]
+        alg = ALG_BYTES.get(k, 0) * n
+        gbs = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        v = _profile_valu(HIP_KERNEL.get(k, k)) or {}
+        out[k] = {"launches": d["launches"], "avg_launch_ms": round(ms, 4), "alg_bytes_per_launch": alg,
+                  "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                  "valu_issue_per_simd_cycle": v.get("valu_issue_per_simd_cycle")}
+    return out
+
+
 def _profile_valu(kernel: str):
     """VALU issue of the kernel from the committed SQ counter summary, if present."""
     path = os.path.join(ROOT, "profiles", "pmc_valu.json")
@@ -559,6 +577,7 @@ def main(argv=None):
         },
         "roofline": roof,
         "step_roofline": step_roof,
+        "kernels_roofline": kernels_roofline(prof, n),
         "kernels_ms_per_step": {k: round(v["total_ms"] / args.steps, 4) for k, v in prof["kernels"].items()},
         "kernels_note": "kernels_ms_per_step: a replay of the same substeps with HIP events around every launch "
                         f"({prof['seconds'] * 1e3 / args.steps:.3f} ms per substep with that overhead)",

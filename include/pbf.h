@@ -284,8 +284,9 @@ int pbf_step_graph(pbf_context* ctx, uint32_t nsteps);
 /* ---- launch tuning (no counterpart in the reference) ----
  * Launch parameters that do not change any result: the block size of the fused
  * per-particle kernels (64, 128 or 256), the candidate budget after which the neighbour
- * walk of one particle is handed from one lane to a whole wavefront (64..65536; it applies
- * to particles whose 27 cells hold more than twice that many candidates), and the number
+ * walk of one particle is handed from one lane to a whole wavefront (64..65536, used up to
+ * 2040; it applies to particles whose 27 cells hold more than twice that many candidates,
+ * or more than 2048), and the number
  * of blocks of that wavefront builder (1..65536).  The bin spacing and the list cap
  * are not tuned: both are part of the reference's neighbour semantics (neighbors.h:39,198).
  * pbf_autotune times `steps` fused steps from the current state for each candidate,

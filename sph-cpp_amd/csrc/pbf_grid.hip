@@ -474,13 +474,19 @@ __device__ __forceinline__ WalkCells walk_cells(f3 ps, float s) {
 // One wavefront per block; each lane stages its entries in its own LDS
 // column (entry k of the 64 lanes is contiguous) and writes them out at the
 // end, so that every global store of entry k covers the wavefront's 64
-// consecutive slots instead of 64 scattered 4-byte writes.
+// consecutive slots instead of 64 scattered 4-byte writes.  A staged entry is
+// 16 bits: the cell's place in the 27-cell walk (c, 5 bits) and the slot's
+// offset in that cell (11 bits), which halves the staging and doubles the
+// wavefronts the LDS holds.  The offset stays below kMaxCellOffset because a
+// lane walks a cell in full only when its 27 cells hold at most that many
+// candidates, and otherwise stops after a budget below it.
 constexpr int kFillBlock = 64;
+constexpr uint32_t kMaxCellOffset = 2048u;
 
 // The lane walk of particle p: writes its list and count and returns true,
 // or returns false when the particle goes to the wavefront builder.  ovf
 // receives the candidates dropped by the cap.
-__device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane, int32_t* __restrict__ s_list,
+__device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane, uint16_t* __restrict__ s_list,
                                           const float4* __restrict__ xs, const uint32_t* __restrict__ start,
                                           const KParams& K, int32_t* __restrict__ nbr,
                                           int32_t* __restrict__ nbr_count, uint32_t& ovf) {
@@ -502,20 +508,25 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
   uint32_t total = 0;
 #pragma unroll
   for (int c = 0; c < 27; ++c) total += ce[c] - cb[c];
+  // the lane's budget; below kMaxCellOffset - 8 so that a staged offset fits
+  // in 11 bits (a batch of 8 may pass the budget before the test)
+  const uint32_t hb = min(K.heavy_budget, kMaxCellOffset - 8u);
   // every candidate must be seen to count the overflow: route by the total
-  if (K.count_overflow && total > K.heavy_budget) return false;
+  if (K.count_overflow && total > hb) return false;
   // otherwise the walk ends when the list is full; a lane whose 27 cells hold
   // more than twice K.heavy_budget candidates and that has scanned
   // K.heavy_budget of them without filling its list hands the particle over
   // (the wavefront builder starts again and writes the same entries); lanes
   // with fewer candidates walk them all
-  const uint32_t budget = total > 2u * K.heavy_budget ? K.heavy_budget : 0xFFFFFFFFu;
+  const uint32_t budget = total > min(2u * hb, kMaxCellOffset) ? hb : 0xFFFFFFFFu;
   uint32_t scanned = 0;
   int count = 0;
   const int cap = K.cap;
 #pragma unroll
   for (int c = 0; c < 27; ++c) {
     const uint32_t b = cb[c], e = ce[c];
+    // staged form of slot q: (c << 11) + (q - b), computed as q + eb in 16 bits
+    const uint32_t eb = ((uint32_t)c << 11) - b;
     // candidates of a cell are contiguous: load 8 at a time, test in order
     // (position buffers carry 8 rows of padding, so the loads need no clamp)
     for (uint32_t q0 = b; q0 < e; q0 += 8) {
@@ -533,7 +544,7 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
         const uint32_t q = q0 + (uint32_t)u;
         const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
         const bool acc = (q < e) & (q != p) & (d2 < K.ss);
-        s_list[min(count, cap) * kFillBlock + lane] = (int32_t)q;
+        s_list[min(count, cap) * kFillBlock + lane] = (uint16_t)(q + eb);
         count += acc ? 1 : 0;
       }
       // a full list cannot change any more; only the overflow count could
@@ -546,7 +557,18 @@ done:
     count = cap;
   }
   nbr_count[p] = count;
-  for (int k = 0; k < count; ++k) nbr[list_at(k, p, K.stride)] = s_list[k * kFillBlock + lane];
+  // staged entries come in walk order, so the cells' first slots are taken
+  // from registers cell by cell
+  int k = 0;
+#pragma unroll
+  for (int c = 0; c < 27; ++c) {
+    const uint32_t b = cb[c];
+    for (; k < count; ++k) {
+      const uint32_t e16 = s_list[k * kFillBlock + lane];
+      if ((e16 >> 11) != (uint32_t)c) break;
+      nbr[list_at(k, p, K.stride)] = (int32_t)(b + (e16 & (kMaxCellOffset - 1u)));
+    }
+  }
   return true;
 }
 
@@ -562,7 +584,7 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
                                                                int32_t* __restrict__ nbr_count,
                                                                uint32_t* __restrict__ heavy,
                                                                unsigned long long* __restrict__ counters) {
-  extern __shared__ int32_t s_list[];  // [cap + 1][kFillBlock]
+  extern __shared__ uint16_t s_list[];  // [cap + 1][kFillBlock]
   const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   bool to_heavy = false;
@@ -698,7 +720,7 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
   if (!n) return;
   KernelTimer* t = timer_begin(c, "fill_neighbors");
   hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n, kFillBlock)), dim3(kFillBlock),
-                     (uint32_t)(kFillBlock * (c->K.cap + 1) * sizeof(int32_t)), c->stream, n, (uint32_t)fb,
+                     (uint32_t)(kFillBlock * (c->K.cap + 1) * sizeof(uint16_t)), c->stream, n, (uint32_t)fb,
                      (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
                      (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count, c->heavy, c->counters);
   timer_end(c, t);

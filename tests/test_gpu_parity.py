@@ -420,6 +420,35 @@ def test_crowded_cells_chunk_walk_bit_exact(pbf, scenes, budget):
     _compare_state(g, o, "crowded cells step")
 
 
+@pytest.mark.parametrize("budget", [768, 4096])
+def test_staged_offsets_near_the_limit(pbf, scenes, budget):
+    """One cell of 2,046 particles whose first 1,986 slots lie beyond s from the last 60: those 60
+    lanes reject 1,986 candidates before they accept, so the offsets they stage reach 2,045 (the
+    lane builder stages 11-bit offsets within a cell).  With budget 4096 (used as 2040) the lanes
+    walk the cell themselves; with 768 they hand it to the wavefront builder.  Lists bit for bit."""
+    rng = np.random.default_rng(5)
+    far = 0.235 + rng.uniform(-0.002, 0.002, size=(1986, 3))
+    near = 0.125 + rng.uniform(-0.002, 0.002, size=(60, 3))
+    pos = np.concatenate([far, near]).astype(np.float32)
+    p = scenes.pbf_params((1.0, 1.0, 1.0), pos.shape[0])
+    g, o = _pair(pbf, p, pos, np.zeros_like(pos))
+    tune = g.tuning()
+    g.set_tuning(tune["block_size"], budget, tune["heavy_blocks"])
+    g.find_neighbors()
+    o.pass_(O.FIND)
+    nb = o.neighbors()
+    b, e = g.bin_bounds()
+    assert (e - b).max() == 2046
+    lists, counts = g.neighbor_lists()
+    assert np.array_equal(counts, nb["counts"])
+    assert np.all(counts[1986:] == 59)
+    mask = np.arange(p.max_neighbors)[None, :] < counts[:, None]
+    assert np.array_equal(np.where(mask, lists, 0), np.where(mask, nb["lists"], 0))
+    g.step(1)
+    o.step(1)
+    _compare_state(g, o, "cell of 2,046 step")
+
+
 def test_partial_bin_span(pbf, scenes):
     """find_neighbors with a bin span shorter than the particle count (halo-free
     reference usage bins [0, count), neighbors.h:241-248)."""

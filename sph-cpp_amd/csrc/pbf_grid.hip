@@ -833,22 +833,23 @@ int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits
   if (key_bits == 0 || key_bits > 32) key_bits = 32;
   const uint64_t hn = radix_hist_size(n);
   uint32_t *k2 = nullptr, *v2 = nullptr, *hist = nullptr, *offs = nullptr, *part = nullptr;
-  if (check_hip(hipMallocAsync((void**)&k2, n * 4, s), "hipMallocAsync") ||
-      check_hip(hipMallocAsync((void**)&v2, n * 4, s), "hipMallocAsync") ||
-      check_hip(hipMallocAsync((void**)&hist, hn * 4, s), "hipMallocAsync") ||
-      check_hip(hipMallocAsync((void**)&offs, (hn + 1) * 4, s), "hipMallocAsync") ||
-      check_hip(hipMallocAsync((void**)&part, scan_partials_size(hn) * 4, s), "hipMallocAsync"))
-    return PBF_ERR_HIP;
-  if (radix_sort_pairs(keys, vals, k2, v2, n, key_bits, hist, offs, part, s)) {
-    hipMemcpyAsync(keys, k2, n * 4, hipMemcpyDeviceToDevice, s);
-    hipMemcpyAsync(vals, v2, n * 4, hipMemcpyDeviceToDevice, s);
+  int rc = PBF_OK;
+  if ((rc = check_hip(hipMallocAsync((void**)&k2, n * 4, s), "hipMallocAsync")) ||
+      (rc = check_hip(hipMallocAsync((void**)&v2, n * 4, s), "hipMallocAsync")) ||
+      (rc = check_hip(hipMallocAsync((void**)&hist, hn * 4, s), "hipMallocAsync")) ||
+      (rc = check_hip(hipMallocAsync((void**)&offs, (hn + 1) * 4, s), "hipMallocAsync")) ||
+      (rc = check_hip(hipMallocAsync((void**)&part, scan_partials_size(hn) * 4, s), "hipMallocAsync"))) {
+    // the buffers allocated before the failure are freed below
+  } else {
+    if (radix_sort_pairs(keys, vals, k2, v2, n, key_bits, hist, offs, part, s)) {
+      rc = check_hip(hipMemcpyAsync(keys, k2, n * 4, hipMemcpyDeviceToDevice, s), "sort_by_key copy");
+      if (!rc) rc = check_hip(hipMemcpyAsync(vals, v2, n * 4, hipMemcpyDeviceToDevice, s), "sort_by_key copy");
+    }
+    if (!rc) rc = check_hip(hipGetLastError(), "radix sort launch");
   }
-  hipFreeAsync(k2, s);
-  hipFreeAsync(v2, s);
-  hipFreeAsync(hist, s);
-  hipFreeAsync(offs, s);
-  hipFreeAsync(part, s);
-  return check_hip(hipGetLastError(), "radix sort launch");
+  for (uint32_t* b : {k2, v2, hist, offs, part})
+    if (b) (void)hipFreeAsync(b, s);
+  return rc;
 }
 
 // lower_bound / upper_bound of every value of a counting range

@@ -364,7 +364,9 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = check_hip(hipEventCreate(&c->region_a), "hipEventCreate")) ||
       (rc = check_hip(hipEventCreate(&c->region_b), "hipEventCreate"))) {
     free_all(c);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->done_host) (void)hipHostFree(c->done_host);
+    if (c->region_a) (void)hipEventDestroy(c->region_a);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return rc;
   }
@@ -375,8 +377,12 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   launch_fill_u32((uint32_t*)c->lsum, 0u, N, c->stream);
   launch_fill_u32(c->ehist, 0u, (uint64_t)kMaxSolveSlots + 1, c->stream);
   launch_fill_u64(c->esum, 0ull, (uint64_t)kMaxSolveSlots + 1, c->stream);
+  if ((rc = check_hip(hipStreamSynchronize(c->stream), "pbf_create"))) {
+    pbf_destroy(c);
+    return rc;
+  }
   *out = c;
-  return check_hip(hipStreamSynchronize(c->stream), "pbf_create");
+  return PBF_OK;
 }
 
 int pbf_destroy(pbf_context* c) {

@@ -142,6 +142,65 @@ def test_neighbor_lists_brute_force_and_order():
     assert o2.counters()["overflow"] == int(np.maximum(full["counts"] - 20, 0).sum())
 
 
+def _cells_f32(pts, s32):
+    """Cell coordinates as the reference computes them in float: floor((x + s) / s) (neighbors.h:108-114)."""
+    return np.floor((pts.astype(np.float32) + s32) / s32).astype(np.int64)
+
+
+def _brute_lists(pts, s, cap):
+    """Every q != p with d^2 < s^2 (float32 arithmetic, the reference's order of the sum), ordered by
+    walk rank (x offset outermost, z innermost, cells in float32) then index, cut at cap; also the number
+    cut."""
+    s32 = np.float32(s)
+    ss = s32 * s32
+    cells = _cells_f32(pts, s32)
+    lists, cut = [], 0
+    for p in range(pts.shape[0]):
+        d = (pts[p][None, :] - pts).astype(np.float32)
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        acc = [q for q in np.nonzero(d2 < ss)[0] if q != p]
+        rank = lambda q: tuple(int(v) for v in cells[q] - cells[p] + 1)
+        assert all(0 <= r <= 2 for q in acc for r in rank(q)), "an accepted neighbour outside the 27 cells"
+        acc.sort(key=lambda q: (rank(q), q))
+        cut += max(0, len(acc) - cap)
+        lists.append(acc[:cap])
+    return lists, cut
+
+
+@pytest.mark.parametrize("case", ["empty", "single", "on_cell_faces", "coincident", "one_cell", "random"])
+@pytest.mark.parametrize("cap", [1, 2, 7, 60])
+def test_neighbor_lists_edge_cases_brute_force(case, cap):
+    """fill_neighbors (neighbors.h:197-230) against brute force on the inputs where a walk goes wrong
+    first: no particles, one, particles exactly on cell faces (the floor of (x + s)/s), coincident
+    duplicates (d^2 = 0 is accepted, self is not), everything in one cell, and a random cloud; with caps
+    that cut most lists.  The overflow counter equals the entries cut."""
+    rng = np.random.default_rng([cap, sum(map(ord, case))])
+    s = 0.125  # exact in binary: points on the cell faces sit exactly s apart (d^2 == s^2 is rejected)
+    if case == "empty":
+        pts = np.zeros((0, 3), np.float32)
+    elif case == "single":
+        pts = np.array([[0.5, 0.5, 0.5]], np.float32)
+    elif case == "on_cell_faces":
+        g = (np.arange(2, 7) * np.float32(s)).astype(np.float32)
+        pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    elif case == "coincident":
+        base = rng.uniform(0.3, 0.6, size=(20, 3)).astype(np.float32)
+        pts = np.concatenate([base, base, base[:7]]).astype(np.float32)
+    elif case == "one_cell":
+        pts = (0.25 + rng.uniform(0.0, 0.1, size=(90, 3))).astype(np.float32)
+    else:
+        pts = rng.uniform(0.05, 0.95, size=(400, 3)).astype(np.float32)
+    o = O.Oracle(mk(box=(1.0, 1.0, 1.0), s=s, cap=cap))
+    o.set_state(pts)
+    o.pass_(O.FIND)
+    nb = o.neighbors()
+    expect, cut = _brute_lists(pts, o.params.s, cap)
+    assert nb["counts"].tolist() == [len(x) for x in expect]
+    for p in range(pts.shape[0]):
+        assert nb["lists"][p, : nb["counts"][p]].tolist() == expect[p], p
+    assert o.counters()["overflow"] == cut
+
+
 def test_stop_at_cap_keeps_lists_and_steps():
     """The checker shortcut that ends a walk once its list is full (for the full-size GPU tests'
     collapsed states) keeps every list, count and step result; only the overflow count is lost."""

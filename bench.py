@@ -492,8 +492,8 @@ def main(argv=None):
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: the single-domain PBF step has no multi-GPU path in this build "
-              "(see the sharding design notes).", file=sys.stderr)
+        print("bench.py runs on one GPU only: the PBF step here is one domain on one GPU (slab decomposition "
+              "over several GPUs is a design, not code).", file=sys.stderr)
         return 2
 
     pbf = _load("sph-cpp_amd")

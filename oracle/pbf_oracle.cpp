@@ -33,7 +33,9 @@
 //   bin_id              neighbor_test.cpp:46-121 known answers (3D) + closed form
 //   sort / bounds       numpy stable argsort / searchsorted on the same keys
 //   fill_neighbors      neighbor_test.cpp:124-298 counts; O(N^2) brute force of
-//                       the accepted set, walk order and cap truncation
+//                       the accepted set, walk order and cap truncation, also
+//                       on empty, single, cell-face, coincident and one-cell
+//                       inputs at caps 1, 2, 7, 60
 //   Poly6 / Del_Spikey  kernels_test.cpp zeros and 101^3 normalisation; Spiky
 //                       gradient against a finite difference of the textbook
 //                       Spiky kernel (Mueller et al. 2003)

@@ -343,6 +343,23 @@ def test_spans_and_remove(pbf, scenes):
     assert_bitwise(g.velocities(), o.state()["v"][:-10], "after remove")
 
 
+def test_capacity_and_counts_known_answers(pbf, scenes):
+    """particle_test.cpp:29-120 (3D): a context for 100,000 particles starts with local_count 0,
+    max_local_count 100,000 and available 100,000; after a 30^3 lattice 27,000 and 73,000.  Adding past
+    the capacity fails like the reference's push_back (array.h) and leaves the count unchanged."""
+    p = scenes.pbf_params((3.2, 1.6, 1.6), 100_000)
+    g = pbf.Particles(p)
+    assert (g.local_count(), g.max_local_count(), g.available()) == (0, 100_000, 100_000)
+    pos = scenes.lattice((30, 30, 30))
+    g.add(pos, pos, np.zeros_like(pos))
+    assert (g.local_count(), g.max_local_count(), g.available()) == (27_000, 100_000, 73_000)
+    extra = scenes.lattice((50, 30, 50))
+    with pytest.raises(pbf.PBFError, match="capacity"):
+        g.add(extra, extra, np.zeros_like(extra))
+    assert g.local_count() == 27_000
+    g.close()
+
+
 def test_sort_by_key_primitive(pbf):
     rng = np.random.default_rng(0)
     for n, bits in ((1, 8), (1000, 5), (100_003, 20), (300_000, 32)):

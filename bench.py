@@ -464,7 +464,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--kernel", default="", help="kernel for the roofline (default: the slowest in the timed run)")
     ap.add_argument("--cap", type=int, default=60, help="neighbour list cap (MAX_NEIGHBORS = 60)")
-    ap.add_argument("--no-variants", dest="variants", action="store_false", help="skip the cap-128 variant")
+    ap.add_argument("--no-variants", dest="variants", action="store_false", help="skip the variants (cap 128, 256 batched scenes, 8M double dam break, host buffers)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device to device 5 times (PMC byte-counter calibration)")
     ap.add_argument("--no-warm", dest="warm", action="store_false",

@@ -102,11 +102,16 @@ def output_gain(model, first, count):
         model.set_params(q)
         losses.append(model.gradient(first, count)[1])
     model.set_params(p)
-    c, lh, l1 = losses
+    return gain_from_losses(*losses), losses
+
+
+def gain_from_losses(c, lh, l1):
+    """Minimiser of the parabola through (0, c), (1/2, lh), (1, l1), clamped to
+    [0, 1]; 0 when the three losses do not curve upwards."""
     a = 2.0 * (l1 - c) - 4.0 * (lh - c)   # L(g) = a g^2 - 2 b g + c
     b = 0.5 * (a - (l1 - c))
     g = b / a if a > 0 else 0.0
-    return float(min(max(g, 0.0), 1.0)), losses
+    return float(min(max(g, 0.0), 1.0))
 
 
 def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print, scale=None, scenes_min=8):

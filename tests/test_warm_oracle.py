@@ -3,6 +3,7 @@ import os
 import sys
 
 import numpy as np
+import pytest
 import torch
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
@@ -68,3 +69,18 @@ def test_adam_first_step_is_signed_learning_rate():
     # m_hat = g and v_hat = g^2 after one step, so the move is lr * g / (|g| + eps)
     assert np.allclose(p - p1, 1e-3 * np.sign(g), rtol=1e-4, atol=1e-9)
     assert np.allclose(m1, 0.1 * g, rtol=1e-6) and np.allclose(v1, 0.001 * g * g, rtol=1e-4)  # 1 - 0.999 in float32
+
+
+def test_output_gain_parabola():
+    """The output-gain calibration (train_warm.gain_from_losses) returns the minimiser of the parabola
+    through the losses at gains 0, 1/2, 1: exact for parabolas with a known minimum, clamped to [0, 1],
+    0 for losses that do not curve upwards; the committed model's held-out losses give its gain 0.19."""
+    import importlib
+    tw = importlib.import_module("sph-cpp_amd.train_warm")
+    for gstar, a, m in ((0.19, 30.0, 9.0), (0.7, 2.0, 1.0), (0.0, 5.0, 3.0)):
+        L = lambda g: a * (g - gstar) ** 2 + m
+        assert tw.gain_from_losses(L(0.0), L(0.5), L(1.0)) == pytest.approx(gstar, abs=1e-12)
+    assert tw.gain_from_losses(1.96, 0.81, 0.16) == 1.0      # (g - 1.4)^2: clamped above
+    assert tw.gain_from_losses(0.09, 0.64, 1.69) == 0.0      # (g + 0.3)^2: clamped below
+    assert tw.gain_from_losses(1.0, 2.0, 1.0) == 0.0         # concave
+    assert tw.gain_from_losses(10.17, 11.78, 27.87) == pytest.approx(0.194, abs=1e-3)

@@ -99,3 +99,29 @@ def test_scene_builders(scenes):
     p2, pos2, _ = scenes.config_c2(seed=0)
     assert pos2.shape[0] == 1_048_576
     assert tuple(p2.boundary_max) == pytest.approx((12.8, 6.4, 6.4))
+
+
+def test_double_dam_break_and_batch_builders(scenes):
+    """The 8M double dam break: two 128 x 256 x 128 blocks on opposite walls of 25.6 x 12.8 x 12.8, cap 64.
+    The batch: 256 scenes, counts log-uniform in 4,096..65,536 (up to the lattice rounding), block
+    aspect ratios in [0.5, 2], each box twice the block's length and its height and depth."""
+    p, pos, _ = scenes.double_dam_break(seed=0)
+    assert pos.shape == (8_388_608, 3) and p.max_neighbors == 64
+    assert tuple(p.boundary_max) == pytest.approx((25.6, 12.8, 12.8))
+    left = pos[: pos.shape[0] // 2]
+    right = pos[pos.shape[0] // 2:]
+    assert left[:, 0].max() < 6.4 + 0.001 and right[:, 0].min() > 19.2 - 0.001
+    assert pos[:, 1].max() < 12.8 and pos[:, 2].max() < 6.4 + 0.001
+    del pos, left, right
+    params, boxes, sc = scenes.batched_dam_breaks(num_scenes=256, seed=0)
+    assert len(sc) == 256 and boxes.shape == (256, 3)
+    counts = np.array([x.shape[0] for x, _ in sc])
+    assert counts.min() >= 3000 and counts.max() <= 80000
+    assert 5_000_000 < counts.sum() < 6_500_000 and params.max_particles_local == counts.sum()
+    s = np.float32(scenes.SPACING)
+    for (x, _), b in zip(sc, boxes):
+        ext = x.max(0) - x.min(0) + s  # block extent: lattice span plus one spacing
+        assert np.all(x >= 0) and np.all(x < b)
+        assert b[0] == pytest.approx(2 * ext[0], rel=0.02)
+        assert b[1] == pytest.approx(ext[1], rel=0.02) and b[2] == pytest.approx(ext[2], rel=0.02)
+        assert 0.4 <= ext[0] / ext[1] <= 2.1 and 0.4 <= ext[2] / ext[1] <= 2.1

@@ -877,13 +877,14 @@ void orc_dims(void* h, uint64_t* dims) {
 int orc_get_neighbors(void* h, uint64_t* keys, uint64_t* ids, uint64_t* begin, uint64_t* end,
                       uint64_t* lists, int32_t* counts) {
   Handle* H = (Handle*)h;
+  // std::copy: an empty state has empty vectors, whose data() may be null
   auto get = [&](auto& S) {
-    if (keys) std::memcpy(keys, S.keys.data(), S.keys.size() * 8);
-    if (ids) std::memcpy(ids, S.ids.data(), S.ids.size() * 8);
-    if (begin) std::memcpy(begin, S.begin.data(), S.begin.size() * 8);
-    if (end) std::memcpy(end, S.end.data(), S.end.size() * 8);
-    if (lists) std::memcpy(lists, S.lists.data(), S.lists.size() * 8);
-    if (counts) std::memcpy(counts, S.counts.data(), S.counts.size() * 4);
+    if (keys) std::copy(S.keys.begin(), S.keys.end(), keys);
+    if (ids) std::copy(S.ids.begin(), S.ids.end(), ids);
+    if (begin) std::copy(S.begin.begin(), S.begin.end(), begin);
+    if (end) std::copy(S.end.begin(), S.end.end(), end);
+    if (lists) std::copy(S.lists.begin(), S.lists.end(), lists);
+    if (counts) std::copy(S.counts.begin(), S.counts.end(), counts);
   };
   if (H->dbl) get(H->d); else get(H->f);
   return 0;

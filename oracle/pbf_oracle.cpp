@@ -40,7 +40,11 @@
 //                       gradient against a finite difference of the textbook
 //                       Spiky kernel (Mueller et al. 2003)
 //   densities           particle_test.cpp:227-267 (mean within 5% of rest
-//                       density); two-particle closed form; hack known answer
+//                       density); two-particle closed form; the near-
+//                       coincidence fix (particles.h:303-305) by closed forms
+//                       of coincident pairs and triples in serial and Jacobi
+//                       modes, with planted mistakes shown to fail
+//                       (tests/test_oracle_density_fix.py)
 //   pressure lambdas    Macklin & Mueller 2013 eq. 8-11 restated independently
 //                       in numpy float64 (compressed and stretched cases)
 //   pressure dps        sum of dp over a symmetric neighbourhood is 0
@@ -51,7 +55,8 @@
 //   viscosity           uniform velocity field is a fixed point; two-particle
 //                       exchange direction; serial vs jacobi order semantics
 //   vorticity           rigid rotation: omega = -(2/3) sum_j k_j |d_j|^2 Omega
-//   apply vorticity     uniform |omega| field leaves v unchanged; gradient sign
+//   apply vorticity     |N| <= 1 bounds |dv|; a gradient of |omega| along +x
+//                       gives N = +x (cross-product order, sign of eta)
 //   whole step          isolated particles follow the closed-form ballistic
 //                       trajectory; float vs double spread
 //   C_Spline            zeros (kernels_test.cpp:209-239); Akinci et al. 2013

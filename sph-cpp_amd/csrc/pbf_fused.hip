@@ -387,7 +387,13 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
                                                         const uint64_t* __restrict__ wmask,
                                                         const uint32_t* __restrict__ dirty,
                                                         unsigned long long* __restrict__ counters) {
-  if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
+  // C_DONE holds the slot that ended the solve, plus one.  Only a solve that
+  // ended at an earlier slot skips this launch: blocks of the ending slot that
+  // start after block 0 has written the flag must still copy xt to x*.
+  if (S.tol >= 0.0f) {
+    const unsigned long long done = counters[C_DONE];
+    if (done != 0ull && done <= (unsigned long long)it) return;
+  }
   const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   // clear the next iteration's flags (nobody reads them until the next density pass)
   if (first) {
@@ -406,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
     }
     if (S.tol >= 0.0f && e <= S.tol) {
       if (first) {
-        counters[C_DONE] = 1ull;
+        counters[C_DONE] = (unsigned long long)(it + 1);
         counters[C_ITERS] = (unsigned long long)(it + 1);
       }
       if (p < n) xs[p] = xt[p];

@@ -84,7 +84,7 @@ enum Counter {
   C_EMAX = 6,       // float bits of the max density error (atomicMax on non-negative floats)
   C_MOVED_N = 7,    // two counts (7, 8): particles moved by the density fix, per iteration parity
   C_MOVED_ALL = 9,  // two flags (9, 10): too many moved particles or marked cells -> treat all as dirty
-  C_DONE = 11,      // iterate-to-tolerance: the error fell below the tolerance
+  C_DONE = 11,      // iterate-to-tolerance: 0, or the slot whose error fell below the tolerance plus one
   C_ITERS = 12,     // iterate-to-tolerance: slots used
   C_WARM_KEPT = 13, // the warm-start positions were kept
   C_NUM = 16

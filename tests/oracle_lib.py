@@ -33,15 +33,18 @@ class OrcParams(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_SO):
-            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
-        L = C.CDLL(ORACLE_SO)
+def lib(path=None):
+    """The oracle library at `path` (default ORACLE_SO), loaded once per path.
+    Another path loads a separately built copy, e.g. one with a planted
+    mistake (tests/test_oracle_density_fix.py)."""
+    path = path or ORACLE_SO
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        L = C.CDLL(path)
         P = C.c_void_p
         D = C.POINTER(C.c_double)
         F = C.POINTER(C.c_float)
@@ -73,8 +76,8 @@ def lib():
         L.orc_boundary.argtypes = [P, D]
         L.orc_solve.argtypes = [P, C.c_float, C.c_int, C.c_int, F, F, F, F, F, C.POINTER(C.c_int)]
         L.orc_solve.restype = C.c_int
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def params_from_pbf(p) -> OrcParams:
@@ -107,8 +110,8 @@ def _dp(a):
 class Oracle:
     """One oracle instance: float (default) or double precision."""
 
-    def __init__(self, params: OrcParams, double=False, mode=JACOBI, threads=1):
-        self.L = lib()
+    def __init__(self, params: OrcParams, double=False, mode=JACOBI, threads=1, so=None):
+        self.L = lib(so)
         self.params = params
         self.h = self.L.orc_create(C.byref(params), int(double), int(mode), int(threads))
         self.n = 0

@@ -1,7 +1,7 @@
 """GPU parity at the full benchmark sizes, through the C ABI.
 
 Three workloads against the CPU oracle (Jacobi mode, the device's semantics):
-the 1,048,576-particle dam break (128^3 lattice, 200 substeps), the
+the 1,048,576-particle dam break (128 x 64 x 128 lattice, 200 substeps), the
 8,388,608-particle double dam break (two 128x256x128 blocks, cap 64 with the
 overflow counted, 100 substeps) and the batch of 256 dam breaks of
 4,096..65,536 particles each, stepped by graph replay.
@@ -136,3 +136,48 @@ def test_double_dam_break_8m_full_run_lists_bit_exact(pbf, scenes):
     o = _oracle(p, x, xs, v, stop_at_cap=True)
     o.pass_(O.FIND)
     _assert_lists_equal(g, o, p.max_neighbors, "8M double dam break after 100 substeps")
+
+
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sph-cpp_amd", "warm_model.npy")
+
+
+def test_dam_break_1m_learned_warm_start(pbf, scenes):
+    """The learned warm start on the 1,048,576-particle dam break with the
+    trained network: after two substeps and one plain solve (whose multiplier
+    sums are the previous sums), a PBF_WARM_LEARNED solve of four slots.  Its
+    features must equal the oracle's bit for bit, and the whole solve must
+    equal the oracle's solve warm-started with Lambda_prev + S * y, y being
+    the network's output for those features (pbf_warm_model_predict): slots,
+    error history, positions, velocities and multiplier sums."""
+    if not os.path.exists(MODEL):
+        pytest.skip("no trained model in the tree")
+    p, pos, vel = scenes.config_c2(seed=0)
+    g = pbf.Particles(p)
+    g.add(pos, pos, vel)
+    o = _oracle(p, pos, pos, vel)
+    g.step(2)
+    o.step(2)
+    _, _, lprev, _ = o.solve(0.0, 3)
+    g.solve(0.0, 3)
+    assert np.array_equal(g.lambda_sums(), lprev), "plain solve: multiplier sums"
+    m = pbf.WarmModel(16)
+    m.set_params(np.load(MODEL, allow_pickle=False))
+    g.set_warm_model(m)
+    print("1M learned warm start: device solve", flush=True)
+    g.solve(0.0, 4, pbf.PBF_WARM_LEARNED)
+    feat = g.warm_features()
+    y = m.predict(feat)
+    warm = (lprev + np.float32(pbf.PBF_WARM_LAMBDA_SCALE) * y).astype(np.float32)
+    print("1M learned warm start: oracle solve", flush=True)
+    used, hist, lsum, kept, ofeat = o.solve(0.0, 4, warm=warm, features=True)
+    assert np.array_equal(feat, ofeat), "features"
+    info = g.solve_info()
+    print(f"1M learned warm start: slots {info['iterations']} (oracle {used}), kept {info['warm_start_kept']} "
+          f"(oracle {kept}), nonzero outputs {int(np.count_nonzero(y))}", flush=True)
+    assert info["iterations"] == used and bool(info["warm_start_kept"]) == kept
+    assert np.array_equal(info["max_error_history"], hist)
+    st = o.state()
+    assert np.array_equal(g.positions(), st["x"].astype(np.float32)), "positions"
+    assert np.array_equal(g.velocities(), st["v"].astype(np.float32)), "velocities"
+    assert np.array_equal(g.lambda_sums(), lsum), "multiplier sums"
+    g.close(); o.close(); m.close()

@@ -143,3 +143,31 @@ def test_solve_argument_errors(pbf, scenes):
     with pytest.raises(pbf.PBFError):
         g.solve_info()  # no solve has run
     g.close()
+
+
+def test_solve_stops_at_a_slot_with_density_fixes(pbf, scenes):
+    """The solve ends at a slot whose density pass moved particles (the
+    near-coincidence fix, particles.h:303-305): every block of that slot's
+    pressure update must still copy the moved x* (xt) to x*, including blocks
+    that start after the first block has raised the done flag.  Every lattice
+    point is doubled, so each density pass moves every particle of a pair;
+    128,000 particles give 500 blocks of 256, more than the chip runs at once
+    when each holds a full list walk."""
+    lat = scenes.jitter(scenes.lattice((40, 40, 20)), seed=47)
+    pos = np.concatenate([lat, lat]).astype(np.float32)
+    params = scenes.pbf_params((2.0, 2.0, 1.0), pos.shape[0])
+    v = np.zeros_like(pos)
+    o = _orc(params, pos, v)
+    _, hist_all, _, _ = o.solve(0.0, 8)
+    o.close()
+    # the first slot after slot 0 whose error is below every earlier one
+    k = next(i for i in range(1, 8) if hist_all[i] < hist_all[:i].min())
+    tol = float(hist_all[k])
+    o = _orc(params, pos, v)
+    used, hist, lsum, kept = o.solve(tol, 8)
+    assert used == k + 1
+    g = _gpu(pbf, params, pos, v)
+    g.solve(tol, 8)
+    assert g.stats()["density_hacks"] > 0
+    _check(g, o, used, hist, lsum, kept, "stop at a slot with density fixes")
+    g.close(); o.close()

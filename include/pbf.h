@@ -324,6 +324,15 @@ int pbf_get_neighbor_lists(pbf_context* ctx, uint64_t* out, int32_t* counts);
 /* sort_by_key (sim_algorithms.h:57-61,134-137,207-210): stable LSD radix sort
  * of n (key, value) pairs in place.  key_bits bounds the key width (1..32). */
 int pbf_sort_by_key_u32(uint32_t* d_keys, uint32_t* d_values, uint64_t n, uint32_t key_bits);
+/* No counterpart in the reference.  pbf_sort_by_key_u32 for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same stable sort and result,
+ * with its work space in a fixed device array of the library (about 10 MiB
+ * per GPU, held until the process ends) instead of hipMallocAsync.  n above
+ * PBF_NOALLOC_MAX_PAIRS returns PBF_ERR_CAPACITY.  Every call on a GPU
+ * shares that GPU's work space: calls that overlap on one GPU race. */
+#define PBF_NOALLOC_MAX_PAIRS (1ull << 20)
+int pbf_sort_by_key_u32_noalloc(uint32_t* d_keys, uint32_t* d_values, uint64_t n, uint32_t key_bits);
 /* lower_bound / upper_bound of the counting range [search.begin, search.end)
  * over sorted keys (sim_algorithms.h:69-99,145-173,218-246); one result per
  * searched value. */

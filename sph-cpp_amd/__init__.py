@@ -198,6 +198,7 @@ def lib():
     _sig(L, "pbf_get_bin_bounds", [_P, _U64, _U64])
     _sig(L, "pbf_get_neighbor_lists", [_P, _U64, _I32])
     _sig(L, "pbf_sort_by_key_u32", [_P, _P, C.c_uint64, C.c_uint32])
+    _sig(L, "pbf_sort_by_key_u32_noalloc", [_P, _P, C.c_uint64, C.c_uint32])
     _sig(L, "pbf_lower_bound_counting", [_P, C.c_uint64, IndexSpan, _P])
     _sig(L, "pbf_upper_bound_counting", [_P, C.c_uint64, IndexSpan, _P])
     _sig(L, "pbf_device_alloc", [C.c_uint64, C.POINTER(_P)])
@@ -562,12 +563,18 @@ def check_arithmetic(samples: int, seed: int = 0) -> dict:
             "divide_fast": int(out[2]), "sqrt_fast": int(out[3]), "divide_mismatches_shaped": int(out[4])}
 
 
-def sort_by_key_u32(keys: np.ndarray, values: np.ndarray, key_bits: int = 32):
-    """sim::algorithms::sort_by_key (sim_algorithms.h:57-61): stable, on the device."""
+NOALLOC_MAX_PAIRS = 1 << 20  # PBF_NOALLOC_MAX_PAIRS
+
+
+def sort_by_key_u32(keys: np.ndarray, values: np.ndarray, key_bits: int = 32, no_alloc: bool = False):
+    """sim::algorithms::sort_by_key (sim_algorithms.h:57-61): stable, on the device.
+    no_alloc=True calls pbf_sort_by_key_u32_noalloc (fixed work space, at most
+    NOALLOC_MAX_PAIRS pairs)."""
     keys = np.ascontiguousarray(keys, dtype=np.uint32)
     values = np.ascontiguousarray(values, dtype=np.uint32)
     kb, vb = DeviceBuffer.from_array(keys), DeviceBuffer.from_array(values)
-    _check(lib().pbf_sort_by_key_u32(kb.ptr, vb.ptr, keys.size, int(key_bits)))
+    f = lib().pbf_sort_by_key_u32_noalloc if no_alloc else lib().pbf_sort_by_key_u32
+    _check(f(kb.ptr, vb.ptr, keys.size, int(key_bits)))
     return kb.to_array(np.uint32, keys.size), vb.to_array(np.uint32, keys.size)
 
 

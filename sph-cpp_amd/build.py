@@ -20,7 +20,8 @@ RUNNER = os.path.join(HERE, "sph_run")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["pbf_api.hip", "pbf_grid.hip", "pbf_passes.hip", "pbf_fused.hip", "pbf_mlp.hip", "pbf_check.hip"]
+SOURCES = ["pbf_api.hip", "pbf_grid.hip", "pbf_passes.hip", "pbf_fused.hip", "pbf_mlp.hip", "pbf_check.hip",
+           "pbf_noalloc.hip"]
 
 # -ffp-contract=off: no fused multiply-add, every op rounds like the reference.
 # HIP's default single-precision divide and sqrt are correctly rounded; the

@@ -24,6 +24,10 @@ import statistics
 import sys
 import time
 
+# the CPU baseline's OpenMP threads stay on neighbouring cores; this must be
+# set before the oracle library loads its OpenMP runtime
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -69,11 +73,42 @@ def _load(name):
     return importlib.import_module(name)
 
 
-def cpu_baseline(params, pos, vel, warm=1, timed=5):
-    """Oracle in the reference's in-place mode with OpenMP on the host cores."""
+def host_cpu_info() -> dict:
+    """The host cores this job may use and the CPU model: the affinity mask,
+    the cgroup CPU quota (cpu.max) and OMP_NUM_THREADS, whichever is smallest,
+    as the thread count of the CPU baseline."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    omp = int(omp) if omp and omp.isdigit() and int(omp) > 0 else None
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = min(x for x in (affinity, quota, omp) if x)
+    return {"threads": threads, "nproc": affinity, "cgroup_cpu_quota": quota, "omp_num_threads_env": omp,
+            "cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+
+
+def cpu_baseline(params, pos, vel, warm=1, timed=5, c1=None):
+    """Oracle in the reference's in-place mode with OpenMP on every host core
+    this job may use (threads bound close), on a bounded sample of the same
+    scene: one warm-up and `timed` timed substeps, median.  With c1 = (params,
+    pos, vel) of the 32,768-particle dam break, also its full 100 substeps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     O = _load("oracle_lib")
-    threads = max(1, min(16, os.cpu_count() or 1))
+    info = host_cpu_info()
+    threads = info["threads"]
     o = O.Oracle(O.params_from_pbf(params), mode=O.SERIAL, threads=threads)
     o.set_state(pos, pos, vel)
     o.step(warm)
@@ -85,14 +120,27 @@ def cpu_baseline(params, pos, vel, warm=1, timed=5):
     o.close()
     med = statistics.median(times)
     n = pos.shape[0]
-    return {
+    out = {
         "value": n / med,
         "unit": "particle-substeps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{n} particles (same scene), {warm} warm-up + {timed} timed substeps, median "
-                  f"{med:.3f} s/substep; oracle in the reference's OpenMP in-place mode",
+                  f"{med:.3f} s/substep; oracle in the reference's OpenMP in-place mode, {threads} threads, "
+                  f"OMP_PROC_BIND={info['omp_proc_bind']}",
+        "host": info,
     }
+    if c1 is not None:
+        p1, x1, v1 = c1
+        o = O.Oracle(O.params_from_pbf(p1), mode=O.SERIAL, threads=threads)
+        o.set_state(x1, x1, v1)
+        t0 = time.perf_counter()
+        o.step(100)
+        wall = time.perf_counter() - t0
+        o.close()
+        out["c1_100_substeps"] = {"particles": x1.shape[0], "substeps": 100, "seconds": round(wall, 3),
+                                  "value": x1.shape[0] * 100 / wall, "unit": "particle-substeps/s"}
+    return out
 
 
 def _profile_traffic(kernel: str):
@@ -143,10 +191,10 @@ KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "cell_box", "fill
 HIP_KERNEL = {
     "forces_predict": "pbf::k_forces_predict", "bin": "pbf::k_bin", "reorder": "pbf::k_reorder",
     "cell_box": "pbf::k_cell_box", "fill_neighbors": "pbf::k_fill_neighbors", "fill_heavy": "pbf::k_fill_heavy",
-    "density_lambda": "pbf::k_density_lambda_c<false>", "mark_dirty": "pbf::k_mark_dirty",
-    "lambda_repair": "pbf::k_lambda_repair", "dp_update": "pbf::k_dp_update_c",
-    "viscosity": "pbf::k_viscosity_first", "vorticity": "pbf::k_vorticity_c",
-    "apply_vorticity": "pbf::k_apply_vorticity_c", "viscosity_positions": "pbf::k_viscosity_last",
+    "density_lambda": "pbf::k_density_lambda_c<0, false>", "mark_dirty": "pbf::k_mark_dirty",
+    "lambda_repair": "pbf::k_lambda_repair<false>", "dp_update": "pbf::k_dp_update_c<false>",
+    "viscosity": "pbf::k_viscosity_first<false>", "vorticity": "pbf::k_vorticity_c<false>",
+    "apply_vorticity": "pbf::k_apply_vorticity_c<false>", "viscosity_positions": "pbf::k_viscosity_last<false>",
 }
 
 
@@ -409,48 +457,84 @@ def host_buffer_run(pbf, params, pos, vel, steps, tuning=None):
     return out
 
 
-def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=128, model_path=WARM_MODEL):
+def _solve_run(pbf, scenes, mode, model, counts, seed, steps, tol, max_slots, cap, mean, label):
+    """One trajectory of `steps` iterate-to-tolerance substeps in one warm-start
+    mode; per substep the slots used, whether the criterion was met and the
+    final max and mean density errors."""
+    params, pos, vel = scenes.config_c2(seed=seed, lattice_counts=counts, cap=cap)
+    if mean:
+        params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
+    n = pos.shape[0]
+    g = pbf.Particles(params)
+    g.add(pos, pos, vel)
+    if mode == pbf.PBF_WARM_LEARNED:
+        g.set_warm_model(model)
+    g.sync()
+    slots, met, emax, emean, kept = [], [], [], [], 0
+    t0 = time.perf_counter()
+    for st in range(steps):
+        if st % 50 == 0:
+            print(f"warm-start comparison ({label}): substep {st}", file=sys.stderr, flush=True)
+        g.solve(tol, max_slots, mode)
+        info = g.solve_info()  # waits for the substep
+        k = info["iterations"]
+        slots.append(k)
+        emax.append(float(info["max_error_history"][k - 1]))
+        emean.append(float(info["mean_error_history"][k - 1]))
+        met.append((emean if mean else emax)[-1] <= float(np.float32(tol)))
+        kept += int(info["warm_start_kept"])
+    wall = time.perf_counter() - t0
+    g.close()
+    return {"n": n, "wall": wall, "slots": np.array(slots), "met": np.array(met), "emax": np.array(emax),
+            "emean": np.array(emean), "kept": kept}
+
+
+def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=256, cap=128, mean=False,
+                  model_path=WARM_MODEL):
     """Plain vs warm-started iterate-to-tolerance solves on the same scene:
     each mode runs its own trajectory of `steps` substeps, every substep
-    iterating until the mean density error is <= tol (at most max_slots)."""
+    iterating until the density error (max, or mean with mean=True) is <= tol
+    or max_slots Jacobi slots are used.  The comparison at equal accuracy
+    counts slots only over the substeps where every mode met the tolerance;
+    for substeps that missed it the final max error (the floor the solve
+    reached) is reported."""
     if not os.path.exists(model_path):
         return {"skipped": "no trained warm-start model (sph-cpp_amd/train_warm.py)"}
     weights = np.load(model_path, allow_pickle=False)
     model = pbf.WarmModel(16)
     model.set_params(weights)
-    out = {"criterion": f"mean density error <= {tol} per substep (at most {max_slots} Jacobi slots)",
-           "tolerance": tol, "max_slots": max_slots, "steps": steps}
+    which = "mean" if mean else "max"
+    out = {"criterion": f"{which} density error <= {tol} per substep (at most {max_slots} Jacobi slots), "
+                        f"neighbour cap {cap}",
+           "tolerance": tol, "error": which, "max_slots": max_slots, "neighbor_cap": cap, "steps": steps}
+    runs = {}
     for name, mode in (("plain", pbf.PBF_WARM_NONE), ("previous", pbf.PBF_WARM_PREVIOUS),
                        ("learned", pbf.PBF_WARM_LEARNED)):
-        params, pos, vel = scenes.config_c2(seed=seed, lattice_counts=counts)
-        params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
-        n = pos.shape[0]
-        g = pbf.Particles(params)
-        g.add(pos, pos, vel)
-        if mode == pbf.PBF_WARM_LEARNED:
-            g.set_warm_model(model)
-        g.sync()
-        slots, met, emax, kept = [], 0, [], 0
-        t0 = time.perf_counter()
-        for st in range(steps):
-            if st % 50 == 0:
-                print(f"warm-start comparison: {name}, substep {st}", file=sys.stderr, flush=True)
-            g.solve(tol, max_slots, mode)
-            info = g.solve_info()  # waits for the substep
-            k = info["iterations"]
-            slots.append(k)
-            met += int(info["mean_error_history"][k - 1] <= np.float32(tol))
-            emax.append(float(info["max_error_history"][k - 1]))
-            kept += int(info["warm_start_kept"])
-        wall = time.perf_counter() - t0
-        g.close()
-        out[name] = {"value": n * steps / wall, "ms_per_step": wall * 1e3 / steps,
-                     "mean_slots": float(np.mean(slots)), "max_slots_used": int(np.max(slots)),
-                     "steps_meeting_tolerance": met, "warm_start_kept": kept,
-                     "final_max_error_median": float(np.median(emax))}
+        runs[name] = _solve_run(pbf, scenes, mode, model, counts, seed, steps, tol, max_slots, cap, mean,
+                                f"{which} error, cap {cap}, {name}")
     model.close()
+    all_met = np.logical_and.reduce([r["met"] for r in runs.values()])
+    for name, r in runs.items():
+        miss = ~r["met"]
+        out[name] = {"value": r["n"] * steps / r["wall"], "ms_per_step": r["wall"] * 1e3 / steps,
+                     "mean_slots": float(np.mean(r["slots"])), "max_slots_used": int(np.max(r["slots"])),
+                     "steps_meeting_tolerance": int(np.sum(r["met"])), "warm_start_kept": r["kept"],
+                     "final_max_error_median": float(np.median(r["emax"])),
+                     "final_mean_error_median": float(np.median(r["emean"])),
+                     "mean_slots_where_all_modes_met": (float(np.mean(r["slots"][all_met]))
+                                                        if all_met.any() else None),
+                     "max_error_floor_where_missed": (
+                         {"median": float(np.median(r["emax"][miss])), "min": float(np.min(r["emax"][miss])),
+                          "max": float(np.max(r["emax"][miss]))} if miss.any() else None)}
+    out["steps_where_all_modes_met"] = int(np.sum(all_met))
     out["learned_vs_plain"] = out["learned"]["value"] / out["plain"]["value"]
     out["previous_vs_plain"] = out["previous"]["value"] / out["plain"]["value"]
+    out["learned_vs_previous"] = out["learned"]["value"] / out["previous"]["value"]
+    if all_met.any():
+        ps = out["plain"]["mean_slots_where_all_modes_met"]
+        out["slot_ratio_plain_over_learned_where_all_met"] = ps / out["learned"]["mean_slots_where_all_modes_met"]
+        out["slot_ratio_previous_over_learned_where_all_met"] = (
+            out["previous"]["mean_slots_where_all_modes_met"] / out["learned"]["mean_slots_where_all_modes_met"])
     return out
 
 
@@ -503,7 +587,9 @@ def main(argv=None):
     n = pos.shape[0]
     if args.warm_only:
         ws = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps, model_path=args.warm_model)
-        print(json.dumps({"warm_start": ws, "model": args.warm_model}), flush=True)
+        wm = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps, max_slots=128, cap=60, mean=True,
+                           model_path=args.warm_model)
+        print(json.dumps({"warm_start": ws, "warm_start_mean_error": wm, "model": args.warm_model}), flush=True)
         return 0
 
     # warm-up on a throwaway context (module load, first launches); the
@@ -569,7 +655,8 @@ def main(argv=None):
                         f"12.8x6.4x6.4, dt 1/120, {I} Jacobi iterations, eps 600, XSPH 0.01, vorticity 0.0005, "
                         f"substeps 1..{args.steps} from the initial scene",
             "particles": n,
-            "solver": "plain, fixed iterations",
+            "solver": f"plain, the reference's fixed {I} Jacobi iterations per substep "
+                      "(Simulation/Source/main.cpp:82); no density-error bound",
             "jacobi_iterations": I,
             "neighbor_cap": int(params.max_neighbors),
             "cells": res["cells"],
@@ -582,6 +669,10 @@ def main(argv=None):
         "kernels_note": "kernels_ms_per_step: a replay of the same substeps with HIP events around every launch "
                         f"({prof['seconds'] * 1e3 / args.steps:.3f} ms per substep with that overhead)",
         "stats_last_step": res["stats"],
+        "value_note": f"value is plain PBF with the reference's fixed {I} Jacobi iterations per substep, which "
+                      "bounds no density error (stats_last_step.max_density_error is the last substep's); the "
+                      "metric's density-error side, plain vs learned warm start each iterating to max density "
+                      "error <= 1%, is warm_start (warm_start_mean_error: the same at mean error <= 1%)",
     }
     if args.variants:
         vparams, _, _ = scenes.config_c2(seed=args.seed, lattice_counts=counts, cap=128)
@@ -590,15 +681,45 @@ def main(argv=None):
                                       "stats_last_step": vr["stats"],
                                       "note": "same scene with a 128-entry neighbour list (no list saturation)"}}
     if args.variants:
+        # the configuration's own run length, whatever --steps asked for
+        if args.steps == 200:
+            r200 = res
+        else:
+            r200 = timed_run(pbf, params, pos, vel, 200, tuning=tuning)
+        out["variants"]["c2_200_substeps"] = {"value": r200["value"], "ms_per_step": r200["seconds"] * 1e3 / 200,
+                                             "stats_last_step": r200["stats"],
+                                             "note": "the same run over the configuration's 200 substeps"}
+        p1, x1, v1 = scenes.config_c1(seed=args.seed)
+        r1 = timed_run(pbf, p1, x1, v1, 100)
+        out["variants"]["c1_100_substeps"] = {"particles": x1.shape[0], "value": r1["value"],
+                                             "ms_per_step": r1["seconds"] * 1e3 / 100,
+                                             "stats_last_step": r1["stats"]}
+        fparams, _, _ = scenes.config_c2(seed=args.seed, lattice_counts=counts, cap=args.cap)
+        fparams.options |= pbf.PBF_OPT_FAST_MATH
+        fv = timed_run(pbf, fparams, pos, vel, args.steps, tuning=tuning)
+        fr = timed_run(pbf, fparams, pos, vel, args.steps, profile=("density_lambda", "dp_update"), tuning=tuning)
+        out["variants"]["fast_math"] = {
+            "value": fv["value"], "ms_per_step": fv["seconds"] * 1e3 / args.steps, "stats_last_step": fv["stats"],
+            "density_lambda_avg_launch_ms": round(fr["kernels"]["density_lambda"]["total_ms"] /
+                                                  max(1, fr["kernels"]["density_lambda"]["launches"]), 5),
+            "dp_update_avg_launch_ms": round(fr["kernels"]["dp_update"]["total_ms"] /
+                                             max(1, fr["kernels"]["dp_update"]["launches"]), 5),
+            "note": "same run with PBF_OPT_FAST_MATH: hardware sqrt and reciprocal in the neighbour sums "
+                    "(lists bit-exact; positions within the tolerance of tests/test_gpu_fast_math.py)"}
         out["variants"]["batched_256_scenes"] = batched_run(pbf, scenes, 100, args.seed)
         out["variants"]["double_dam_break_8M"] = double_dam_break_run(pbf, scenes, 100, args.seed)
         out["variants"]["host_buffers"] = host_buffer_run(pbf, params, pos, vel, args.steps, tuning=tuning)
     if args.warm:
         out["warm_start"] = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps,
                                           model_path=args.warm_model)
+        out["warm_start_mean_error"] = warm_start_ab(pbf, scenes, counts, args.seed, args.warm_steps,
+                                                     max_slots=128, cap=60, mean=True, model_path=args.warm_model)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(params, pos, vel)
+        out["cpu_baseline"] = cpu_baseline(params, pos, vel, c1=scenes.config_c1(seed=args.seed))
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        if "variants" in out:
+            out["speedup_vs_cpu_c1_100_substeps"] = (out["variants"]["c1_100_substeps"]["value"] /
+                                                     out["cpu_baseline"]["c1_100_substeps"]["value"])
     print(json.dumps(out), flush=True)
     return 0
 

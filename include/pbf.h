@@ -82,6 +82,17 @@ typedef struct pbf_params {
 /* pbf_solve compares the tolerance with the mean density error
  * (1/n) sum_p max(rho_p/rho0 - 1, 0) instead of the maximum. */
 #define PBF_OPT_SOLVE_MEAN_ERROR 2u
+/* The first density pass of a warm-started pbf_solve also stores the learned warm
+ * start's features in device memory (pbf_get_warm_features,
+ * pbf_warm_model_add_from_context).  Without it they stay in registers: the
+ * learned warm start evaluates its network inside that pass. */
+#define PBF_OPT_KEEP_WARM_FEATURES 4u
+/* The fused step (pbf_step, pbf_step_graph, pbf_solve, pbf_solve_graph) takes the
+ * hardware square root and reciprocal (within about 1 ulp) in its neighbour sums in
+ * place of the correctly rounded square root and divide.  Cell keys, particle order
+ * and neighbour lists stay bit-exact; positions then match the exact mode within the
+ * tolerance tests/test_gpu_fast_math.py states.  The per-pass entry points stay exact. */
+#define PBF_OPT_FAST_MATH 8u
 
 typedef struct pbf_context pbf_context;
 
@@ -196,7 +207,8 @@ uint32_t pbf_num_scenes(const pbf_context* ctx);
  *   PBF_WARM_PREVIOUS  the multipliers summed over the previous solve or step
  * Every solve keeps the sum of the multipliers it applied (the warm ones included);
  * pbf_get_lambda_sums returns it.  Slots are queued in chunks of 8; between chunks
- * the call waits for the device to say whether the solve has ended. */
+ * the call waits for the device to say whether the solve has ended (pbf_solve_graph
+ * queues every slot instead and never waits). */
 enum { PBF_WARM_NONE = 0, PBF_WARM_GIVEN = 1, PBF_WARM_PREVIOUS = 2, PBF_WARM_LEARNED = 3 };
 
 /* The error e_k compared with the tolerance is the maximum by default and the mean with
@@ -219,6 +231,34 @@ int pbf_get_lambda_sums(pbf_context* ctx, float* out);
  * and feature 2 of the learned warm start */
 int pbf_set_lambda_sums(pbf_context* ctx, const float* in);
 int pbf_get_solve_info(pbf_context* ctx, pbf_solve_info* out);
+
+/* No counterpart in the reference (the loop of Simulation/Source/main.cpp:82-99 run
+ * to a tolerance, on the device).  nsteps pbf_solve steps with no host round trip:
+ * two steps are captured into one HIP graph and replayed (an odd count ends with one
+ * step queued the same way).  Every slot of a step is queued; once a slot's density
+ * error is <= tolerance the later slots' kernels return at once, so the call never
+ * waits for the device.  Results are bit for bit those of pbf_solve.  The graph is
+ * captured again when the particle count, parameters, tuning, tolerance,
+ * max_iterations, warm_start or warm-start gain change. */
+int pbf_solve_graph(pbf_context* ctx, float tolerance, uint32_t max_iterations, int warm_start,
+                    uint32_t nsteps);
+
+/* One record per pbf_solve / pbf_solve_graph step, written by the device at the
+ * end of the step (so graph replays record every step).  The context keeps the
+ * last PBF_SOLVE_LOG_SIZE. */
+typedef struct pbf_solve_record {
+  uint64_t index;                   /* 0, 1, 2, ... over the context's solves */
+  uint32_t iterations;              /* slots used */
+  int32_t warm_start_kept;
+  float max_error;                  /* of the last used slot */
+  float mean_error;
+  float max_error_before_warm_start;
+  float mean_error_before_warm_start;
+} pbf_solve_record;
+#define PBF_SOLVE_LOG_SIZE 4096
+/* The last min(count, solves so far, PBF_SOLVE_LOG_SIZE) records, oldest first;
+ * *returned = how many were written. */
+int pbf_get_solve_log(pbf_context* ctx, pbf_solve_record* out, uint32_t count, uint32_t* returned);
 
 /* ---- learned warm start (PBF_WARM_LEARNED; no counterpart in the reference) ----
  * Features of particle p, from the first density pass of a warm-started solve
@@ -272,7 +312,8 @@ int pbf_set_warm_model(pbf_context* ctx, pbf_warm_model* model);
 /* scale of the network's correction: Lambda = Lambda_prev + gain * S * y (default 1, range [0, 4];
  * 0 is PBF_WARM_PREVIOUS) */
 int pbf_set_warm_gain(pbf_context* ctx, float gain);
-/* features of the last warm-started solve, n x 8, original order */
+/* features of the last warm-started solve (run with PBF_OPT_KEEP_WARM_FEATURES), n x 8,
+ * original order */
 int pbf_get_warm_features(pbf_context* ctx, float* out);
 
 /* pbf_step with the launches of two steps captured once into a HIP graph and replayed

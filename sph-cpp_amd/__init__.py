@@ -95,6 +95,8 @@ class Stats(C.Structure):
 PBF_MAX_SOLVE_ITERATIONS = 256
 PBF_WARM_NONE, PBF_WARM_GIVEN, PBF_WARM_PREVIOUS, PBF_WARM_LEARNED = 0, 1, 2, 3
 PBF_OPT_SOLVE_MEAN_ERROR = 2
+PBF_OPT_KEEP_WARM_FEATURES = 4
+PBF_OPT_FAST_MATH = 8
 PBF_WARM_FEATURES = 8
 PBF_WARM_HIDDEN = 64
 PBF_WARM_PARAMS = 4801
@@ -115,6 +117,20 @@ class SolveInfo(C.Structure):
         ("mean_error_history", C.c_float * PBF_MAX_SOLVE_ITERATIONS),
     ]
 
+
+class SolveRecord(C.Structure):
+    _fields_ = [
+        ("index", C.c_uint64),
+        ("iterations", C.c_uint32),
+        ("warm_start_kept", C.c_int32),
+        ("max_error", C.c_float),
+        ("mean_error", C.c_float),
+        ("max_error_before_warm_start", C.c_float),
+        ("mean_error_before_warm_start", C.c_float),
+    ]
+
+
+PBF_SOLVE_LOG_SIZE = 4096
 
 _lib = None
 
@@ -160,6 +176,8 @@ def lib():
     _sig(L, "pbf_apply_surface_tension", [_P, IndexSpan, IndexSpan])
     _sig(L, "pbf_step", [_P, C.c_uint32])
     _sig(L, "pbf_solve", [_P, C.c_float, C.c_uint32, C.c_int])
+    _sig(L, "pbf_solve_graph", [_P, C.c_float, C.c_uint32, C.c_int, C.c_uint32])
+    _sig(L, "pbf_get_solve_log", [_P, C.POINTER(SolveRecord), C.c_uint32, C.POINTER(C.c_uint32)])
     _sig(L, "pbf_step_graph", [_P, C.c_uint32])
     _sig(L, "pbf_set_tuning", [_P, C.POINTER(Tuning)])
     _sig(L, "pbf_get_tuning", [_P, C.POINTER(Tuning)])
@@ -397,6 +415,23 @@ class Particles:
     def solve(self, tolerance: float, max_iterations: int, warm_start: int = PBF_WARM_NONE) -> None:
         """One step with an iterate-to-tolerance pressure solve (pbf_solve)."""
         _check(lib().pbf_solve(self._ctx, float(tolerance), int(max_iterations), int(warm_start)))
+
+    def solve_graph(self, tolerance: float, max_iterations: int, warm_start: int = PBF_WARM_NONE,
+                    nsteps: int = 1) -> None:
+        """nsteps iterate-to-tolerance steps replayed from a HIP graph (pbf_solve_graph)."""
+        _check(lib().pbf_solve_graph(self._ctx, float(tolerance), int(max_iterations), int(warm_start),
+                                     int(nsteps)))
+
+    def solve_log(self, count: int) -> list:
+        """The last `count` per-solve records (pbf_get_solve_log), oldest first."""
+        count = min(int(count), PBF_SOLVE_LOG_SIZE)
+        recs = (SolveRecord * max(1, count))()
+        got = C.c_uint32(0)
+        _check(lib().pbf_get_solve_log(self._ctx, recs, count, C.byref(got)))
+        return [{"index": r.index, "iterations": r.iterations, "warm_start_kept": bool(r.warm_start_kept),
+                 "max_error": r.max_error, "mean_error": r.mean_error,
+                 "max_error_before_warm_start": r.max_error_before_warm_start,
+                 "mean_error_before_warm_start": r.mean_error_before_warm_start} for r in recs[:got.value]]
 
     def set_warm_lambdas(self, lambdas) -> None:
         a = np.ascontiguousarray(lambdas, dtype=np.float32).reshape(-1)

@@ -62,7 +62,8 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs,
+          "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # C++ host driver over the C ABI (restates Simulation/Source/main.cpp)
     drv = os.path.join(HOST, "sph_main.cpp")

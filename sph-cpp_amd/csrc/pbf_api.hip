@@ -179,6 +179,7 @@ static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
   K.cap = (int)P.max_neighbors;
   K.stride = (K.cap + 7) / 8 * 8;
   K.count_overflow = (P.options & PBF_OPT_COUNT_OVERFLOW) ? 1 : 0;
+  K.fast = (P.options & PBF_OPT_FAST_MATH) ? 1 : 0;
   K.batched = 0;
   K.scenes = nullptr;
   K.sid = nullptr;
@@ -232,7 +233,8 @@ static void free_all(pbf_context* c) {
                   c->stage64, c->cell_start, c->scan_partials, c->sk[0], c->sk[1], c->sv[0], c->sv[1],
                   c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty,
                   c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave, c->esum, c->feat, c->mw1,
-                  c->mw2, c->mp, c->scenes, c->sid, c->sida, c->cbox, c->chbox, c->wmask, c->mflag};
+                  c->mw2, c->mp, c->scenes, c->sid, c->sida, c->cbox, c->chbox, c->wmask, c->mflag,
+                  c->slog, c->slog_count};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -281,6 +283,13 @@ void drop_graph(pbf_context* c) {
   c->graph = nullptr;
 }
 
+static void drop_solve_graph(pbf_context* c) {
+  if (c->sgraph_exec) hipGraphExecDestroy(c->sgraph_exec);
+  if (c->sgraph) hipGraphDestroy(c->sgraph);
+  c->sgraph_exec = nullptr;
+  c->sgraph = nullptr;
+}
+
 // The buffers whose roles the host swaps (reorder, per-pass calls): a graph
 // is valid only while they are where they were at capture.
 static std::vector<void*> buffer_sig(const pbf_context* c) {
@@ -298,6 +307,7 @@ extern "C" {
 const char* pbf_last_error(void) { return g_last_error.c_str(); }
 
 int pbf_params_derive(pbf_params* p) {
+  PBF_TRACE("pbf_params_derive");
   if (!p) return set_error(PBF_ERR_ARG, "null params");
   // parameters.h:123-148
   if (!(p->particle_rest_spacing > 0.f)) return set_error(PBF_ERR_ARG, "particle_rest_spacing must be positive");
@@ -313,6 +323,7 @@ int pbf_params_derive(pbf_params* p) {
 }
 
 int pbf_create(const pbf_params* params, pbf_context** out) {
+  PBF_TRACE("pbf_create");
   if (!params || !out) return set_error(PBF_ERR_ARG, "null argument");
   *out = nullptr;
   if (params->max_particles_local == 0 || params->max_particles_local >= (1ull << 31))
@@ -360,6 +371,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->mw2, (uint64_t)PBF_WARM_HIDDEN * PBF_WARM_HIDDEN)) ||
       (rc = dalloc(&c->mp, (uint64_t)PBF_WARM_PARAMS)) ||
       (rc = dalloc(&c->scan_partials, scan_partials_size(N + 1))) ||
+      (rc = dalloc(&c->slog, (uint64_t)PBF_SOLVE_LOG_SIZE)) || (rc = dalloc(&c->slog_count, 1)) ||
       (rc = check_hip(hipHostMalloc((void**)&c->done_host, sizeof(unsigned long long)), "hipHostMalloc")) ||
       (rc = check_hip(hipEventCreate(&c->region_a), "hipEventCreate")) ||
       (rc = check_hip(hipEventCreate(&c->region_b), "hipEventCreate"))) {
@@ -377,6 +389,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   launch_fill_u32((uint32_t*)c->lsum, 0u, N, c->stream);
   launch_fill_u32(c->ehist, 0u, (uint64_t)kMaxSolveSlots + 1, c->stream);
   launch_fill_u64(c->esum, 0ull, (uint64_t)kMaxSolveSlots + 1, c->stream);
+  launch_fill_u64(c->slog_count, 0ull, 1, c->stream);
   if ((rc = check_hip(hipStreamSynchronize(c->stream), "pbf_create"))) {
     pbf_destroy(c);
     return rc;
@@ -386,9 +399,11 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
 }
 
 int pbf_destroy(pbf_context* c) {
+  PBF_TRACE("pbf_destroy");
   if (!c) return PBF_OK;
   hipStreamSynchronize(c->stream);
   drop_graph(c);
+  drop_solve_graph(c);
   timers_release(c);
   free_all(c);
   if (c->done_host) hipHostFree(c->done_host);
@@ -400,6 +415,7 @@ int pbf_destroy(pbf_context* c) {
 }
 
 int pbf_set_params(pbf_context* c, const pbf_params* params) {
+  PBF_TRACE("pbf_set_params");
   if (!c || !params) return set_error(PBF_ERR_ARG, "null argument");
   if (params->max_particles_local != c->capacity || (int)params->max_neighbors != c->cap)
     return set_error(PBF_ERR_ARG, "capacity and max_neighbors are fixed at pbf_create");
@@ -437,6 +453,7 @@ uint64_t pbf_available(const pbf_context* c) { return c ? c->capacity - c->n : 0
 uint64_t pbf_max_local_count(const pbf_context* c) { return c ? c->capacity : 0; }
 
 int pbf_add(pbf_context* c, const float* pos, const float* pos_star, const float* vel, uint64_t count) {
+  PBF_TRACE("pbf_add");
   if (!c || (count && (!pos || !pos_star || !vel))) return set_error(PBF_ERR_ARG, "null argument");
   if (c->n + count > c->capacity) return set_error(PBF_ERR_CAPACITY, "Not enough capacity to push_back");
   if (!count) return PBF_OK;
@@ -467,6 +484,7 @@ int pbf_add(pbf_context* c, const float* pos, const float* pos_star, const float
 
 int pbf_add_to_scene(pbf_context* c, uint32_t scene, const float* pos, const float* pos_star, const float* vel,
                      uint64_t count) {
+  PBF_TRACE("pbf_add_to_scene");
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   if (scene >= c->nscenes) return set_error(PBF_ERR_ARG, "scene index out of range");
   const uint64_t first = c->n;
@@ -480,6 +498,7 @@ int pbf_add_to_scene(pbf_context* c, uint32_t scene, const float* pos, const flo
 // and its grid is derived from it as for a single scene; all other
 // parameters are shared.
 int pbf_create_batched(const pbf_params* params, uint32_t num_scenes, const float* boxes, pbf_context** out) {
+  PBF_TRACE("pbf_create_batched");
   if (!params || !boxes || !out) return set_error(PBF_ERR_ARG, "null argument");
   *out = nullptr;
   if (num_scenes == 0) return set_error(PBF_ERR_ARG, "num_scenes must be positive");
@@ -537,6 +556,7 @@ int pbf_create_batched(const pbf_params* params, uint32_t num_scenes, const floa
 uint32_t pbf_num_scenes(const pbf_context* c) { return c ? c->nscenes : 0; }
 
 int pbf_remove(pbf_context* c, uint64_t count) {
+  PBF_TRACE("pbf_remove");
   if (!c) return set_error(PBF_ERR_ARG, "null argument");
   if (count > c->n) return set_error(PBF_ERR_ARG, "remove: count exceeds local count");
   if (!count) return PBF_OK;
@@ -556,6 +576,7 @@ int pbf_remove(pbf_context* c, uint64_t count) {
 
 // ---- getters / setters in original order ----
 static int get_vec(pbf_context* c, const float4* src, float* out) {
+  PBF_TRACE("pbf_get_state");
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
   if (!c->n) return PBF_OK;
   launch_gather_out_f4(c, src, c->stage, 3);
@@ -565,6 +586,7 @@ static int get_vec(pbf_context* c, const float4* src, float* out) {
   return check_hip(hipStreamSynchronize(c->stream), "download");
 }
 static int get_scalar(pbf_context* c, const float* src, float* out) {
+  PBF_TRACE("pbf_get_state");
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
   if (!c->n) return PBF_OK;
   launch_gather_out_f1(c, src, c->stage);
@@ -574,6 +596,7 @@ static int get_scalar(pbf_context* c, const float* src, float* out) {
   return check_hip(hipStreamSynchronize(c->stream), "download");
 }
 static int set_vec(pbf_context* c, float4* dst, const float* in) {
+  PBF_TRACE("pbf_set_state");
   if (!c || !in) return set_error(PBF_ERR_ARG, "null argument");
   if (!c->n) return PBF_OK;
   int rc = check_hip(hipMemcpyAsync(c->stage, in, c->n * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream),
@@ -583,6 +606,7 @@ static int set_vec(pbf_context* c, float4* dst, const float* in) {
   return check_hip(hipStreamSynchronize(c->stream), "upload");
 }
 static int set_scalar(pbf_context* c, float* dst, const float* in) {
+  PBF_TRACE("pbf_set_state");
   if (!c || !in) return set_error(PBF_ERR_ARG, "null argument");
   if (!c->n) return PBF_OK;
   int rc = check_hip(hipMemcpyAsync(c->stage, in, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream), "upload");
@@ -612,18 +636,21 @@ int pbf_set_lambdas(pbf_context* c, const float* in) { return set_scalar(c, c ? 
   } while (0)
 
 int pbf_apply_external_forces(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_apply_external_forces");
   PBF_CHECK_CTX_SPAN(c, s);
   launch_forces_predict(c, F_FORCES, s.begin, s.end);
   return check_hip(hipGetLastError(), "apply_external_forces");
 }
 
 int pbf_predict_positions(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_predict_positions");
   PBF_CHECK_CTX_SPAN(c, s);
   launch_forces_predict(c, F_PREDICT, s.begin, s.end);
   return check_hip(hipGetLastError(), "predict_positions");
 }
 
 int pbf_find_neighbors(pbf_context* c, pbf_index_span bin_span, pbf_index_span fill_span) {
+  PBF_TRACE("pbf_find_neighbors");
   PBF_CHECK_CTX_SPAN(c, bin_span);
   PBF_CHECK_CTX_SPAN(c, fill_span);
   if (bin_span.begin != 0)
@@ -634,6 +661,7 @@ int pbf_find_neighbors(pbf_context* c, pbf_index_span bin_span, pbf_index_span f
 }
 
 int pbf_compute_densities(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_compute_densities");
   PBF_CHECK_CTX_SPAN(c, s);
   if (int rc = need_lists(c)) return rc;
   launch_density(c, s.begin, s.end);
@@ -642,6 +670,7 @@ int pbf_compute_densities(pbf_context* c, pbf_index_span s) {
 }
 
 int pbf_compute_pressure_lambdas(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_compute_pressure_lambdas");
   PBF_CHECK_CTX_SPAN(c, s);
   if (int rc = need_lists(c)) return rc;
   launch_lambda(c, c->xs, s.begin, s.end);
@@ -649,6 +678,7 @@ int pbf_compute_pressure_lambdas(pbf_context* c, pbf_index_span s) {
 }
 
 int pbf_compute_pressure_dps(pbf_context* c, pbf_index_span s, int substep) {
+  PBF_TRACE("pbf_compute_pressure_dps");
   (void)substep;  // unused by the reference as well (particles.h:348)
   PBF_CHECK_CTX_SPAN(c, s);
   if (int rc = need_lists(c)) return rc;
@@ -657,18 +687,21 @@ int pbf_compute_pressure_dps(pbf_context* c, pbf_index_span s, int substep) {
 }
 
 int pbf_update_position_stars(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_update_position_stars");
   PBF_CHECK_CTX_SPAN(c, s);
   launch_update_position_stars(c, s.begin, s.end);
   return check_hip(hipGetLastError(), "update_position_stars");
 }
 
 int pbf_update_velocities(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_update_velocities");
   PBF_CHECK_CTX_SPAN(c, s);
   launch_update_velocities(c, s.begin, s.end);
   return check_hip(hipGetLastError(), "update_velocities");
 }
 
 int pbf_apply_surface_tension(pbf_context* c, pbf_index_span cs, pbf_index_span ts) {
+  PBF_TRACE("pbf_apply_surface_tension");
   PBF_CHECK_CTX_SPAN(c, cs);
   PBF_CHECK_CTX_SPAN(c, ts);
   if (int rc = need_lists(c)) return rc;
@@ -677,6 +710,7 @@ int pbf_apply_surface_tension(pbf_context* c, pbf_index_span cs, pbf_index_span 
 }
 
 int pbf_apply_viscosity(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_apply_viscosity");
   PBF_CHECK_CTX_SPAN(c, s);
   if (int rc = need_lists(c)) return rc;
   launch_viscosity(c, c->v, c->va, 0, s.begin, s.end);
@@ -685,6 +719,7 @@ int pbf_apply_viscosity(pbf_context* c, pbf_index_span s) {
 }
 
 int pbf_compute_vorticity(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_compute_vorticity");
   PBF_CHECK_CTX_SPAN(c, s);
   if (int rc = need_lists(c)) return rc;
   launch_vorticity(c, c->v, s.begin, s.end);
@@ -692,6 +727,7 @@ int pbf_compute_vorticity(pbf_context* c, pbf_index_span s) {
 }
 
 int pbf_apply_vorticity(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_apply_vorticity");
   PBF_CHECK_CTX_SPAN(c, s);
   if (int rc = need_lists(c)) return rc;
   launch_apply_vorticity(c, c->v, s.begin, s.end);
@@ -699,6 +735,7 @@ int pbf_apply_vorticity(pbf_context* c, pbf_index_span s) {
 }
 
 int pbf_update_positions(pbf_context* c, pbf_index_span s) {
+  PBF_TRACE("pbf_update_positions");
   PBF_CHECK_CTX_SPAN(c, s);
   launch_update_positions(c, s.begin, s.end);
   return check_hip(hipGetLastError(), "update_positions");
@@ -744,6 +781,7 @@ static void step_body(pbf_context* c) {
 }
 
 int pbf_step(pbf_context* c, uint32_t nsteps) {
+  PBF_TRACE("pbf_step");
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   for (uint32_t st = 0; st < nsteps; ++st) step_body(c);
   return check_hip(hipGetLastError(), "pbf_step");
@@ -755,6 +793,7 @@ int pbf_step(pbf_context* c, uint32_t nsteps) {
 // count ends with one ordinary step.  The graph is captured again after
 // anything that changes a step's launches (particle count, parameters).
 int pbf_step_graph(pbf_context* c, uint32_t nsteps) {
+  PBF_TRACE("pbf_step_graph");
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   int rc;
   if (nsteps >= 2 && (!c->graph_exec || c->graph_n != c->n || c->graph_version != c->version ||
@@ -787,10 +826,7 @@ int pbf_step_graph(pbf_context* c, uint32_t nsteps) {
   return check_hip(hipGetLastError(), "pbf_step_graph");
 }
 
-// One step with an iterate-to-tolerance pressure solve.  The loop runs on the
-// device without host round trips: every slot's kernels are queued, and once
-// a slot measures an error <= tol the remaining ones return at once.
-int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm_start) {
+static int solve_check_args(const pbf_context* c, float tolerance, uint32_t max_iterations, int warm_start) {
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   if (!(tolerance >= 0.0f)) return set_error(PBF_ERR_ARG, "tolerance must be >= 0");
   if (max_iterations < 1 || max_iterations > kMaxSolveSlots)
@@ -801,11 +837,26 @@ int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm
     return set_error(PBF_ERR_STATE, "PBF_WARM_GIVEN without pbf_set_warm_lambdas");
   if (warm_start == PBF_WARM_LEARNED && !c->has_model)
     return set_error(PBF_ERR_STATE, "PBF_WARM_LEARNED without pbf_set_warm_model");
+  return PBF_OK;
+}
+
+// One step with an iterate-to-tolerance pressure solve, queued on the stream
+// (arguments checked by solve_check_args).  Every slot's kernels are queued;
+// once a slot measures an error <= tol the later slots' kernels return at
+// once.  With host_chunks the host reads the done flag after every 8 slots
+// and stops queueing once the solve has ended (a finished solve's kernels
+// still cost their dispatch); without it nothing waits for the device, which
+// is what graph capture needs.
+static int solve_body(pbf_context* c, float tolerance, uint32_t max_iterations, int warm_start, bool host_chunks) {
   const uint64_t n = c->n;
   const int warm = warm_start != PBF_WARM_NONE;
   const bool had_lsum = c->track_lsum;
   c->track_lsum = true;
   launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
+  if (c->capturing) {  // stamps repeat on every replay
+    launch_fill_u32(c->dirty, 0u, n, c->stream);
+    launch_fill_u32(c->mflag, 0u, n, c->stream);
+  }
   launch_fill_u64(c->counters + C_ITERS, (unsigned long long)max_iterations, 1, c->stream);
   launch_fill_u64(c->counters + C_WARM_KEPT, (unsigned long long)warm, 1, c->stream);
   launch_fill_u32(c->ehist, 0u, (uint64_t)max_iterations + 1, c->stream);
@@ -824,15 +875,22 @@ int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm
     // first density pass (its error is the reference for the fallback; it
     // writes the features), then the warm update
     ++c->dirty_stamp;
+    // writes the features, or with the learned warm start evaluates the
+    // network on them in the same pass), then the warm update
     const SolveCtl S0{-1.0f, 0, mean, c->ehist, c->esum, nullptr, nullptr};
-    launch_density_lambda_masked(c, 1, 0, S0, FeatCtl{c->feat, c->lsuma});
-    c->feat_valid = true;
-    if (warm_start == PBF_WARM_LEARNED) launch_mlp_warm(c);
+    const bool keep = (c->P.options & PBF_OPT_KEEP_WARM_FEATURES) != 0;
+    FeatCtl F{keep ? c->feat : nullptr, c->lsuma, nullptr, nullptr, nullptr, 0.0f, nullptr};
+    if (warm_start == PBF_WARM_LEARNED) {
+      F.w1 = c->mw1;
+      F.w2 = c->mw2;
+      F.mp = c->mp;
+      F.scale = c->warm_gain * kWarmLambdaScale;
+      F.wlam = c->wlam;
+    }
+    launch_density_lambda_masked(c, 1, 0, S0, F);
+    c->feat_valid = keep;
     launch_warm_dp(c);
   }
-  // Slots are queued in chunks; between chunks the host reads the done flag
-  // and stops queueing once the solve has ended (the kernels of a finished
-  // solve would return at once, but each launch still costs its dispatch).
   constexpr uint32_t kChunk = 8;
   for (uint32_t it = 0; it < max_iterations; ++it) {
     ++c->dirty_stamp;
@@ -840,7 +898,7 @@ int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm
     launch_mark_dirty(c, (int)it, S);
     launch_lambda_repair(c, (int)it, S);
     launch_dp_update_masked(c, (int)it, 0, S);
-    if ((it + 1) % kChunk == 0 && it + 1 < max_iterations) {
+    if (host_chunks && (it + 1) % kChunk == 0 && it + 1 < max_iterations) {
       int rc = check_hip(hipMemcpyAsync(c->done_host, c->counters + C_DONE, sizeof(unsigned long long),
                                         hipMemcpyDeviceToHost, c->stream), "pbf_solve");
       if (!rc) rc = check_hip(hipStreamSynchronize(c->stream), "pbf_solve");
@@ -850,15 +908,108 @@ int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm
   }
   if (warm) launch_add_warm(c);
   launch_update_velocities(c, 0, n);
+  launch_log_solve(c);
   c->last_iterations = max_iterations;
   c->last_was_solve = true;
   step_tail(c);
+  return PBF_OK;
+}
+
+// One step with an iterate-to-tolerance pressure solve, queued slot chunk by
+// slot chunk (the host stops queueing once the device reports the solve done).
+int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm_start) {
+  PBF_TRACE("pbf_solve");
+  int rc = solve_check_args(c, tolerance, max_iterations, warm_start);
+  if (rc) return rc;
+  if ((rc = solve_body(c, tolerance, max_iterations, warm_start, true))) return rc;
   return check_hip(hipGetLastError(), "pbf_solve");
 }
 
+// Two iterate-to-tolerance steps captured into one HIP graph and replayed
+// (two for the same reason as pbf_step_graph: each step swaps the reordered
+// buffers and the multiplier sums on the host).  Nothing in a captured step
+// depends on the host once it is queued.
+int pbf_solve_graph(pbf_context* c, float tolerance, uint32_t max_iterations, int warm_start, uint32_t nsteps) {
+  PBF_TRACE("pbf_solve_graph");
+  int rc = solve_check_args(c, tolerance, max_iterations, warm_start);
+  if (rc) return rc;
+  if (!c->track_lsum) {
+    // the captured steps read the previous sums: start them at zero here, so
+    // that both captured steps see kept sums
+    if (c->n) launch_fill_u32((uint32_t*)c->lsum, 0u, c->n, c->stream);
+    c->track_lsum = true;
+  }
+  const bool stale = !c->sgraph_exec || c->sgraph_n != c->n || c->sgraph_version != c->version ||
+                     c->sgraph_sig != buffer_sig(c) || c->sgraph_max_it != max_iterations ||
+                     c->sgraph_warm != warm_start || std::memcmp(&c->sgraph_tol, &tolerance, 4) != 0 ||
+                     std::memcmp(&c->sgraph_gain, &c->warm_gain, 4) != 0;
+  if (nsteps >= 2 && stale) {
+    drop_solve_graph(c);
+    c->capturing = true;
+    rc = check_hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (!rc) rc = solve_body(c, tolerance, max_iterations, warm_start, false);
+    if (!rc) rc = solve_body(c, tolerance, max_iterations, warm_start, false);
+    const int rc2 = check_hip(hipStreamEndCapture(c->stream, &c->sgraph), "hipStreamEndCapture");
+    c->capturing = false;
+    if (rc || (rc = rc2)) {
+      drop_solve_graph(c);
+      return rc;
+    }
+    if ((rc = check_hip(hipGraphInstantiate(&c->sgraph_exec, c->sgraph, nullptr, nullptr, 0), "hipGraphInstantiate"))) {
+      drop_solve_graph(c);
+      return rc;
+    }
+    c->sgraph_n = c->n;
+    c->sgraph_version = c->version;
+    c->sgraph_sig = buffer_sig(c);
+    c->sgraph_max_it = max_iterations;
+    c->sgraph_warm = warm_start;
+    c->sgraph_tol = tolerance;
+    c->sgraph_gain = c->warm_gain;
+  }
+  uint32_t st = 0;
+  for (; st + 2 <= nsteps; st += 2)  // nsteps >= 2: the graph is current
+    if ((rc = check_hip(hipGraphLaunch(c->sgraph_exec, c->stream), "hipGraphLaunch"))) return rc;
+  if (st < nsteps && (rc = solve_body(c, tolerance, max_iterations, warm_start, false))) return rc;
+  if (nsteps && warm_start != PBF_WARM_NONE) c->feat_valid = (c->P.options & PBF_OPT_KEEP_WARM_FEATURES) != 0;
+  if (nsteps) {
+    c->last_was_solve = true;
+    c->last_iterations = max_iterations;
+  }
+  return check_hip(hipGetLastError(), "pbf_solve_graph");
+}
+
+int pbf_get_solve_log(pbf_context* c, pbf_solve_record* out, uint32_t count, uint32_t* returned) {
+  PBF_TRACE("pbf_get_solve_log");
+  if (!c || !returned || (count && !out)) return set_error(PBF_ERR_ARG, "null argument");
+  *returned = 0;
+  unsigned long long total = 0;
+  int rc = check_hip(hipMemcpyAsync(&total, c->slog_count, sizeof(total), hipMemcpyDeviceToHost, c->stream),
+                     "solve log");
+  if (!rc) rc = check_hip(hipStreamSynchronize(c->stream), "solve log");
+  if (rc) return rc;
+  const uint64_t m = std::min<uint64_t>({(uint64_t)count, (uint64_t)total, (uint64_t)PBF_SOLVE_LOG_SIZE});
+  // records total-m .. total-1 sit at (index % size): at most two runs of the ring
+  uint64_t done = 0;
+  while (done < m) {
+    const uint64_t idx = total - m + done;
+    const uint64_t at = idx % PBF_SOLVE_LOG_SIZE;
+    const uint64_t run = std::min<uint64_t>(m - done, PBF_SOLVE_LOG_SIZE - at);
+    if ((rc = check_hip(hipMemcpyAsync(out + done, c->slog + at, run * sizeof(pbf_solve_record),
+                                       hipMemcpyDeviceToHost, c->stream), "solve log")))
+      return rc;
+    done += run;
+  }
+  if ((rc = check_hip(hipStreamSynchronize(c->stream), "solve log"))) return rc;
+  *returned = (uint32_t)m;
+  return PBF_OK;
+}
+
 int pbf_get_warm_features(pbf_context* c, float* out) {
+  PBF_TRACE("pbf_get_warm_features");
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
-  if (!c->feat_valid) return set_error(PBF_ERR_STATE, "no warm-started pbf_solve has run");
+  if (!c->feat_valid)
+    return set_error(PBF_ERR_STATE, "no warm-started pbf_solve with PBF_OPT_KEEP_WARM_FEATURES has run");
   if (!c->n) return PBF_OK;
   float* tmp = nullptr;
   int rc = check_hip(hipMallocAsync((void**)&tmp, c->n * PBF_WARM_FEATURES * sizeof(float), c->stream), "alloc");
@@ -872,6 +1023,7 @@ int pbf_get_warm_features(pbf_context* c, float* out) {
 }
 
 int pbf_set_warm_lambdas(pbf_context* c, const float* lambdas) {
+  PBF_TRACE("pbf_set_warm_lambdas");
   if (!c || !lambdas) return set_error(PBF_ERR_ARG, "null argument");
   if (c->n) {
     int rc = check_hip(hipMemcpyAsync(c->wgiven, lambdas, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream),
@@ -884,6 +1036,7 @@ int pbf_set_warm_lambdas(pbf_context* c, const float* lambdas) {
 }
 
 int pbf_set_lambda_sums(pbf_context* c, const float* in) {
+  PBF_TRACE("pbf_set_lambda_sums");
   if (!c || !in) return set_error(PBF_ERR_ARG, "null argument");
   if (c->n) {
     int rc = check_hip(hipMemcpyAsync(c->stage, in, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream),
@@ -897,6 +1050,7 @@ int pbf_set_lambda_sums(pbf_context* c, const float* in) {
 }
 
 int pbf_get_lambda_sums(pbf_context* c, float* out) {
+  PBF_TRACE("pbf_get_lambda_sums");
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
   if (!c->track_lsum) return set_error(PBF_ERR_STATE, "no pbf_solve has run");
   if (!c->n) return PBF_OK;
@@ -908,6 +1062,7 @@ int pbf_get_lambda_sums(pbf_context* c, float* out) {
 }
 
 int pbf_get_solve_info(pbf_context* c, pbf_solve_info* out) {
+  PBF_TRACE("pbf_get_solve_info");
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
   if (!c->last_was_solve) return set_error(PBF_ERR_STATE, "the last step was not a pbf_solve");
   unsigned long long h[C_NUM];
@@ -935,6 +1090,7 @@ int pbf_get_solve_info(pbf_context* c, pbf_solve_info* out) {
 }
 
 int pbf_set_tuning(pbf_context* c, const pbf_tuning* t) {
+  PBF_TRACE("pbf_set_tuning");
   if (!c || !t) return set_error(PBF_ERR_ARG, "null argument");
   if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
     return set_error(PBF_ERR_ARG, "block_size must be 64, 128 or 256");
@@ -957,6 +1113,7 @@ int pbf_get_tuning(const pbf_context* c, pbf_tuning* out) {
 // restores that state: the next step sorts by (cell, original id) and
 // recomputes everything else, so it is the step an untuned run would take.
 int pbf_autotune(pbf_context* c, uint32_t steps, pbf_tuning* best, double* best_ms) {
+  PBF_TRACE("pbf_autotune");
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   if (steps == 0) return set_error(PBF_ERR_ARG, "steps must be positive");
   const uint64_t n = c->n;
@@ -1033,11 +1190,13 @@ int pbf_autotune(pbf_context* c, uint32_t steps, pbf_tuning* best, double* best_
 }
 
 int pbf_sync(pbf_context* c) {
+  PBF_TRACE("pbf_sync");
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   return check_hip(hipStreamSynchronize(c->stream), "pbf_sync");
 }
 
 int pbf_get_stats(pbf_context* c, pbf_stats* out) {
+  PBF_TRACE("pbf_get_stats");
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
   unsigned long long h[C_NUM];
   int rc = check_hip(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream), "stats");
@@ -1062,6 +1221,7 @@ int pbf_get_stats(pbf_context* c, pbf_stats* out) {
 
 // ---- introspection ----
 int pbf_grid_dims(const pbf_context* c, uint64_t dims[3]) {
+  PBF_TRACE("pbf_grid_dims");
   if (!c || !dims) return set_error(PBF_ERR_ARG, "null argument");
   for (int i = 0; i < 3; ++i) dims[i] = c->dims[i];
   return PBF_OK;
@@ -1077,6 +1237,7 @@ static int download_u32(pbf_context* c, const uint32_t* src, uint64_t count, std
 }
 
 int pbf_get_bin_ids(pbf_context* c, uint64_t* out) {
+  PBF_TRACE("pbf_get_bin_ids");
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
   if (int rc = need_lists(c)) return rc;
   std::vector<uint32_t> k;
@@ -1086,6 +1247,7 @@ int pbf_get_bin_ids(pbf_context* c, uint64_t* out) {
 }
 
 int pbf_get_particle_ids(pbf_context* c, uint64_t* out) {
+  PBF_TRACE("pbf_get_particle_ids");
   if (!c || !out) return set_error(PBF_ERR_ARG, "null argument");
   if (int rc = need_lists(c)) return rc;
   std::vector<uint32_t> o;
@@ -1095,6 +1257,7 @@ int pbf_get_particle_ids(pbf_context* c, uint64_t* out) {
 }
 
 int pbf_get_bin_bounds(pbf_context* c, uint64_t* begin, uint64_t* end) {
+  PBF_TRACE("pbf_get_bin_bounds");
   if (!c || !begin || !end) return set_error(PBF_ERR_ARG, "null argument");
   if (int rc = need_lists(c)) return rc;
   std::vector<uint32_t> st;
@@ -1107,6 +1270,7 @@ int pbf_get_bin_bounds(pbf_context* c, uint64_t* begin, uint64_t* end) {
 }
 
 int pbf_get_neighbor_lists(pbf_context* c, uint64_t* out, int32_t* counts) {
+  PBF_TRACE("pbf_get_neighbor_lists");
   if (!c || !out || !counts) return set_error(PBF_ERR_ARG, "null argument");
   if (int rc = need_lists(c)) return rc;
   const uint64_t n = c->n, K = (uint64_t)c->cap;
@@ -1126,6 +1290,7 @@ int pbf_get_neighbor_lists(pbf_context* c, uint64_t* out, int32_t* counts) {
 
 // ---- primitives ----
 int pbf_sort_by_key_u32(uint32_t* d_keys, uint32_t* d_values, uint64_t n, uint32_t key_bits) {
+  PBF_TRACE("pbf_sort_by_key_u32");
   if (n && (!d_keys || !d_values)) return set_error(PBF_ERR_ARG, "null argument");
   int rc = radix_sort_u32(d_keys, d_values, n, key_bits, 0);
   if (rc) return rc;
@@ -1133,6 +1298,7 @@ int pbf_sort_by_key_u32(uint32_t* d_keys, uint32_t* d_values, uint64_t n, uint32
 }
 
 int pbf_lower_bound_counting(const uint32_t* d_sorted, uint64_t n, pbf_index_span search, uint32_t* d_out) {
+  PBF_TRACE("pbf_lower_bound_counting");
   if (search.end < search.begin || (search.end > search.begin && (!d_out || (n && !d_sorted))))
     return set_error(PBF_ERR_ARG, "bad argument");
   launch_bounds_counting(d_sorted, n, search.begin, search.end, d_out, 0, 0);
@@ -1140,6 +1306,7 @@ int pbf_lower_bound_counting(const uint32_t* d_sorted, uint64_t n, pbf_index_spa
 }
 
 int pbf_upper_bound_counting(const uint32_t* d_sorted, uint64_t n, pbf_index_span search, uint32_t* d_out) {
+  PBF_TRACE("pbf_upper_bound_counting");
   if (search.end < search.begin || (search.end > search.begin && (!d_out || (n && !d_sorted))))
     return set_error(PBF_ERR_ARG, "bad argument");
   launch_bounds_counting(d_sorted, n, search.begin, search.end, d_out, 1, 0);
@@ -1152,12 +1319,15 @@ int pbf_device_alloc(uint64_t bytes, void** out) {
 }
 int pbf_device_free(void* p) { return p ? check_hip(hipFree(p), "hipFree") : PBF_OK; }
 int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes) {
+  PBF_TRACE("pbf_memcpy_to_device");
   return check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
 }
 int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes) {
+  PBF_TRACE("pbf_memcpy_to_host");
   return check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
 }
 int pbf_memcpy_device(void* dst, const void* src, uint64_t bytes) {
+  PBF_TRACE("pbf_memcpy_device");
   return check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");
 }
 
@@ -1230,6 +1400,7 @@ static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, u
 }  // namespace pbf
 
 extern "C" int pbf_append_positions_record(pbf_context* c, const char* path) {
+  PBF_TRACE("pbf_append_positions_record");
   if (!c || !path) return set_error(PBF_ERR_ARG, "null argument");
   std::vector<float> pos(c->n * 3);
   if (int rc = pbf_get_positions(c, pos.data())) return rc;

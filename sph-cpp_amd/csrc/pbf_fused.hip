@@ -23,7 +23,7 @@
 //    x*_p in the middle of the walk; when it fires, the density pass finishes
 //    that particle with a plain walk and raises the iteration's flag, and the
 //    lambda and dp passes of that iteration fall back to plain walks.
-#include "pbf_device.h"
+#include "pbf_mlp_device.h"
 
 namespace pbf {
 
@@ -70,14 +70,14 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return x 
 // and is in the sums).  With FEAT, also the learned warm start's features of
 // the particle (FeatCtl).
 // Returns the particle's density error max(rho/rho0 - 1, 0).
-template <bool FEAT>
+template <bool FEAT, bool FAST>
 __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int it, const KParams& K,
                                                     const float4* __restrict__ xs, const float4* __restrict__ v,
                                                     float4* __restrict__ xt, float* __restrict__ rho,
                                                     float* __restrict__ lam, const int32_t* __restrict__ nbr,
                                                     const int32_t* __restrict__ nbr_count,
                                                     uint64_t* __restrict__ wmask, const FeatCtl& F,
-                                                    uint32_t& hacks_out) {
+                                                    uint32_t& hacks_out, float (&fo)[8]) {
   f3 xp = ld3(xs, p);
   const int cnt = nbr_count[p];
   float density = K.mass_W0;
@@ -107,9 +107,9 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
       bits |= (uint32_t)acc << u;
       hack_at = hk ? k0 + u : hack_at;
       if (acc) {
-        const float r = sqrt_cr(d2);
+        const float r = nb_sqrt<FAST>(d2);
         density += K.mass * poly6_in(r, K);
-        const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
+        const f3 grad = smul3(K.neg_inv_rho0, nb_spiky<FAST>(d, r, K));
         dec3(sg, grad);
         sum_C += magsq3(grad);
         if (FEAT) {
@@ -143,9 +143,9 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
         d2 = magsq3(d);
         if (!within_h(d2, K)) continue;
       }
-      const float r = sqrt_cr(d2);
+      const float r = nb_sqrt<FAST>(d2);
       density += K.mass * poly6_in(r, K);
-      const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
+      const f3 grad = smul3(K.neg_inv_rho0, nb_spiky<FAST>(d, r, K));
       dec3(sg, grad);
       sum_C += magsq3(grad);
       if (FEAT) {
@@ -167,11 +167,14 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   if (FEAT) {
     const float inv_s = 1.0f / kWarmLambdaScale;
     const float dyn = dy_sum / (float)(within > 0 ? within : 1);
-    float4* o = (float4*)(F.feat + (uint64_t)p * 8);
-    o[0] = make_float4(clampf(constraint, -1.0f, 4.0f), clampf(lp * inv_s, -64.0f, 0.0f),
-                       clampf(F.lprev[p] * inv_s, -64.0f, 0.0f), clampf(sum_C * 1e-5f, 0.0f, 8.0f));
-    o[1] = make_float4(clampf((float)within / 32.0f, 0.0f, 4.0f), clampf(vp.y * K.dt / K.h, -4.0f, 4.0f),
-                       clampf(dyn / K.h, -2.0f, 2.0f), clampf(div * K.mass * K.dt, -4.0f, 4.0f));
+    fo[0] = clampf(constraint, -1.0f, 4.0f);
+    fo[1] = clampf(lp * inv_s, -64.0f, 0.0f);
+    fo[2] = clampf(F.lprev[p] * inv_s, -64.0f, 0.0f);
+    fo[3] = clampf(sum_C * 1e-5f, 0.0f, 8.0f);
+    fo[4] = clampf((float)within / 32.0f, 0.0f, 4.0f);
+    fo[5] = clampf(vp.y * K.dt / K.h, -4.0f, 4.0f);
+    fo[6] = clampf(dyn / K.h, -2.0f, 2.0f);
+    fo[7] = clampf(div * K.mass * K.dt, -4.0f, 4.0f);
   }
   return constraint > 0.0f ? constraint : 0.0f;
 }
@@ -196,7 +199,39 @@ __device__ __forceinline__ unsigned long long fixed_error(float e) {
 // error goes into slot eslot (atomicMax on the bits of a non-negative float
 // orders like the float) and its fixed-point sum into S.esum[eslot].  With
 // S.tol >= 0 nothing runs once the solve is done.
-template <bool FEAT>
+// The learned warm start for one wavefront's 64 particles (every lane of the
+// wavefront calls it; fo holds the lane's features, zero past n): the
+// features go through LDS into the B operands of the network, which runs on
+// four tiles of 16 particles, and wlam[q] = lprev[q] + scale * net(q).  The
+// same operands and MFMA sequence as k_mlp_forward, so the same results.
+__device__ __forceinline__ void warm_network(const FeatCtl& F, uint32_t wave_first, uint32_t n, const float (&fo)[8],
+                                             float* __restrict__ s_feat) {
+  const int l = threadIdx.x & 63, c = l & 15, g = l >> 4;
+  float4* row = reinterpret_cast<float4*>(s_feat + l * 8);
+  row[0] = make_float4(fo[0], fo[1], fo[2], fo[3]);
+  row[1] = make_float4(fo[4], fo[5], fo[6], fo[7]);
+  __syncthreads();
+  const MlpW W{F.w1, F.w2, nullptr, F.mp};
+  v4s a1[4], a2[4][4];
+  load_forward_operands(W, c, g, a1, a2);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    v4s xb = {0, 0, 0, 0};
+    if (g < 2) {
+      const float4 f4 = *reinterpret_cast<const float4*>(s_feat + (16 * t + c) * 8 + 4 * g);
+      xb = v4s{f2bf(f4.x), f2bf(f4.y), f2bf(f4.z), f2bf(f4.w)};
+    }
+    v4f z1[4], h1[4], z2[4], h2[4];
+    const float y = mlp_forward(W, a1, a2, xb, g, z1, h1, z2, h2);
+    const uint32_t q = wave_first + 16u * (uint32_t)t + (uint32_t)c;
+    if (g == 0 && q < n) F.wlam[q] = F.lprev[q] + F.scale * y;
+  }
+}
+
+// FM = 0: density and lambda only; 1: also the warm start's features (stored
+// when F.feat is set); 2: also the network on them (F.wlam).  FAST: the
+// PBF_OPT_FAST_MATH arithmetic.
+template <int FM, bool FAST>
 __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S, FeatCtl F,
                                                              KParams K, const float4* __restrict__ xs,
                                                              const float4* __restrict__ v, float4* __restrict__ xt,
@@ -211,7 +246,17 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
   float e = 0.0f;
   uint32_t hacks = 0;
-  if (p < n) e = density_lambda_one<FEAT>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, F, hacks);
+  float fo[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (p < n) e = density_lambda_one<FM != 0, FAST>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, F, hacks, fo);
+  if (FM != 0 && F.feat && p < n) {
+    float4* o = reinterpret_cast<float4*>(F.feat + (uint64_t)p * 8);
+    o[0] = make_float4(fo[0], fo[1], fo[2], fo[3]);
+    o[1] = make_float4(fo[4], fo[5], fo[6], fo[7]);
+  }
+  if constexpr (FM == 2) {
+    __shared__ __attribute__((aligned(16))) float s_feat[kBlock * 8];
+    warm_network(F, p - (threadIdx.x & 63u), n, fo, s_feat + (threadIdx.x & ~63u) * 8);
+  }
   // particles moved by the density fix: one set of counter updates per
   // wavefront (every lane of the wavefront is here)
   const uint64_t hm = __ballot(hacks != 0);
@@ -314,6 +359,7 @@ __device__ __forceinline__ f3 ld3_words(const float4* a, uint32_t i) {
   return {f[0], f[1], f[2]};
 }
 
+template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, uint32_t stamp, SolveCtl S,
                                                           KParams K, float4* __restrict__ xt,
                                                           const float* __restrict__ rho, float* __restrict__ lam,
@@ -355,8 +401,8 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
       const f3 d = sub3(xp, t[u]);
       const float d2 = magsq3(d);
       if (k0 + u >= cnt || !within_h(d2, K)) continue;
-      const float r = sqrt_cr(d2);
-      const f3 grad = smul3(K.neg_inv_rho0, spiky_in(d, r, K));
+      const float r = nb_sqrt<FAST>(d2);
+      const f3 grad = smul3(K.neg_inv_rho0, nb_spiky<FAST>(d, r, K));
       dec3(sg, grad);
       sum_C += magsq3(grad);
     }
@@ -379,6 +425,7 @@ __device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) 
 // density pass measured: a rejected warm start (slot 0 error above the error
 // before the warm start) restores xsave; an error <= tol ends the solve with
 // x* = xt.  Neither applies an update.
+template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, SolveCtl S,
                                                         KParams K, const float4* __restrict__ xt,
                                                         const float4* __restrict__ x, float4* __restrict__ xs,
@@ -444,8 +491,8 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
       const f3 d = sub3(xp, xyz(t[u]));
       const float d2 = magsq3(d);
       if (k0 + u >= cnt || !((bits >> u) & 1u) || (plain && !within_h(d2, K))) continue;
-      const float r = sqrt_cr(d2);
-      acc3(dp, smul3(lp + t[u].w, spiky_in(d, r, K)));
+      const float r = nb_sqrt<FAST>(d2);
+      acc3(dp, smul3(lp + t[u].w, nb_spiky<FAST>(d, r, K)));
     }
   }
   f3 nw = add3(xp, smul3(K.inv_rho0, dp));
@@ -464,6 +511,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
 // multipliers in place of lambda, over the full lists with the within-h test.
 // xt holds the position stars after the first density pass; they are kept in
 // xsave for the fallback.
+template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_warm_dp(uint32_t n, KParams K, const float4* __restrict__ xt,
                                                     const float* __restrict__ wlam,
                                                     const int32_t* __restrict__ nbr,
@@ -491,8 +539,8 @@ __global__ void __launch_bounds__(kBlock) k_warm_dp(uint32_t n, KParams K, const
       const f3 d = sub3(xp, xyz(t[u]));
       const float d2 = magsq3(d);
       if (k0 + u >= cnt || !within_h(d2, K)) continue;
-      const float r = sqrt_cr(d2);
-      acc3(dp, smul3(lp + lq[u], spiky_in(d, r, K)));
+      const float r = nb_sqrt<FAST>(d2);
+      acc3(dp, smul3(lp + lq[u], nb_spiky<FAST>(d, r, K)));
     }
   }
   f3 nw = add3(xp, smul3(K.inv_rho0, dp));
@@ -511,7 +559,7 @@ __global__ void k_add_warm(uint32_t n, const float* __restrict__ wlam, float* __
 
 // XSPH sum (particles.h:441-444) over a compact row, from global memory
 // (FROM_LDS = false: cnb, tiled like the lists) or from the lane's LDS column
-template <bool FROM_LDS>
+template <bool FROM_LDS, bool FAST>
 __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_t n, uint32_t lane, int nc,
                                        uint32_t p, f3 xp, f3 vp, const KParams& K, const float4* __restrict__ xs,
                                        const float* __restrict__ rho, const float4* __restrict__ vin) {
@@ -533,8 +581,8 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= nc) continue;
-      const float r = sqrt_cr(magsq3(sub3(xp, xyz(t[u]))));
-      acc3(dv, div3(mul3(sub3(xyz(vq[u]), vp), poly6_in(r, K)), rq[u]));
+      const float r = nb_sqrt<FAST>(magsq3(sub3(xp, xyz(t[u]))));
+      acc3(dv, nb_div3<FAST>(mul3(sub3(xyz(vq[u]), vp), poly6_in(r, K)), rq[u]));
     }
   }
   return dv;
@@ -543,6 +591,7 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
 // ---------------------------------------------------------------------------
 // apply_viscosity (particles.h:436-447), first call: walks the whole list and
 // leaves the compact rows of the final positions for the passes that follow.
+template <bool FAST>
 __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                             const float* __restrict__ rho,
                                                             const float4* __restrict__ vin,
@@ -575,13 +624,14 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
   }
   cnb_count[p] = nc;
   flush_compact(s_list, lane, nc, cnb, K.stride, p);
-  const f3 dv = xsph_sum<true>(s_list, n, lane, nc, p, xp, vp, K, xs, rho, vin);
+  const f3 dv = xsph_sum<true, FAST>(s_list, n, lane, nc, p, xp, vp, K, xs, rho, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
   st3(vout, p, nv);
 }
 
 // apply_viscosity, second call, fused with update_positions (particles.h:390-394)
+template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                            const float* __restrict__ rho,
                                                            const float4* __restrict__ vin, float4* __restrict__ vout,
@@ -591,7 +641,7 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K
   if (p >= n) return;
   const f3 xp = ld3(xs, p);
   const f3 vp = ld3(vin, p);
-  const f3 dv = xsph_sum<false>(cnb, n, 0, cnb_count[p], p, xp, vp, K, xs, rho, vin);
+  const f3 dv = xsph_sum<false, FAST>(cnb, n, 0, cnb_count[p], p, xp, vp, K, xs, rho, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
   st3(vout, p, nv);
@@ -599,6 +649,7 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K
 }
 
 // compute_vorticity (particles.h:452-466) -> scr
+template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                         const float4* __restrict__ vin, float4* __restrict__ scr,
                                                         const int32_t* __restrict__ cnb,
@@ -622,14 +673,15 @@ __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, c
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= cnt) continue;
       const f3 d = sub3(xp, xyz(t[u]));
-      const float r = sqrtf(magsq3(d));
-      acc3(w, cross3(sub3(xyz(vq[u]), vp), spiky_in(d, r, K)));
+      const float r = FAST ? __builtin_amdgcn_sqrtf(magsq3(d)) : sqrtf(magsq3(d));
+      acc3(w, cross3(sub3(xyz(vq[u]), vp), nb_spiky<FAST>(d, r, K)));
     }
   }
   st3(scr, p, w);
 }
 
 // apply_vorticity (particles.h:471-487), v in place (own particle only)
+template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                               const float4* __restrict__ scr,
                                                               float4* __restrict__ v,
@@ -653,9 +705,9 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= cnt) continue;
       const f3 d = sub3(xp, xyz(t[u]));
-      const float r = sqrtf(magsq3(d));
-      const float wm = sqrtf(magsq3(xyz(wq[u])));
-      acc3(eta, smul3(wm, spiky_in(d, r, K)));
+      const float r = FAST ? __builtin_amdgcn_sqrtf(magsq3(d)) : sqrtf(magsq3(d));
+      const float wm = FAST ? __builtin_amdgcn_sqrtf(magsq3(xyz(wq[u]))) : sqrtf(magsq3(xyz(wq[u])));
+      acc3(eta, smul3(wm, nb_spiky<FAST>(d, r, K)));
     }
   }
   const f3 N = div3(eta, sqrtf(magsq3(eta)) + K.r_eps);
@@ -685,15 +737,35 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     timer_end((c), t_);                                                                                     \
   } while (0)
 
-void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S, const FeatCtl& F) {
-  if (F.feat)
-    PBF_LAUNCH(c, "density_features", k_density_lambda_c<true>, it, eslot, S, F, c->K, (const float4*)c->xs,
+// the kernel template KT instantiated for the context's arithmetic (PBF_OPT_FAST_MATH)
+#define PBF_LAUNCH_ARITH(launch, c, name, KT, ...)        \
+  do {                                                    \
+    if ((c)->K.fast)                                      \
+      launch(c, name, (KT<true>), __VA_ARGS__);           \
+    else                                                  \
+      launch(c, name, (KT<false>), __VA_ARGS__);          \
+  } while (0)
+
+template <int FM>
+static void launch_density_fm(pbf_context* c, const char* name, int it, int eslot, const SolveCtl& S,
+                              const FeatCtl& F) {
+  if (c->K.fast)
+    PBF_LAUNCH(c, name, (k_density_lambda_c<FM, true>), it, eslot, S, F, c->K, (const float4*)c->xs,
                (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
                c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters);
   else
-    PBF_LAUNCH(c, "density_lambda", k_density_lambda_c<false>, it, eslot, S, F, c->K, (const float4*)c->xs,
+    PBF_LAUNCH(c, name, (k_density_lambda_c<FM, false>), it, eslot, S, F, c->K, (const float4*)c->xs,
                (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
                c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters);
+}
+
+void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S, const FeatCtl& F) {
+  if (F.wlam)
+    launch_density_fm<2>(c, "density_warm_network", it, eslot, S, F);
+  else if (F.feat)
+    launch_density_fm<1>(c, "density_features", it, eslot, S, F);
+  else
+    launch_density_fm<0>(c, "density_lambda", it, eslot, S, F);
 }
 
 void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S) {
@@ -707,20 +779,49 @@ void launch_mark_dirty(pbf_context* c, int it, const SolveCtl& S) {
 }
 
 void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S) {
-  PBF_LAUNCH(c, "lambda_repair", k_lambda_repair, it, c->dirty_stamp, S, c->K, c->xt,
+  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "lambda_repair", k_lambda_repair, it, c->dirty_stamp, S, c->K, c->xt,
              (const float*)c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
              (const uint32_t*)c->mflag, c->dirty, (const unsigned long long*)c->counters);
 }
 
 void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S) {
-  PBF_LAUNCH(c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
+  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
              (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
              (const int32_t*)c->nbr_count, (const uint64_t*)c->wmask, (const uint32_t*)c->dirty, c->counters);
 }
 
 void launch_warm_dp(pbf_context* c) {
-  PBF_LAUNCH(c, "warm_dp", k_warm_dp, c->K, (const float4*)c->xt, (const float*)c->wlam, (const int32_t*)c->nbr,
+  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "warm_dp", k_warm_dp, c->K, (const float4*)c->xt, (const float*)c->wlam, (const int32_t*)c->nbr,
              (const int32_t*)c->nbr_count, c->xs, c->xsave);
+}
+
+// The record of the solve that just ran (pbf_get_solve_log): slots used,
+// whether the warm start was kept, the last used slot's errors and the errors
+// before the warm start.  One thread: the ring index advances on the device,
+// so graph replays record every solve.
+__global__ void k_log_solve(uint32_t n, const unsigned long long* __restrict__ counters,
+                            const uint32_t* __restrict__ ehist, const unsigned long long* __restrict__ esum,
+                            pbf_solve_record* __restrict__ log, unsigned long long* __restrict__ count) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const unsigned long long idx = *count;
+  const uint32_t k = (uint32_t)counters[C_ITERS];  // 1..kMaxSolveSlots
+  const uint32_t nn = n ? n : 1u;
+  pbf_solve_record r;
+  r.index = idx;
+  r.iterations = k;
+  r.warm_start_kept = (int32_t)counters[C_WARM_KEPT];
+  r.max_error = __uint_as_float(ehist[k]);
+  r.mean_error = mean_error(esum[k], nn);
+  r.max_error_before_warm_start = __uint_as_float(ehist[0]);
+  r.mean_error_before_warm_start = mean_error(esum[0], nn);
+  log[idx % PBF_SOLVE_LOG_SIZE] = r;
+  *count = idx + 1;
+}
+
+void launch_log_solve(pbf_context* c) {
+  hipLaunchKernelGGL(k_log_solve, dim3(1), dim3(64), 0, c->stream, (uint32_t)c->n,
+                     (const unsigned long long*)c->counters, (const uint32_t*)c->ehist,
+                     (const unsigned long long*)c->esum, c->slog, c->slog_count);
 }
 
 void launch_add_warm(pbf_context* c) {
@@ -728,23 +829,23 @@ void launch_add_warm(pbf_context* c) {
 }
 
 void launch_viscosity_first(pbf_context* c) {
-  PBF_LAUNCH_STAGED(c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
+  PBF_LAUNCH_ARITH(PBF_LAUNCH_STAGED, c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
              (const float4*)c->v, c->va, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb,
              c->cnb_count);
 }
 
 void launch_vorticity_masked(pbf_context* c) {
-  PBF_LAUNCH(c, "vorticity", k_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->va, c->scr,
+  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "vorticity", k_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->va, c->scr,
              (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 
 void launch_apply_vorticity_masked(pbf_context* c) {
-  PBF_LAUNCH(c, "apply_vorticity", k_apply_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->scr, c->va,
+  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "apply_vorticity", k_apply_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->scr, c->va,
              (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 
 void launch_viscosity_last(pbf_context* c) {
-  PBF_LAUNCH(c, "viscosity_positions", k_viscosity_last, c->K, (const float4*)c->xs, (const float*)c->rho,
+  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "viscosity_positions", k_viscosity_last, c->K, (const float4*)c->xs, (const float*)c->rho,
              (const float4*)c->va, c->v, c->x, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 

@@ -30,6 +30,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdint.h>
 
 #include <string>
@@ -68,6 +69,7 @@ struct KParams {
   int cap;
   int stride;                    // list row length: cap rounded up to a multiple of 8
   int count_overflow;            // PBF_OPT_COUNT_OVERFLOW
+  int fast;                      // PBF_OPT_FAST_MATH: hardware sqrt and reciprocal in the fused step
   uint32_t heavy_budget;         // candidates one lane scans before a wavefront takes over (see walk_list)
   // batched scenes: ncells is the total over the scenes, sid the scene of each slot
   int batched;
@@ -106,10 +108,18 @@ struct SolveCtl {
   const float4* xsave;         // position stars to restore when the warm start is rejected
 };
 
-// Features for the learned warm start (written by the first density pass)
+// The first density pass of a warm-started solve: the learned warm start's
+// features, kept in device memory when feat is set, and with wlam set the
+// network evaluated on them in the same pass:
+// wlam[p] = lprev[p] + scale * net(features of p).
 struct FeatCtl {
-  float* feat;          // [n][8] (null: no features)
+  float* feat;          // [n][8] (null: the features are not stored)
   const float* lprev;   // previous multiplier sums
+  const short* w1;      // bf16 [64][16] first layer (features 8..15 zero)
+  const short* w2;      // bf16 [64][64] second layer
+  const float* mp;      // fp32 parameters (biases, output layer)
+  float scale;
+  float* wlam;          // null: no network in this pass
 };
 
 // Neighbour lists are tiled by groups of 64 particles: entry k of particle p
@@ -187,6 +197,17 @@ struct pbf_context {
   std::vector<void*> graph_sig;  // buffer pointers when the graph was captured
   uint64_t version = 0;  // bumped by anything that changes the launches of a step
   bool capturing = false;
+  // captured pair of iterate-to-tolerance steps (pbf_solve_graph) and what it was captured for
+  hipGraph_t sgraph = nullptr;
+  hipGraphExec_t sgraph_exec = nullptr;
+  uint64_t sgraph_n = 0, sgraph_version = 0;
+  float sgraph_tol = 0.f, sgraph_gain = 0.f;
+  uint32_t sgraph_max_it = 0;
+  int sgraph_warm = -1;
+  std::vector<void*> sgraph_sig;
+  // per-solve records: a ring of PBF_SOLVE_LOG_SIZE and the number written so far
+  pbf_solve_record* slog = nullptr;
+  unsigned long long* slog_count = nullptr;
   pbf_tuning tune{256u, 768u, 8192u};
   unsigned long long* done_host = nullptr;  // pinned copy of C_DONE, polled between slot chunks
   float *lsum = nullptr, *lsuma = nullptr, *wlam = nullptr, *wgiven = nullptr;
@@ -203,6 +224,17 @@ namespace pbf {
 
 // error plumbing (pbf_api.cpp)
 int set_error(int code, const std::string& msg);
+
+// A roctx range over the host side of one entry point (shown by
+// rocprofv3 --marker-trace; a push and a pop that return at once when no
+// profiler is attached).
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
+#define PBF_TRACE(name) ::pbf::TraceRange pbf_trace_range_(name)
 int check_hip(hipError_t e, const char* what);
 
 // timing hooks around launches (pbf_api.cpp)
@@ -240,6 +272,7 @@ void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S);  // lam fr
 void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S);
 void launch_warm_dp(pbf_context* c);      // xt, wlam -> xs (warm start), xsave = xt
 void launch_add_warm(pbf_context* c);     // lsum += wlam when the warm start was kept
+void launch_log_solve(pbf_context* c);    // one pbf_solve_record of the solve just queued
 void launch_viscosity_first(pbf_context* c);                  // v -> va, compact lists of the final positions
 void launch_vorticity_masked(pbf_context* c);                 // va -> scr
 void launch_apply_vorticity_masked(pbf_context* c);           // va += ...
@@ -258,7 +291,6 @@ void launch_copy_f4(const float4* src, float4* dst, uint64_t n, hipStream_t s);
 void launch_copy_f1(const float* src, float* dst, uint64_t n, hipStream_t s);
 void launch_scale_f1(const float* src, float* dst, float scale, uint64_t n, hipStream_t s);
 // ---- learned warm start (pbf_mlp.hip) ----
-void launch_mlp_warm(pbf_context* c);  // feat -> wlam
 // original-order transfer helpers
 void launch_gather_out_f4(pbf_context* c, const float4* src, float* dst3, int comps);
 void launch_gather_out_f1(pbf_context* c, const float* src, float* dst);

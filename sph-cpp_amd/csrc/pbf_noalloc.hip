@@ -75,6 +75,7 @@ int scratch_for_current_device(uint32_t** out) {
 using namespace pbf;
 
 extern "C" int pbf_sort_by_key_u32_noalloc(uint32_t* d_keys, uint32_t* d_values, uint64_t n, uint32_t key_bits) {
+  PBF_TRACE("pbf_sort_by_key_u32_noalloc");
   if (n && (!d_keys || !d_values)) return set_error(PBF_ERR_ARG, "null argument");
   if (n > kMaxPairs)
     return set_error(PBF_ERR_CAPACITY, "pbf_sort_by_key_u32_noalloc: n = " + std::to_string(n) +

@@ -2,20 +2,27 @@
 
 Training pairs come from randomised dam breaks (seeded block sizes at box
 scales 1, 2 and 4 of the 3.2 x 1.6 x 1.6 box, jitter uniform in
-[0, 0.05 h]).  Each scene advances with the solves the warm start serves:
-iterate to a mean density error of ``tol``, warm-started from the previous
-substep's multiplier sums.  After every ``every``-th substep whose warm start
-was kept, the features of that solve's first density pass become samples,
-with target the change of the multiplier sum over the substep,
-(Lambda - Lambda_prev) / S: the network learns to extrapolate the previous
-sums to the current substep.  It is trained with the library's own MFMA
-forward/backward and Adam kernels (pbf_warm_model_train).  Afterwards the
-output layer is scaled by the least-squares gain of the output on samples of
-held-out scenes at the largest box scale (clamped to [0, 1]), so that only as
-much of the correction is applied as carries over to large scenes the network
-has not seen.
+[0, 0.05 h]).  With the default source, "plain50", each scene advances with
+the plain solver run to 50 Jacobi iterations per substep, and the pair of a
+particle is (features of the substep's first density pass, (Lambda_50 -
+Lambda_prev) / S): Lambda_50 is the sum of the multipliers the 50
+iterations applied (the multiplier the plain solve converges to), Lambda_prev
+the previous substep's, S = PBF_WARM_LAMBDA_SCALE.  The features come from a
+zero warm start, which moves nothing, so the step is the plain one.  The
+network thus learns how far the converged multiplier moves from one substep
+to the next.  Source "tolerance" instead advances the scene with the solves
+the warm start serves (iterate to a mean density error of ``tol``,
+warm-started from the previous sums) and takes the change of their sums.
+After every ``every``-th sampled substep the features become samples (a
+stride over the cell order).  The network is trained with the library's own
+MFMA forward/backward and Adam kernels (pbf_warm_model_train).  Afterwards
+the output layer is scaled by the least-squares gain of the output on samples
+of held-out scenes at the largest box scale (clamped to [0, 1]), so that only
+as much of the correction is applied as carries over to large scenes the
+network has not seen.
 
-    python sph-cpp_amd/train_warm.py [--pairs 2000000] [--epochs 40] [--out sph-cpp_amd/warm_model.npy]
+    python sph-cpp_amd/train_warm.py [--pairs 2000000] [--epochs 40] [--source plain50|tolerance]
+                                     [--out sph-cpp_amd/warm_model.npy]
 
 Prints one JSON line per scene and a summary; writes the 4801 parameters
 (float32, numpy .npy).
@@ -47,12 +54,15 @@ def random_scene(rng, cap=60, scale=None):
     amp = float(rng.uniform(0.0, 0.05 * scenes.H))
     seed = int(rng.integers(1 << 31))
     params, pos, vel = scenes.dam_break(counts, box, seed=seed, cap=cap, jitter_amplitude=amp)
-    params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
+    params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR | pbf.PBF_OPT_KEEP_WARM_FEATURES
     return params, pos, vel, {"scale": scale, "counts": counts, "jitter": round(amp, 6), "seed": seed}
 
 
+PLAIN_ITERATIONS = 50
+
+
 def generate(model, pairs, seed=0, steps=200, cap=60, tol=0.01, max_slots=128, per_sample=10_000,
-             scenes_min=8, log=print, budget=None, scale=None):
+             scenes_min=8, log=print, budget=None, scale=None, source="plain50"):
     """Add samples until the model holds `pairs`.  Each scene contributes at
     most `budget` samples (default: pairs / scenes_min), each sampled substep
     at most per_sample (a stride over the cell order), and the sampled
@@ -71,8 +81,15 @@ def generate(model, pairs, seed=0, steps=200, cap=60, tol=0.01, max_slots=128, p
         per_substep = (len(pos) + stride - 1) // stride
         every = max(2, steps * per_substep // budget)
         kept = 0
+        zero = np.zeros(len(pos), np.float32)
         for step in range(1, steps + 1):
-            t.solve(tol, max_slots, pbf.PBF_WARM_PREVIOUS if step > 1 else pbf.PBF_WARM_NONE)
+            if source == "plain50":
+                # the plain solver to 50 iterations; the zero warm start only
+                # computes the first density pass's features
+                t.set_warm_lambdas(zero)
+                t.solve(0.0, PLAIN_ITERATIONS, pbf.PBF_WARM_GIVEN if step > 1 else pbf.PBF_WARM_NONE)
+            else:
+                t.solve(tol, max_slots, pbf.PBF_WARM_PREVIOUS if step > 1 else pbf.PBF_WARM_NONE)
             if step > 1 and step % every == 0 and t.solve_info()["warm_start_kept"]:
                 model.add_from(t, stride)
                 kept += 1
@@ -114,10 +131,11 @@ def gain_from_losses(c, lh, l1):
     return float(min(max(g, 0.0), 1.0))
 
 
-def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print, scale=None, scenes_min=8):
+def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print, scale=None, scenes_min=8,
+          source="plain50"):
     model = pbf.WarmModel(pairs + 2 * VALIDATION_PAIRS)
     t0 = time.time()
-    generate(model, pairs, seed, log=log, scale=scale, scenes_min=scenes_min)
+    generate(model, pairs, seed, log=log, scale=scale, scenes_min=scenes_min, source=source)
     t1 = time.time()
     model.init(seed)
     n = model.sample_count()
@@ -132,13 +150,13 @@ def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print, sc
     # that does not carry over to new scenes shrinks towards zero and the warm
     # start towards the previous multiplier sums it corrects
     generate(model, n + VALIDATION_PAIRS, seed + 1000, log=log, budget=VALIDATION_PAIRS // VALIDATION_SCENES + 1,
-             scale=VALIDATION_SCALE)
+             scale=VALIDATION_SCALE, source=source)
     nv = model.sample_count() - n
     gain, vl = output_gain(model, n, nv)
     p = model.params()
     p[-(PBF_HIDDEN + 1):] *= np.float32(gain)
     model.set_params(p)
-    summary = {"samples": n, "epochs": epochs, "batch": batch, "loss_before": l0, "loss_after": l1,
+    summary = {"source": source, "samples": n, "epochs": epochs, "batch": batch, "loss_before": l0, "loss_after": l1,
                "mean_batch_loss": [m1, m2], "validation_samples": nv,
                "validation_loss_at_gain_0_half_1": vl, "output_gain": gain,
                "data_seconds": round(t1 - t0, 1), "train_seconds": round(t2 - t1, 1)}
@@ -155,10 +173,12 @@ def main(argv=None):
     ap.add_argument("--scale", type=int, default=0, choices=(0, 1, 2, 4),
                     help="box scale of the training scenes (0: 1, 2 and 4 at random)")
     ap.add_argument("--scenes", type=int, default=8, help="training scenes (each gives pairs / scenes samples)")
+    ap.add_argument("--source", default="plain50", choices=("plain50", "tolerance"),
+                    help="plain50: plain solves of 50 iterations; tolerance: warm-started tolerance solves")
     ap.add_argument("--out", default=MODEL_PATH)
     a = ap.parse_args(argv)
     model, summary = train(a.pairs, a.epochs, a.batch, a.lr, a.seed, log=lambda s: print(s, flush=True),
-                           scale=a.scale or None, scenes_min=a.scenes)
+                           scale=a.scale or None, scenes_min=a.scenes, source=a.source)
     np.save(a.out, model.params())
     print(json.dumps({"out": a.out, **summary}), flush=True)
     model.close()

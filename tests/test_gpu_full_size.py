@@ -57,6 +57,7 @@ def _assert_lists_equal(g, o, cap, what):
 
 def test_dam_break_1m_first_substeps_bit_exact(pbf, scenes):
     p, pos, vel = scenes.config_c2(seed=0)
+    p.options |= pbf.PBF_OPT_KEEP_WARM_FEATURES
     g = pbf.Particles(p)
     g.add(pos, pos, vel)
     o = _oracle(p, pos, pos, vel)
@@ -152,6 +153,7 @@ def test_dam_break_1m_learned_warm_start(pbf, scenes):
     if not os.path.exists(MODEL):
         pytest.skip("no trained model in the tree")
     p, pos, vel = scenes.config_c2(seed=0)
+    p.options |= pbf.PBF_OPT_KEEP_WARM_FEATURES
     g = pbf.Particles(p)
     g.add(pos, pos, vel)
     o = _oracle(p, pos, pos, vel)

@@ -171,3 +171,93 @@ def test_solve_stops_at_a_slot_with_density_fixes(pbf, scenes):
     assert g.stats()["density_hacks"] > 0
     _check(g, o, used, hist, lsum, kept, "stop at a slot with density fixes")
     g.close(); o.close()
+
+
+def _solve_state(g):
+    return g.positions(), g.velocities(), g.lambda_sums()
+
+
+@pytest.mark.parametrize("warm", ["none", "previous", "learned"])
+def test_solve_graph_equals_eager_solve(pbf, scenes, warm):
+    """pbf_solve_graph (two captured solves replayed, odd counts finished by
+    one queued solve) against pbf_solve, step by step: positions, velocities,
+    multiplier sums and the per-solve records, bit for bit.  The graph is
+    replayed several times with eager solves and host reads between the
+    replays, and every replay is checked."""
+    mode = {"none": pbf.PBF_WARM_NONE, "previous": pbf.PBF_WARM_PREVIOUS, "learned": pbf.PBF_WARM_LEARNED}[warm]
+    params, x, v = _state(scenes)
+    model = None
+    if mode == pbf.PBF_WARM_LEARNED:
+        path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sph-cpp_amd",
+                            "warm_model.npy")
+        model = pbf.WarmModel(16)
+        model.set_params(np.load(path, allow_pickle=False))
+    e = _gpu(pbf, params, x, v)
+    g = _gpu(pbf, params, x, v)
+    if model is not None:
+        e.set_warm_model(model)
+        g.set_warm_model(model)
+    # a tolerance the plain solve of the first step meets at its 11th slot
+    probe = _gpu(pbf, params, x, v)
+    probe.solve(0.0, 24)
+    tol, slots = float(probe.solve_info()["max_error_history"][10]), 24
+    probe.close()
+    infos = []
+    # graph calls of 2, 3 (2 + 1 queued), 4 and 2 steps, with eager solves between
+    plan = [("graph", 2), ("eager", 1), ("graph", 3), ("graph", 4), ("eager", 1), ("graph", 2)]
+    done = 0
+    for kind, k in plan:
+        for _ in range(k):
+            e.solve(tol, slots, mode)
+            infos.append(e.solve_info())
+        if kind == "graph":
+            g.solve_graph(tol, slots, mode, k)
+        else:
+            g.solve(tol, slots, mode)
+        done += k
+        what = f"{warm}: after {done} steps ({kind} {k})"
+        print(what, flush=True)
+        for a, b, name in zip(_solve_state(g), _solve_state(e), ("positions", "velocities", "sums")):
+            assert np.array_equal(a, b), what + ": " + name
+    log = g.solve_log(done)
+    assert [r["index"] for r in log] == list(range(done))
+    for r, info in zip(log, infos):
+        k = info["iterations"]
+        assert r["iterations"] == k
+        assert r["warm_start_kept"] == info["warm_start_kept"]
+        assert np.float32(r["max_error"]) == info["max_error_history"][k - 1]
+        assert np.float32(r["mean_error"]) == info["mean_error_history"][k - 1]
+    assert any(r["iterations"] < slots for r in log), "no solve ended before its last slot"
+    gi = g.solve_info()
+    assert gi["iterations"] == infos[-1]["iterations"]
+    assert np.array_equal(gi["max_error_history"], infos[-1]["max_error_history"])
+    e.close(); g.close()
+    if model is not None:
+        model.close()
+
+
+def test_solve_graph_recaptures_on_new_arguments(pbf, scenes):
+    """A new tolerance or slot limit gives a new graph: the results follow the
+    arguments of each call (checked against eager solves)."""
+    params, x, v = _state(scenes)
+    e = _gpu(pbf, params, x, v)
+    g = _gpu(pbf, params, x, v)
+    for tol, slots in ((0.05, 8), (0.02, 16), (0.02, 5), (0.05, 8)):
+        for _ in range(2):
+            e.solve(tol, slots)
+        g.solve_graph(tol, slots, pbf.PBF_WARM_NONE, 2)
+        for a, b in zip(_solve_state(g), _solve_state(e)):
+            assert np.array_equal(a, b), (tol, slots)
+    e.close(); g.close()
+
+
+def test_solve_graph_argument_errors(pbf, scenes):
+    params, x, v = _state(scenes, steps=0)
+    g = _gpu(pbf, params, x, v)
+    for args in ((-1.0, 4, 0, 2), (0.01, 0, 0, 2), (0.01, pbf.PBF_MAX_SOLVE_ITERATIONS + 1, 0, 2),
+                 (0.01, 4, 9, 2)):
+        with pytest.raises(pbf.PBFError):
+            g.solve_graph(*args)
+    with pytest.raises(pbf.PBFError):
+        g.solve_graph(0.01, 4, pbf.PBF_WARM_LEARNED, 2)  # no model
+    g.close()

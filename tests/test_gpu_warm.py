@@ -112,6 +112,7 @@ def _state(scenes, steps=10):
 
 def test_features_match_oracle(pbf, scenes):
     params, x, v = _state(scenes)
+    params.options |= pbf.PBF_OPT_KEEP_WARM_FEATURES
     o = O.Oracle(O.params_from_pbf(params), mode=O.JACOBI, threads=THREADS)
     o.set_state(x, x, v)
     g = pbf.Particles(params)
@@ -132,11 +133,17 @@ def test_features_match_oracle(pbf, scenes):
 
 
 def test_learned_equals_given_prediction(pbf, scenes):
-    """Lambda = Lambda_prev + S * net(features), fed back as a given warm start."""
+    """Lambda = Lambda_prev + S * net(features), fed back as a given warm
+    start: the network evaluated inside the density pass (k_density_lambda_c)
+    equals the separate forward kernel (pbf_warm_model_predict) on the stored
+    features, bit for bit.  A context that keeps no features (the network's
+    input stays in registers) gives the same step."""
     params, x, v = _state(scenes)
+    keep = pbf.Params.from_buffer_copy(params)
+    keep.options |= pbf.PBF_OPT_KEEP_WARM_FEATURES
     m = pbf.WarmModel(len(x))
     m.init(5)
-    a = pbf.Particles(params)
+    a = pbf.Particles(keep)
     a.add(x, x, v)
     a.solve(0.0, 3)  # previous sums for the learned solve
     prev = a.lambda_sums()
@@ -144,17 +151,27 @@ def test_learned_equals_given_prediction(pbf, scenes):
     b = pbf.Particles(params)
     b.add(x, x, v)
     b.solve(0.0, 3)
+    c = pbf.Particles(params)
+    c.add(x, x, v)
+    c.solve(0.0, 3)
     a.set_warm_model(m)
     a.solve(0.0, 6, pbf.PBF_WARM_LEARNED)
+    c.set_warm_model(m)
+    c.solve(0.0, 6, pbf.PBF_WARM_LEARNED)
+    with pytest.raises(pbf.PBFError):
+        c.warm_features()  # not kept
     warm = prev + np.float32(pbf.PBF_WARM_LAMBDA_SCALE) * m.predict(a.warm_features())
     b.set_warm_lambdas(warm.astype(np.float32))
     b.solve(0.0, 6, pbf.PBF_WARM_GIVEN)
-    ia, ib = a.solve_info(), b.solve_info()
-    assert ia["iterations"] == ib["iterations"] and ia["warm_start_kept"] == ib["warm_start_kept"]
+    ia, ib, ic = a.solve_info(), b.solve_info(), c.solve_info()
+    assert ia["iterations"] == ib["iterations"] == ic["iterations"]
+    assert ia["warm_start_kept"] == ib["warm_start_kept"] == ic["warm_start_kept"]
     assert np.array_equal(ia["max_error_history"], ib["max_error_history"])
+    assert np.array_equal(ia["max_error_history"], ic["max_error_history"])
     for k in ("positions", "velocities", "lambda_sums"):
         assert np.array_equal(getattr(a, k)(), getattr(b, k)()), k
-    a.close(); b.close(); m.close()
+        assert np.array_equal(getattr(a, k)(), getattr(c, k)()), k + " (features not kept)"
+    a.close(); b.close(); c.close(); m.close()
 
 
 def test_mean_error_criterion_matches_oracle(pbf, scenes):

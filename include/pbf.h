@@ -88,11 +88,16 @@ typedef struct pbf_params {
  * learned warm start evaluates its network inside that pass. */
 #define PBF_OPT_KEEP_WARM_FEATURES 4u
 /* The fused step (pbf_step, pbf_step_graph, pbf_solve, pbf_solve_graph) takes the
- * hardware square root and reciprocal (within about 1 ulp) in its neighbour sums in
- * place of the correctly rounded square root and divide.  Cell keys, particle order
+ * hardware square root and reciprocal (within about 1 ulp) in the neighbour sums of its
+ * pressure solve (densities, multipliers, position updates) in place of the correctly
+ * rounded square root and divide; XSPH and vorticity stay exact.  Cell keys, particle order
  * and neighbour lists stay bit-exact; positions then match the exact mode within the
  * tolerance tests/test_gpu_fast_math.py states.  The per-pass entry points stay exact. */
 #define PBF_OPT_FAST_MATH 8u
+/* A warm-started pbf_solve falls back to a zero warm start when either density error
+ * measure grows over the warm update (the max and the mean), not only the one the
+ * tolerance applies to. */
+#define PBF_OPT_WARM_STRICT 16u
 
 typedef struct pbf_context pbf_context;
 
@@ -124,8 +129,9 @@ int pbf_create(const pbf_params* params, pbf_context** out);
 int pbf_destroy(pbf_context* ctx);
 
 /* Replace the parameters (the renderer re-broadcasts them,
- * Simulation/Source/main.cpp:62-63).  Capacity may not change; a new boundary or bin
- * spacing re-sizes the grid. */
+ * Simulation/Source/main.cpp:62-63).  Capacity may not change and max_neighbors may not
+ * exceed the value the context was created with; a new boundary or bin spacing re-sizes
+ * the grid (in a batched context, every scene's grid). */
 int pbf_set_params(pbf_context* ctx, const pbf_params* params);
 int pbf_get_params(const pbf_context* ctx, pbf_params* out);
 
@@ -340,6 +346,28 @@ typedef struct pbf_tuning {
 } pbf_tuning;
 int pbf_set_tuning(pbf_context* ctx, const pbf_tuning* tuning);
 int pbf_get_tuning(const pbf_context* ctx, pbf_tuning* out);
+
+/* No counterpart in the reference.  Picks the bin spacing (cell size and list radius)
+ * and the list cap for speed, for callers that do not need the reference's list layout
+ * (bin spacing 1.2 h, neighbors.h:198, cap 60, neighbors.h:39).  Candidate spacings are
+ * h, 1.1 h and 1.2 h.  For each, one list build from the current state with the largest
+ * cap the context holds gives the longest list; a spacing whose lists do not fit is
+ * dropped, and the others get a cap of 1.25x that longest list (rows of 8, at most the
+ * created cap).  Every candidate therefore lists every neighbour within h, and every
+ * neighbour sum runs over the same particles as without a cap (entries between h and the
+ * spacing add exactly zero): layouts differ only in the order of the sums.  Each is timed
+ * over `steps` fused steps from the current state, the state is restored, and the fastest
+ * is applied with pbf_set_params (read it back with pbf_get_params).  A cap that later
+ * overflows is counted with PBF_OPT_COUNT_OVERFLOW.  The reference's own layout
+ * truncates lists under compression, so a tuned layout changes results against it:
+ * leave the layout alone where results must match the reference's. */
+typedef struct pbf_layout {
+  float neighbor_bin_spacing;
+  uint32_t max_neighbors;
+  double ms_per_step;      /* of the chosen layout */
+  uint32_t longest_list;   /* at the chosen spacing, on the state it was tuned on */
+} pbf_layout;
+int pbf_autotune_layout(pbf_context* ctx, uint32_t steps, pbf_layout* best);
 int pbf_autotune(pbf_context* ctx, uint32_t steps, pbf_tuning* best, double* best_ms_per_step);
 
 /* Block until the context's queued work is done. */

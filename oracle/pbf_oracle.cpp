@@ -636,14 +636,19 @@ struct Sim {
     find_neighbors(0, n, 0, n);
     std::vector<R> acc(n, (R)0);
     std::vector<V3<R>> saved;
-    R e_pre = (R)0;
+    R e_pre = (R)0, o_pre = (R)0;
     int keep = 0;
-    auto error = [&]() -> R { return use_mean ? (R)emean_fixed(0, n) : emax_r(0, n); };
+    // use_mean bit 0: the tolerance applies to the mean; bit 1: strict fallback
+    // (the other measure must not grow over the warm update either)
+    const bool mean = (use_mean & 1) != 0, strict = (use_mean & 2) != 0;
+    auto error = [&]() -> R { return mean ? (R)emean_fixed(0, n) : emax_r(0, n); };
+    auto other = [&]() -> R { return mean ? emax_r(0, n) : (R)emean_fixed(0, n); };
     if (warm) {
       std::vector<float> f;
       density_features(f);
       if (feat) *feat = f;
       e_pre = error();
+      o_pre = other();
       saved = xs;
       lam.assign(warm, warm + n);
       compute_pressure_dps(0, n);
@@ -658,7 +663,7 @@ struct Sim {
       if (hist_max) hist_max[it] = (float)emax_r(0, n);
       if (hist_mean) hist_mean[it] = emean_fixed(0, n);
       used = it + 1;
-      if (warm && it == 0 && e > e_pre) {
+      if (warm && it == 0 && (e > e_pre || (strict && other() > o_pre))) {
         xs = saved;
         keep = 0;
         continue;

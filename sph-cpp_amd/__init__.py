@@ -97,6 +97,7 @@ PBF_WARM_NONE, PBF_WARM_GIVEN, PBF_WARM_PREVIOUS, PBF_WARM_LEARNED = 0, 1, 2, 3
 PBF_OPT_SOLVE_MEAN_ERROR = 2
 PBF_OPT_KEEP_WARM_FEATURES = 4
 PBF_OPT_FAST_MATH = 8
+PBF_OPT_WARM_STRICT = 16
 PBF_WARM_FEATURES = 8
 PBF_WARM_HIDDEN = 64
 PBF_WARM_PARAMS = 4801
@@ -116,6 +117,11 @@ class SolveInfo(C.Structure):
         ("max_error_history", C.c_float * PBF_MAX_SOLVE_ITERATIONS),
         ("mean_error_history", C.c_float * PBF_MAX_SOLVE_ITERATIONS),
     ]
+
+
+class Layout(C.Structure):
+    _fields_ = [("neighbor_bin_spacing", C.c_float), ("max_neighbors", C.c_uint32), ("ms_per_step", C.c_double),
+                ("longest_list", C.c_uint32)]
 
 
 class SolveRecord(C.Structure):
@@ -182,6 +188,7 @@ def lib():
     _sig(L, "pbf_set_tuning", [_P, C.POINTER(Tuning)])
     _sig(L, "pbf_get_tuning", [_P, C.POINTER(Tuning)])
     _sig(L, "pbf_autotune", [_P, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_double)])
+    _sig(L, "pbf_autotune_layout", [_P, C.c_uint32, C.POINTER(Layout)])
     _sig(L, "pbf_create_batched", [C.POINTER(Params), C.c_uint32, _F, C.POINTER(_P)])
     _sig(L, "pbf_add_to_scene", [_P, C.c_uint32, _F, _F, _F, C.c_uint64])
     L.pbf_num_scenes.argtypes = [_P]
@@ -313,8 +320,9 @@ class Particles:
         _check(lib().pbf_remove(self._ctx, int(count)))
 
     def set_params(self, params: Params) -> None:
-        self.params = params.copy()
-        _check(lib().pbf_set_params(self._ctx, C.byref(self.params)))
+        p = params.copy()
+        _check(lib().pbf_set_params(self._ctx, C.byref(p)))
+        self.params = p
 
     def _get(self, name: str, comps: int) -> np.ndarray:
         n = self.local_count()
@@ -484,6 +492,16 @@ class Particles:
         ms = C.c_double(0.0)
         _check(lib().pbf_autotune(self._ctx, int(steps), C.byref(t), C.byref(ms)))
         return {**{k: getattr(t, k) for k, _ in Tuning._fields_}, "ms_per_step": ms.value}
+
+    def autotune_layout(self, steps: int = 3) -> dict:
+        """pbf_autotune_layout: pick the bin spacing and list cap (changes the
+        list layout against the reference's; see include/pbf.h)."""
+        lay = Layout()
+        _check(lib().pbf_autotune_layout(self._ctx, int(steps), C.byref(lay)))
+        p = Params()
+        _check(lib().pbf_get_params(self._ctx, C.byref(p)))
+        self.params = p
+        return {k: getattr(lay, k) for k, _ in Layout._fields_}
 
     def step_graph(self, nsteps: int = 1) -> None:
         """pbf_step through a captured HIP graph of two steps."""

@@ -297,6 +297,37 @@ static std::vector<void*> buffer_sig(const pbf_context* c) {
           c->oid, c->oida, c->key, c->keya, c->lsum, c->lsuma, c->sid, c->sida, (void*)(uintptr_t)c->track_lsum};
 }
 
+// The grid of every scene of a batched context (boxes: 3 extents per scene)
+// and the total cell count; scene s takes the keys [off_s, off_s + ncells_s).
+static int build_scene_geo(const pbf_params& params, const float* boxes, uint32_t num_scenes,
+                           std::vector<SceneGeo>& geo, uint64_t& total) {
+  geo.assign(num_scenes, SceneGeo{});
+  total = 0;
+  for (uint32_t sc = 0; sc < num_scenes; ++sc) {
+    pbf_params ps = params;
+    for (int i = 0; i < 3; ++i) {
+      if (!(boxes[3 * sc + i] > 0.0f)) return set_error(PBF_ERR_ARG, "scene boxes must be positive");
+      ps.boundary_max[i] = boxes[3 * sc + i];
+    }
+    KParams Ks;
+    uint64_t d[3];
+    int rc = compute_kparams(ps, Ks, d);
+    if (rc) return rc;
+    SceneGeo& g = geo[sc];
+    std::memset(&g, 0, sizeof(g));
+    g.dimx = Ks.dimx;
+    g.dimy = Ks.dimy;
+    g.dimz = Ks.dimz;
+    g.off = (uint32_t)total;
+    g.ncells = Ks.ncells;
+    g.dxy = Ks.dxy;
+    for (int i = 0; i < 3; ++i) g.bmax[i] = Ks.bmax[i];
+    total += Ks.ncells;
+    if (total >= 4000000000ull) return set_error(PBF_ERR_UNSUPPORTED, "scenes hold more than 4e9 cells");
+  }
+  return PBF_OK;
+}
+
 }  // namespace pbf
 
 using namespace pbf;
@@ -337,6 +368,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   }
   c->ncells = c->K.ncells;
   c->cap = c->K.cap;
+  c->cap_alloc = c->K.cap;
   c->capacity = params->max_particles_local;
   if ((rc = check_hip(hipGetDevice(&c->device), "hipGetDevice")) ||
       (rc = check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate"))) {
@@ -417,21 +449,42 @@ int pbf_destroy(pbf_context* c) {
 int pbf_set_params(pbf_context* c, const pbf_params* params) {
   PBF_TRACE("pbf_set_params");
   if (!c || !params) return set_error(PBF_ERR_ARG, "null argument");
-  if (params->max_particles_local != c->capacity || (int)params->max_neighbors != c->cap)
-    return set_error(PBF_ERR_ARG, "capacity and max_neighbors are fixed at pbf_create");
+  if (params->max_particles_local != c->capacity)
+    return set_error(PBF_ERR_ARG, "capacity is fixed at pbf_create");
+  if ((int)params->max_neighbors > c->cap_alloc)
+    return set_error(PBF_ERR_ARG, "max_neighbors may not exceed the cap the context was created with");
   KParams K;
   uint64_t dims[3];
   int rc = compute_kparams(*params, K, dims);
   if (rc) return rc;
   if (c->K.batched) {
-    if (K.s != c->K.s) return set_error(PBF_ERR_UNSUPPORTED, "a batched context keeps its bin spacing");
+    if (K.s != c->K.s) {
+      // new bin spacing: every scene's grid is derived again from its box
+      std::vector<float> boxes(3 * (size_t)c->nscenes);
+      for (uint32_t sc = 0; sc < c->nscenes; ++sc)
+        for (int i = 0; i < 3; ++i) boxes[3 * sc + i] = c->scene_geo[sc].bmax[i];
+      std::vector<SceneGeo> geo;
+      uint64_t total = 0;
+      if ((rc = build_scene_geo(*params, boxes.data(), c->nscenes, geo, total))) return rc;
+      if ((rc = check_hip(hipStreamSynchronize(c->stream), "sync")) ||
+          (rc = check_hip(hipMemcpy(c->scenes, geo.data(), geo.size() * sizeof(SceneGeo), hipMemcpyHostToDevice),
+                          "upload")))
+        return rc;
+      c->scene_geo = geo;
+      c->K.ncells = (uint32_t)total;
+      c->dims[0] = geo[0].dimx;
+      c->dims[1] = geo[0].dimy;
+      c->dims[2] = geo[0].dimz;
+    }
     keep_scenes(c, K);
     for (int i = 0; i < 3; ++i) dims[i] = c->dims[i];
   }
-  const bool grid_changed = K.ncells != c->K.ncells || K.s != c->K.s || K.dimx != c->K.dimx || K.dimy != c->K.dimy;
+  const bool grid_changed = K.ncells != c->K.ncells || K.s != c->K.s || K.dimx != c->K.dimx ||
+                            K.dimy != c->K.dimy || K.cap != c->K.cap;
   K.heavy_budget = c->tune.heavy_budget;
   c->P = *params;
   c->K = K;
+  c->cap = K.cap;
   ++c->version;
   for (int i = 0; i < 3; ++i) c->dims[i] = dims[i];
   c->ncells = K.ncells;
@@ -502,30 +555,9 @@ int pbf_create_batched(const pbf_params* params, uint32_t num_scenes, const floa
   if (!params || !boxes || !out) return set_error(PBF_ERR_ARG, "null argument");
   *out = nullptr;
   if (num_scenes == 0) return set_error(PBF_ERR_ARG, "num_scenes must be positive");
-  std::vector<SceneGeo> geo(num_scenes);
+  std::vector<SceneGeo> geo;
   uint64_t total = 0;
-  for (uint32_t sc = 0; sc < num_scenes; ++sc) {
-    pbf_params ps = *params;
-    for (int i = 0; i < 3; ++i) {
-      if (!(boxes[3 * sc + i] > 0.0f)) return set_error(PBF_ERR_ARG, "scene boxes must be positive");
-      ps.boundary_max[i] = boxes[3 * sc + i];
-    }
-    KParams Ks;
-    uint64_t d[3];
-    int rc = compute_kparams(ps, Ks, d);
-    if (rc) return rc;
-    SceneGeo& g = geo[sc];
-    std::memset(&g, 0, sizeof(g));
-    g.dimx = Ks.dimx;
-    g.dimy = Ks.dimy;
-    g.dimz = Ks.dimz;
-    g.off = (uint32_t)total;
-    g.ncells = Ks.ncells;
-    g.dxy = Ks.dxy;
-    for (int i = 0; i < 3; ++i) g.bmax[i] = Ks.bmax[i];
-    total += Ks.ncells;
-    if (total >= 4000000000ull) return set_error(PBF_ERR_UNSUPPORTED, "scenes hold more than 4e9 cells");
-  }
+  if (int rc = build_scene_geo(*params, boxes, num_scenes, geo, total)) return rc;
   pbf_params p0 = *params;
   for (int i = 0; i < 3; ++i) p0.boundary_max[i] = boxes[i];
   pbf_context* c = nullptr;
@@ -870,7 +902,8 @@ static int solve_body(pbf_context* c, float tolerance, uint32_t max_iterations, 
   if (warm_start == PBF_WARM_PREVIOUS) launch_copy_f1(c->lsuma, c->wlam, n, c->stream);
   else if (warm_start == PBF_WARM_GIVEN) launch_scatter_in_f1(c, c->wgiven, c->wlam);
   const int mean = (c->P.options & PBF_OPT_SOLVE_MEAN_ERROR) ? 1 : 0;
-  const SolveCtl S{tolerance, warm, mean, c->ehist, c->esum, c->lsum, c->xsave};
+  const int strict = (c->P.options & PBF_OPT_WARM_STRICT) ? 1 : 0;
+  const SolveCtl S{tolerance, warm, mean, c->ehist, c->esum, c->lsum, c->xsave, strict};
   if (warm) {
     // first density pass (its error is the reference for the fallback; it
     // writes the features), then the warm update
@@ -1112,46 +1145,79 @@ int pbf_get_tuning(const pbf_context* c, pbf_tuning* out) {
 // original ids, and the scene ids and multiplier sums where kept), then
 // restores that state: the next step sorts by (cell, original id) and
 // recomputes everything else, so it is the step an untuned run would take.
+// The state a step starts from (positions, velocities, original ids, and
+// the scene ids and multiplier sums where kept), saved on the device so that
+// candidates can each be timed from it: the next step sorts by (cell,
+// original id) and recomputes everything else, so after restore() it is the
+// step an untuned run would take.
+struct SavedState {
+  pbf_context* c = nullptr;
+  uint64_t n = 0;
+  float4 *x = nullptr, *v = nullptr;
+  uint32_t *oid = nullptr, *sid = nullptr;
+  float* lsum = nullptr;
+  ~SavedState() {
+    dfree(x);
+    dfree(v);
+    dfree(oid);
+    dfree(sid);
+    dfree(lsum);
+  }
+  int copy(void* d, const void* s, uint64_t bytes) {
+    return check_hip(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream), "autotune copy");
+  }
+  int save(pbf_context* ctx) {
+    c = ctx;
+    n = c->n;
+    int r;
+    if ((r = dalloc(&x, n)) || (r = dalloc(&v, n)) || (r = dalloc(&oid, n)) || (r = dalloc(&sid, n)) ||
+        (r = dalloc(&lsum, n)))
+      return r;
+    r = copy(x, c->x, n * 16);
+    if (!r) r = copy(v, c->v, n * 16);
+    if (!r) r = copy(oid, c->oid, n * 4);
+    if (!r && c->K.batched) r = copy(sid, c->sid, n * 4);
+    if (!r && c->track_lsum) r = copy(lsum, c->lsum, n * 4);
+    return r;
+  }
+  int restore() {
+    int r = copy(c->x, x, n * 16);
+    if (!r) r = copy(c->v, v, n * 16);
+    if (!r) r = copy(c->oid, oid, n * 4);
+    if (!r && c->K.batched) r = copy(c->sid, sid, n * 4);
+    if (!r && c->track_lsum) r = copy(c->lsum, lsum, n * 4);
+    c->lists_valid = false;
+    return r;
+  }
+};
+
+// Mean ms of `steps` fused steps from the saved state (one untimed step first).
+static int time_steps(pbf_context* c, SavedState& S, uint32_t steps, hipEvent_t a, hipEvent_t b, double* ms_out) {
+  int rc;
+  if ((rc = S.restore())) return rc;
+  step_body(c);  // first launches of this setting, untimed
+  if ((rc = S.restore())) return rc;
+  hipEventRecord(a, c->stream);
+  for (uint32_t st = 0; st < steps; ++st) step_body(c);
+  hipEventRecord(b, c->stream);
+  if ((rc = check_hip(hipEventSynchronize(b), "autotune"))) return rc;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, a, b);
+  *ms_out = (double)ms / steps;
+  return PBF_OK;
+}
+
 int pbf_autotune(pbf_context* c, uint32_t steps, pbf_tuning* best, double* best_ms) {
   PBF_TRACE("pbf_autotune");
   if (!c) return set_error(PBF_ERR_ARG, "null context");
   if (steps == 0) return set_error(PBF_ERR_ARG, "steps must be positive");
-  const uint64_t n = c->n;
-  if (!n) return set_error(PBF_ERR_STATE, "no particles");
-  float4 *sx = nullptr, *sv = nullptr;
-  uint32_t *so = nullptr, *ss = nullptr;
-  float* sl = nullptr;
-  int rc;
-  if ((rc = dalloc(&sx, n)) || (rc = dalloc(&sv, n)) || (rc = dalloc(&so, n)) || (rc = dalloc(&ss, n)) ||
-      (rc = dalloc(&sl, n))) {
-    dfree(sx); dfree(sv); dfree(so); dfree(ss); dfree(sl);
-    return rc;
-  }
-  auto copy = [&](void* d, const void* s, uint64_t bytes) {
-    return check_hip(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream), "autotune copy");
-  };
-  auto save = [&]() {
-    int r = copy(sx, c->x, n * 16);
-    if (!r) r = copy(sv, c->v, n * 16);
-    if (!r) r = copy(so, c->oid, n * 4);
-    if (!r && c->K.batched) r = copy(ss, c->sid, n * 4);
-    if (!r && c->track_lsum) r = copy(sl, c->lsum, n * 4);
-    return r;
-  };
-  auto restore = [&]() {
-    int r = copy(c->x, sx, n * 16);
-    if (!r) r = copy(c->v, sv, n * 16);
-    if (!r) r = copy(c->oid, so, n * 4);
-    if (!r && c->K.batched) r = copy(c->sid, ss, n * 4);
-    if (!r && c->track_lsum) r = copy(c->lsum, sl, n * 4);
-    c->lists_valid = false;
-    return r;
-  };
+  if (!c->n) return set_error(PBF_ERR_STATE, "no particles");
+  SavedState S;
+  int rc = S.save(c);
   const pbf_tuning start = c->tune;
   pbf_tuning best_t = start;
   double best_v = 1e300;
   hipEvent_t a = nullptr, b = nullptr;
-  rc = save();
   if (!rc) rc = check_hip(hipEventCreate(&a), "hipEventCreate");
   if (!rc) rc = check_hip(hipEventCreate(&b), "hipEventCreate");
   const uint32_t blocks[] = {64u, 128u, 256u};
@@ -1162,30 +1228,100 @@ int pbf_autotune(pbf_context* c, uint32_t steps, pbf_tuning* best, double* best_
       for (uint32_t hk : hblocks) {
         if (rc) break;
         const pbf_tuning t{bs, hb, hk};
-        if ((rc = pbf_set_tuning(c, &t)) || (rc = restore())) break;
-        step_body(c);  // first launches of this setting, untimed
-        if ((rc = restore())) break;
-        hipEventRecord(a, c->stream);
-        for (uint32_t st = 0; st < steps; ++st) step_body(c);
-        hipEventRecord(b, c->stream);
-        if ((rc = check_hip(hipEventSynchronize(b), "autotune"))) break;
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, a, b);
-        if (ms / steps < best_v) {
-          best_v = ms / steps;
+        double ms = 0.0;
+        if ((rc = pbf_set_tuning(c, &t)) || (rc = time_steps(c, S, steps, a, b, &ms))) break;
+        if (ms < best_v) {
+          best_v = ms;
           best_t = t;
         }
       }
-  if (!rc) rc = restore();
+  if (!rc) rc = S.restore();
   if (!rc) rc = pbf_set_tuning(c, &best_t);
   if (!rc) rc = check_hip(hipStreamSynchronize(c->stream), "autotune");
   if (rc) pbf_set_tuning(c, &start);
   if (a) hipEventDestroy(a);
   if (b) hipEventDestroy(b);
-  dfree(sx); dfree(sv); dfree(so); dfree(ss); dfree(sl);
   if (rc) return rc;
   if (best) *best = best_t;
   if (best_ms) *best_ms = best_v;
+  return PBF_OK;
+}
+
+static int download_u32(pbf_context* c, const uint32_t* src, uint64_t count, std::vector<uint32_t>& out);
+
+// Largest list length of the last list build (host read of the counts).
+static int max_list_length(pbf_context* c, uint32_t* out) {
+  std::vector<uint32_t> cnt;
+  if (int rc = download_u32(c, (const uint32_t*)c->nbr_count, c->n, cnt)) return rc;
+  uint32_t m = 0;
+  for (uint32_t v : cnt) m = std::max(m, v);
+  *out = m;
+  return PBF_OK;
+}
+
+int pbf_autotune_layout(pbf_context* c, uint32_t steps, pbf_layout* best) {
+  PBF_TRACE("pbf_autotune_layout");
+  if (!c) return set_error(PBF_ERR_ARG, "null context");
+  if (steps == 0) return set_error(PBF_ERR_ARG, "steps must be positive");
+  if (!c->n) return set_error(PBF_ERR_STATE, "no particles");
+  SavedState S;
+  int rc = S.save(c);
+  const pbf_params P0 = c->P;
+  const float h = P0.smoothing_radius;
+  pbf_layout best_l{P0.neighbor_bin_spacing, P0.max_neighbors, 1e300, 0u};
+  bool found = false;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (!rc) rc = check_hip(hipEventCreate(&a), "hipEventCreate");
+  if (!rc) rc = check_hip(hipEventCreate(&b), "hipEventCreate");
+  const float factors[] = {1.0f, 1.1f, 1.2f};
+  for (float f : factors) {
+    if (rc) break;
+    const float spacing = h * f;
+    // the longest list at this spacing with the largest cap the context holds,
+    // the cap's rejections counted
+    pbf_params P = P0;
+    P.neighbor_bin_spacing = spacing;
+    P.max_neighbors = (uint32_t)c->cap_alloc;
+    P.options |= PBF_OPT_COUNT_OVERFLOW;
+    if ((rc = pbf_set_params(c, &P)) || (rc = S.restore())) break;
+    launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
+    launch_forces_predict(c, F_FORCES | F_PREDICT, 0, c->n);
+    find_neighbors_impl(c, c->n, 0, c->n);
+    unsigned long long over = 0;
+    uint32_t longest = 0;
+    if ((rc = check_hip(hipMemcpyAsync(&over, c->counters + C_OVERFLOW, sizeof(over), hipMemcpyDeviceToHost,
+                                       c->stream), "autotune")) ||
+        (rc = max_list_length(c, &longest)))
+      break;
+    if (over != 0) continue;  // this spacing needs a longer list than the context holds
+    // room for 25% longer lists than the state's longest, in rows of 8
+    uint32_t cap = (uint32_t)std::ceil(1.25 * (double)longest);
+    cap = std::min<uint32_t>((cap + 7u) / 8u * 8u, (uint32_t)c->cap_alloc);
+    cap = std::max<uint32_t>(cap, std::max<uint32_t>(longest, 1u));
+    P.max_neighbors = cap;
+    P.options = P0.options;
+    double ms = 0.0;
+    if ((rc = pbf_set_params(c, &P)) || (rc = time_steps(c, S, steps, a, b, &ms))) break;
+    if (ms < best_l.ms_per_step) {
+      best_l = pbf_layout{spacing, cap, ms, longest};
+      found = true;
+    }
+  }
+  pbf_params chosen = P0;
+  if (!rc && found) {
+    chosen.neighbor_bin_spacing = best_l.neighbor_bin_spacing;
+    chosen.max_neighbors = best_l.max_neighbors;
+  }
+  // whatever happened above, leave the context on a valid layout and its state
+  const int rc2 = pbf_set_params(c, rc ? &P0 : &chosen);
+  const int rc3 = S.restore();
+  if (!rc) rc = rc2 ? rc2 : rc3;
+  if (!rc) rc = check_hip(hipStreamSynchronize(c->stream), "autotune_layout");
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (rc) return rc;
+  if (!found) return set_error(PBF_ERR_CAPACITY, "no bin spacing keeps every neighbour within the list cap");
+  if (best) *best = best_l;
   return PBF_OK;
 }
 

@@ -120,11 +120,13 @@ __device__ __forceinline__ f3 spiky_in(f3 d, float r, const KParams& K) {
   const float s = K.spiky_norm * (K.h - r) * (K.h - r);
   return div3_shared(smul3(s, d), r + K.r_eps);
 }
-// The neighbour sums of the fused step in either arithmetic: FAST (the
+// The pressure-solve sums of the fused step in either arithmetic: FAST (the
 // PBF_OPT_FAST_MATH mode) takes the hardware square root and reciprocal
 // (v_sqrt_f32, v_rcp_f32, within about 1 ulp) in place of the correctly
 // rounded square root and divide; otherwise the operations are exactly the
-// reference's.
+// reference's.  (The XSPH and vorticity passes stay exact in both modes: with
+// the hardware operations they measured slower, 110 -> 132 us for the
+// vorticity pass at 1M, from a different register allocation.)
 template <bool FAST>
 __device__ __forceinline__ float nb_sqrt(float d2) {
   if constexpr (FAST) return __builtin_amdgcn_sqrtf(d2);
@@ -138,11 +140,6 @@ __device__ __forceinline__ f3 nb_spiky(f3 d, float r, const KParams& K) {
   } else {
     return spiky_in(d, r, K);
   }
-}
-template <bool FAST>
-__device__ __forceinline__ f3 nb_div3(f3 a, float b) {
-  if constexpr (FAST) return mul3(a, __builtin_amdgcn_rcpf(b));
-  else return div3(a, b);
 }
 
 // C_Spline<float,3>::operator() (kernels.h:220-228); the first branch is in double

@@ -452,10 +452,18 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
   if (S.ehist) {
     const float e = S.mean ? mean_error(S.esum[it + 1], n) : __uint_as_float(S.ehist[it + 1]);
     const float e_pre = S.mean ? mean_error(S.esum[0], n) : __uint_as_float(S.ehist[0]);
-    if (S.warm && it == 0 && e > e_pre) {
-      if (first) counters[C_WARM_KEPT] = 0ull;
-      if (p < n) xs[p] = S.xsave[p];
-      return;
+    if (S.warm && it == 0) {
+      bool grew = e > e_pre;
+      if (S.strict) {  // the other error measure must not grow either
+        const float o = S.mean ? __uint_as_float(S.ehist[1]) : mean_error(S.esum[1], n);
+        const float o_pre = S.mean ? __uint_as_float(S.ehist[0]) : mean_error(S.esum[0], n);
+        grew = grew || o > o_pre;
+      }
+      if (grew) {
+        if (first) counters[C_WARM_KEPT] = 0ull;
+        if (p < n) xs[p] = S.xsave[p];
+        return;
+      }
     }
     if (S.tol >= 0.0f && e <= S.tol) {
       if (first) {
@@ -559,7 +567,7 @@ __global__ void k_add_warm(uint32_t n, const float* __restrict__ wlam, float* __
 
 // XSPH sum (particles.h:441-444) over a compact row, from global memory
 // (FROM_LDS = false: cnb, tiled like the lists) or from the lane's LDS column
-template <bool FROM_LDS, bool FAST>
+template <bool FROM_LDS>
 __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_t n, uint32_t lane, int nc,
                                        uint32_t p, f3 xp, f3 vp, const KParams& K, const float4* __restrict__ xs,
                                        const float* __restrict__ rho, const float4* __restrict__ vin) {
@@ -581,8 +589,8 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= nc) continue;
-      const float r = nb_sqrt<FAST>(magsq3(sub3(xp, xyz(t[u]))));
-      acc3(dv, nb_div3<FAST>(mul3(sub3(xyz(vq[u]), vp), poly6_in(r, K)), rq[u]));
+      const float r = sqrt_cr(magsq3(sub3(xp, xyz(t[u]))));
+      acc3(dv, div3(mul3(sub3(xyz(vq[u]), vp), poly6_in(r, K)), rq[u]));
     }
   }
   return dv;
@@ -591,7 +599,6 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
 // ---------------------------------------------------------------------------
 // apply_viscosity (particles.h:436-447), first call: walks the whole list and
 // leaves the compact rows of the final positions for the passes that follow.
-template <bool FAST>
 __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                             const float* __restrict__ rho,
                                                             const float4* __restrict__ vin,
@@ -624,14 +631,13 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
   }
   cnb_count[p] = nc;
   flush_compact(s_list, lane, nc, cnb, K.stride, p);
-  const f3 dv = xsph_sum<true, FAST>(s_list, n, lane, nc, p, xp, vp, K, xs, rho, vin);
+  const f3 dv = xsph_sum<true>(s_list, n, lane, nc, p, xp, vp, K, xs, rho, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
   st3(vout, p, nv);
 }
 
 // apply_viscosity, second call, fused with update_positions (particles.h:390-394)
-template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                            const float* __restrict__ rho,
                                                            const float4* __restrict__ vin, float4* __restrict__ vout,
@@ -641,7 +647,7 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K
   if (p >= n) return;
   const f3 xp = ld3(xs, p);
   const f3 vp = ld3(vin, p);
-  const f3 dv = xsph_sum<false, FAST>(cnb, n, 0, cnb_count[p], p, xp, vp, K, xs, rho, vin);
+  const f3 dv = xsph_sum<false>(cnb, n, 0, cnb_count[p], p, xp, vp, K, xs, rho, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
   st3(vout, p, nv);
@@ -649,7 +655,6 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K
 }
 
 // compute_vorticity (particles.h:452-466) -> scr
-template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                         const float4* __restrict__ vin, float4* __restrict__ scr,
                                                         const int32_t* __restrict__ cnb,
@@ -673,15 +678,14 @@ __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, c
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= cnt) continue;
       const f3 d = sub3(xp, xyz(t[u]));
-      const float r = FAST ? __builtin_amdgcn_sqrtf(magsq3(d)) : sqrtf(magsq3(d));
-      acc3(w, cross3(sub3(xyz(vq[u]), vp), nb_spiky<FAST>(d, r, K)));
+      const float r = sqrtf(magsq3(d));
+      acc3(w, cross3(sub3(xyz(vq[u]), vp), spiky_in(d, r, K)));
     }
   }
   st3(scr, p, w);
 }
 
 // apply_vorticity (particles.h:471-487), v in place (own particle only)
-template <bool FAST>
 __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                               const float4* __restrict__ scr,
                                                               float4* __restrict__ v,
@@ -705,9 +709,9 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= cnt) continue;
       const f3 d = sub3(xp, xyz(t[u]));
-      const float r = FAST ? __builtin_amdgcn_sqrtf(magsq3(d)) : sqrtf(magsq3(d));
-      const float wm = FAST ? __builtin_amdgcn_sqrtf(magsq3(xyz(wq[u]))) : sqrtf(magsq3(xyz(wq[u])));
-      acc3(eta, smul3(wm, nb_spiky<FAST>(d, r, K)));
+      const float r = sqrtf(magsq3(d));
+      const float wm = sqrtf(magsq3(xyz(wq[u])));
+      acc3(eta, smul3(wm, spiky_in(d, r, K)));
     }
   }
   const f3 N = div3(eta, sqrtf(magsq3(eta)) + K.r_eps);
@@ -829,23 +833,23 @@ void launch_add_warm(pbf_context* c) {
 }
 
 void launch_viscosity_first(pbf_context* c) {
-  PBF_LAUNCH_ARITH(PBF_LAUNCH_STAGED, c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
+  PBF_LAUNCH_STAGED(c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
              (const float4*)c->v, c->va, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb,
              c->cnb_count);
 }
 
 void launch_vorticity_masked(pbf_context* c) {
-  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "vorticity", k_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->va, c->scr,
+  PBF_LAUNCH(c, "vorticity", k_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->va, c->scr,
              (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 
 void launch_apply_vorticity_masked(pbf_context* c) {
-  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "apply_vorticity", k_apply_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->scr, c->va,
+  PBF_LAUNCH(c, "apply_vorticity", k_apply_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->scr, c->va,
              (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 
 void launch_viscosity_last(pbf_context* c) {
-  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "viscosity_positions", k_viscosity_last, c->K, (const float4*)c->xs, (const float*)c->rho,
+  PBF_LAUNCH(c, "viscosity_positions", k_viscosity_last, c->K, (const float4*)c->xs, (const float*)c->rho,
              (const float4*)c->va, c->v, c->x, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 

@@ -106,6 +106,7 @@ struct SolveCtl {
   unsigned long long* esum;    // sum of the errors in 2^-32 units, same slots
   float* lsum;                 // sum of the applied multipliers (null: not kept)
   const float4* xsave;         // position stars to restore when the warm start is rejected
+  int strict;                  // reject the warm start when the max or the mean error grows
 };
 
 // The first density pass of a warm-started solve: the learned warm start's
@@ -150,6 +151,7 @@ struct pbf_context {
   uint64_t ncells = 0;
   uint64_t grid_alloc = 0;  // cells allocated
   int cap = 60;
+  int cap_alloc = 60;  // list rows allocated at pbf_create: the largest cap pbf_set_params accepts
   bool lists_valid = false;
   uint32_t last_iterations = 0;
   uint32_t dirty_stamp = 0;  // incremented per Jacobi iteration of the fused step

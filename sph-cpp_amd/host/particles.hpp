@@ -93,6 +93,17 @@ class Particles {
     check(pbf_get_solve_info(ctx_, &info));
     return info;
   }
+  // nsteps such steps replayed from captured HIP graphs (no host round trip),
+  // then their per-step records
+  std::vector<pbf_solve_record> solve_graph(float tolerance, unsigned max_iterations, int warm_start,
+                                            unsigned nsteps) {
+    check(pbf_solve_graph(ctx_, tolerance, max_iterations, warm_start, nsteps));
+    std::vector<pbf_solve_record> log(nsteps < PBF_SOLVE_LOG_SIZE ? nsteps : PBF_SOLVE_LOG_SIZE);
+    uint32_t got = 0;
+    check(pbf_get_solve_log(ctx_, log.data(), (uint32_t)log.size(), &got));
+    log.resize(got);
+    return log;
+  }
   void set_warm_lambdas(const float* lambdas) { check(pbf_set_warm_lambdas(ctx_, lambdas)); }
   void set_warm_model(pbf_warm_model* model) { check(pbf_set_warm_model(ctx_, model)); }
   pbf_tuning autotune(unsigned steps) {
@@ -100,6 +111,13 @@ class Particles {
     double ms = 0.0;
     check(pbf_autotune(ctx_, steps, &t, &ms));
     return t;
+  }
+  // bin spacing and list cap for speed (changes the list layout against the reference's)
+  pbf_layout autotune_layout(unsigned steps) {
+    pbf_layout l;
+    check(pbf_autotune_layout(ctx_, steps, &l));
+    check(pbf_get_params(ctx_, &params_));
+    return l;
   }
   void sync() { check(pbf_sync(ctx_)); }
 

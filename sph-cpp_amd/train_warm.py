@@ -2,26 +2,29 @@
 
 Training pairs come from randomised dam breaks (seeded block sizes at box
 scales 1, 2 and 4 of the 3.2 x 1.6 x 1.6 box, jitter uniform in
-[0, 0.05 h]).  With the default source, "plain50", each scene advances with
-the plain solver run to 50 Jacobi iterations per substep, and the pair of a
-particle is (features of the substep's first density pass, (Lambda_50 -
-Lambda_prev) / S): Lambda_50 is the sum of the multipliers the 50
-iterations applied (the multiplier the plain solve converges to), Lambda_prev
-the previous substep's, S = PBF_WARM_LAMBDA_SCALE.  The features come from a
-zero warm start, which moves nothing, so the step is the plain one.  The
-network thus learns how far the converged multiplier moves from one substep
-to the next.  Source "tolerance" instead advances the scene with the solves
-the warm start serves (iterate to a mean density error of ``tol``,
-warm-started from the previous sums) and takes the change of their sums.
-After every ``every``-th sampled substep the features become samples (a
-stride over the cell order).  The network is trained with the library's own
-MFMA forward/backward and Adam kernels (pbf_warm_model_train).  Afterwards
-the output layer is scaled by the least-squares gain of the output on samples
-of held-out scenes at the largest box scale (clamped to [0, 1]), so that only
-as much of the correction is applied as carries over to large scenes the
-network has not seen.
+[0, 0.05 h]).  A pair is (features of a substep's first density pass,
+(Lambda - Lambda_prev) / S), Lambda being the sum of the multipliers a solve
+applied and Lambda_prev the previous substep's, S = PBF_WARM_LAMBDA_SCALE:
+the network learns how far the multiplier sum moves from one substep to the
+next.  Two sources of trajectories:
 
-    python sph-cpp_amd/train_warm.py [--pairs 2000000] [--epochs 40] [--source plain50|tolerance]
+* "tolerance" (default; sph-cpp_amd/warm_model.npy): each scene advances
+  with the solves the warm start serves, iterating to a mean density error of
+  ``tol`` warm-started from the previous sums.
+* "plain50" (sph-cpp_amd/warm_model_plain50.npy): each scene advances with
+  the plain solver run to 50 Jacobi iterations per substep, so Lambda is the
+  multiplier the plain solve converges to.  The features come from a zero
+  warm start, which moves nothing, so the step is the plain one.
+
+The bench runs both networks.  After every ``every``-th sampled substep the
+features become samples (a stride over the cell order).  The network is
+trained with the library's own MFMA forward/backward and Adam kernels
+(pbf_warm_model_train).  Afterwards the output layer is scaled by the
+least-squares gain of the output on samples of held-out scenes at the largest
+box scale (clamped to [0, 1]), so that only as much of the correction is
+applied as carries over to large scenes the network has not seen.
+
+    python sph-cpp_amd/train_warm.py [--pairs 2000000] [--epochs 40] [--source tolerance|plain50]
                                      [--out sph-cpp_amd/warm_model.npy]
 
 Prints one JSON line per scene and a summary; writes the 4801 parameters
@@ -62,7 +65,7 @@ PLAIN_ITERATIONS = 50
 
 
 def generate(model, pairs, seed=0, steps=200, cap=60, tol=0.01, max_slots=128, per_sample=10_000,
-             scenes_min=8, log=print, budget=None, scale=None, source="plain50"):
+             scenes_min=8, log=print, budget=None, scale=None, source="tolerance"):
     """Add samples until the model holds `pairs`.  Each scene contributes at
     most `budget` samples (default: pairs / scenes_min), each sampled substep
     at most per_sample (a stride over the cell order), and the sampled
@@ -132,7 +135,7 @@ def gain_from_losses(c, lh, l1):
 
 
 def train(pairs=2_000_000, epochs=40, batch=4096, lr=3e-3, seed=0, log=print, scale=None, scenes_min=8,
-          source="plain50"):
+          source="tolerance"):
     model = pbf.WarmModel(pairs + 2 * VALIDATION_PAIRS)
     t0 = time.time()
     generate(model, pairs, seed, log=log, scale=scale, scenes_min=scenes_min, source=source)
@@ -173,7 +176,7 @@ def main(argv=None):
     ap.add_argument("--scale", type=int, default=0, choices=(0, 1, 2, 4),
                     help="box scale of the training scenes (0: 1, 2 and 4 at random)")
     ap.add_argument("--scenes", type=int, default=8, help="training scenes (each gives pairs / scenes samples)")
-    ap.add_argument("--source", default="plain50", choices=("plain50", "tolerance"),
+    ap.add_argument("--source", default="tolerance", choices=("tolerance", "plain50"),
                     help="plain50: plain solves of 50 iterations; tolerance: warm-started tolerance solves")
     ap.add_argument("--out", default=MODEL_PATH)
     a = ap.parse_args(argv)

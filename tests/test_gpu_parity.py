@@ -530,6 +530,50 @@ def test_serial_mode_difference(pbf, scenes):
         assert rms_gpu <= 3.0 * rms_fp, (k, rms_gpu, rms_fp)
 
 
+def test_c1_serial_backend_after_10_and_100_substeps(pbf, scenes):
+    """The 32,768-particle dam break against the reference's serial CPP_PAR
+    semantics over its whole run, and the FP64 rerun of the same steps.
+
+    Stated tolerance: after 10 and 100 substeps the RMS |dx|/h between the GPU
+    and the serial float run is at most 3x the RMS between the serial float
+    and serial double runs (the reference at Real = float and Real = double),
+    and the same holds for the centre of mass (largest component).  The two
+    serial runs take ~45 s each on one core and run side by side.
+    """
+    import threading
+
+    p, pos, vel = scenes.config_c1(seed=0)
+    g = pbf.Particles(p)
+    g.add(pos, pos, vel)
+    op = O.params_from_pbf(p)
+    s32 = O.Oracle(op, mode=O.SERIAL, threads=1)
+    s64 = O.Oracle(op, double=True, mode=O.SERIAL, threads=1)
+    s32.set_state(pos, pos, vel)
+    s64.set_state(pos.astype(np.float64), pos.astype(np.float64), vel.astype(np.float64))
+    done = 0
+    for k in (10, 100):
+        g.step(k - done)
+        runs = [threading.Thread(target=o.step, args=(k - done,)) for o in (s32, s64)]
+        for t in runs:
+            t.start()
+        for t in runs:
+            t.join()
+        done = k
+        x32, x64 = s32.state()["x"], s64.state()["x"]
+        gx = g.positions().astype(np.float64)
+        a = np.linalg.norm(gx - x32, axis=1) / 0.1
+        b = np.linalg.norm(x32 - x64, axis=1) / 0.1
+        rms_gpu, rms_fp = np.sqrt((a ** 2).mean()), np.sqrt((b ** 2).mean())
+        com_gpu = float(np.abs(gx.mean(axis=0) - x32.mean(axis=0)).max())
+        com_fp = float(np.abs(x32.mean(axis=0) - x64.mean(axis=0)).max())
+        print(f"C1 substep {k}: GPU vs serial float RMS {rms_gpu:.3e} h (max {a.max():.3e} h), serial float vs "
+              f"double RMS {rms_fp:.3e} h (max {b.max():.3e} h); centre of mass {com_gpu:.3e} vs {com_fp:.3e}",
+              flush=True)
+        assert rms_gpu <= 3.0 * rms_fp, (k, rms_gpu, rms_fp)
+        assert com_gpu <= 3.0 * com_fp + 1e-6, (k, com_gpu, com_fp)
+    g.close(); s32.close(); s64.close()
+
+
 def test_surface_tension_equals_oracle(pbf, scenes):
     """apply_surface_tension (particles.h:400-431) with gamma != 0, per pass and
     inside the fused step, against the oracle bit for bit (the colour field is

@@ -129,6 +129,33 @@ def test_warm_start_previous_over_steps(pbf, scenes):
     g.close(); o.close()
 
 
+def test_warm_start_strict_fallback(pbf, scenes):
+    """PBF_OPT_WARM_STRICT: the warm start is also rejected when the error
+    measure the tolerance does not use grows.  A given start of 4x the sums of
+    a plain 5-slot solve lowers the max error but raises the mean: kept by the
+    default fallback, rejected by the strict one; both bit for bit against the
+    oracle."""
+    params, x, v = _state(scenes)
+    o = _orc(params, x, v)
+    _, _, lsum, _ = o.solve(0.0, 5)
+    o.close()
+    w = (lsum * np.float32(4.0)).astype(np.float32)
+    kept_by = {}
+    for strict in (False, True):
+        p = pbf.Params.from_buffer_copy(params)
+        if strict:
+            p.options |= pbf.PBF_OPT_WARM_STRICT
+        o = _orc(params, x, v)
+        used, hist, lsum2, kept = o.solve(0.0, 4, warm=w, strict=strict)
+        g = _gpu(pbf, p, x, v)
+        g.set_warm_lambdas(w)
+        g.solve(0.0, 4, pbf.PBF_WARM_GIVEN)
+        _check(g, o, used, hist, lsum2, kept, f"given 4x sums, strict {strict}")
+        kept_by[strict] = kept
+        g.close(); o.close()
+    assert kept_by == {False: True, True: False}
+
+
 def test_solve_argument_errors(pbf, scenes):
     params, x, v = _state(scenes, steps=1)
     g = _gpu(pbf, params, x, v)

@@ -202,31 +202,36 @@ MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_trained_warm_start_gain(pbf, scenes):
     """What the trained network (sph-cpp_amd/warm_model.npy, made by
-    sph-cpp_amd/train_warm.py on the GPU) gains over the plain solve: a C1
-    dam break advanced 60 substeps, each iterated to a mean density error of
-    1%, needs at least 1.5x fewer Jacobi slots with the learned warm start."""
+    sph-cpp_amd/train_warm.py on the GPU) gains on the 32,768-particle dam
+    break advanced 60 substeps, each iterated to a mean density error of 1%:
+    at least 1.5x fewer Jacobi slots with the learned warm start than the
+    plain solve (measured 2,685 vs 1,023).  Against the previous multiplier
+    sums alone (PBF_WARM_PREVIOUS: the same start with a network that outputs
+    0) the network gains nothing at this size: it uses at most 5% more slots
+    (measured 1,023 vs 995); tests/test_gpu_full_size.py shows its gain over
+    them at 1M.  The substeps run from graph replays; the per-solve log gives
+    their slots."""
     if not os.path.exists(MODEL):
         pytest.skip("no trained model in the tree")
     m = pbf.WarmModel(16)
     m.set_params(np.load(MODEL, allow_pickle=False))
     res = {}
-    for name, mode in (("plain", pbf.PBF_WARM_NONE), ("learned", pbf.PBF_WARM_LEARNED)):
+    for name, mode in (("plain", pbf.PBF_WARM_NONE), ("previous", pbf.PBF_WARM_PREVIOUS),
+                       ("learned", pbf.PBF_WARM_LEARNED)):
         params, pos, vel = scenes.config_c1(seed=11)
         params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
         g = pbf.Particles(params)
         g.add(pos, pos, vel)
-        if mode:
+        if mode == pbf.PBF_WARM_LEARNED:
             g.set_warm_model(m)
-        slots, met = 0, 0
-        for _ in range(60):
-            g.solve(0.01, 128, mode)
-            info = g.solve_info()
-            slots += info["iterations"]
-            met += int(info["mean_error_history"][-1] <= np.float32(0.01))
-        res[name] = (slots, met)
+        g.solve_graph(0.01, 128, mode, 60)
+        log = g.solve_log(60)
+        res[name] = (sum(r["iterations"] for r in log), sum(int(r["mean_error"] <= np.float32(0.01)) for r in log))
         g.close()
     m.close()
     print(f"slots over 60 substeps: plain {res['plain'][0]} ({res['plain'][1]} met), "
+          f"previous {res['previous'][0]} ({res['previous'][1]} met), "
           f"learned {res['learned'][0]} ({res['learned'][1]} met)", flush=True)
     assert res["learned"][1] >= res["plain"][1]
     assert res["learned"][0] * 1.5 <= res["plain"][0]
+    assert res["learned"][0] <= 1.05 * res["previous"][0]

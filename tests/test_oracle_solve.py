@@ -136,3 +136,37 @@ def test_converged_warm_start_lowers_the_error(scenes):
     _, hw, _, kept = _oracle(params, x, v).solve(0.0, 4, warm=lsum)
     assert kept
     assert hw[0] < h0[0]
+
+
+def test_strict_fallback_rejects_what_raises_either_error(scenes):
+    """The strict fallback (PBF_OPT_WARM_STRICT) rejects a warm start that
+    lowers the max error but raises the mean; the default keeps it.  A
+    rejected start leaves the plain solve with its first slot spent: its
+    positions and sums equal the plain solve with one slot fewer (no
+    near-coincidence fix fires in this state)."""
+    params, pos, vel = scenes.config_c1(seed=5)
+    o = O.Oracle(O.params_from_pbf(params), mode=O.JACOBI, threads=THREADS)
+    o.set_state(pos, pos, vel)
+    o.step(10)
+    st = o.state()
+    o.close()
+    x, v = st["x"].astype(np.float32), st["v"].astype(np.float32)
+    a = _oracle(params, x, v)
+    _, _, lsum, _ = a.solve(0.0, 5)
+    a.close()
+    w = (lsum * np.float32(4.0)).astype(np.float32)
+    res = {}
+    for strict in (False, True):
+        b = _oracle(params, x, v)
+        used, hist, sums, kept = b.solve(0.0, 4, warm=w, strict=strict)
+        mx, mn = b.last_solve["max"], b.last_solve["mean"]
+        res[strict] = (kept, b.state(), sums, b.counters()["hacks"])
+        b.close()
+    assert res[False][0] is True and res[True][0] is False
+    plain = _oracle(params, x, v)
+    _, _, psums, _ = plain.solve(0.0, 3)
+    pst = plain.state()
+    plain.close()
+    assert res[True][3] == 0
+    assert np.array_equal(res[True][1]["x"], pst["x"])
+    assert np.array_equal(res[True][2], psums)

@@ -233,14 +233,14 @@ def timed_run(pbf, params, pos, vel, steps, profile=(), tuning=None):
 # rocprofv3 output -> the text summaries kept under profiles/
 PROFILED = "python3 bench.py --no-cpu --no-variants --no-warm"
 PROFILE_COMMANDS = {
-    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/trace -o c2 -- " + PROFILED,
-    "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o c2 -- " + PROFILED,
-    "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o c2 -- " + PROFILED,
+    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d /tmp/pbf_prof/trace -o c2 -- " + PROFILED,
+    "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d /tmp/pbf_prof/pmc_fetch -o c2 -- " + PROFILED,
+    "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d /tmp/pbf_prof/pmc_write -o c2 -- " + PROFILED,
     "sq": "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES "
-          "--output-format csv -d gpurun_out/pmc_sq -o c2 -- " + PROFILED,
-    "cal_fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/cal_fetch -o cp -- "
+          "--output-format csv -d /tmp/pbf_prof/pmc_sq -o c2 -- " + PROFILED,
+    "cal_fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d /tmp/pbf_prof/cal_fetch -o cp -- "
                  "python3 bench.py --calibrate-copy 512",
-    "cal_write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/cal_write -o cp -- "
+    "cal_write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d /tmp/pbf_prof/cal_write -o cp -- "
                  "python3 bench.py --calibrate-copy 512",
 }
 # The timed run is the last context the benchmark creates: its launches are
@@ -272,10 +272,10 @@ def _pmc(path: str, counter: str) -> dict:
     return out
 
 
-def summarize(d: str, workload: str) -> None:
+def summarize(d: str, workload: str, out_dir: str = "") -> None:
     """Write profiles/<workload>_kernels.md, <workload>_hbm_traffic.md and
     pmc_traffic.json from the rocprofv3 runs of PROFILE_COMMANDS under d."""
-    prof = os.path.join(ROOT, "profiles")
+    prof = out_dir or os.path.join(ROOT, "profiles")
     os.makedirs(prof, exist_ok=True)
     timed = _timed(_rows(os.path.join(d, "trace", "c2_kernel_trace.csv")), "Start_Timestamp")
     per: dict = {}
@@ -286,7 +286,7 @@ def summarize(d: str, workload: str) -> None:
     lines = [f"# Kernel times: {workload}", "",
              "Made by (from the repository root, on one MI355X):", "",
              "    " + PROFILE_COMMANDS["trace"], "",
-             "then `python3 bench.py --summarize gpurun_out`.  The command is the benchmark without the CPU",
+             "then `python3 bench.py --summarize /tmp/pbf_prof`.  The command is the benchmark without the CPU",
              "baseline, the variants and the warm-start comparison: a warm-up context advanced 100 substeps,",
              "the autotuner on it, then the timed run of 200 substeps of the 1,048,576-particle dam break from",
              "the initial scene (neighbour cap 60).  The table covers the timed run only (the launches after",
@@ -300,6 +300,20 @@ def summarize(d: str, workload: str) -> None:
                      f"{sum(v) / 1e3:.2f} | {100 * sum(v) / total_us:.1f} |")
     lines += ["", f"Sum of kernel time in the timed run: {total_us / 1e3:.1f} ms."]
     open(os.path.join(prof, f"{workload}_kernels.md"), "w").write("\n".join(lines) + "\n")
+    # rocprofv3's own --stats table of the same command (every launch of the process)
+    stats = os.path.join(d, "trace", "c2_kernel_stats.csv")
+    if os.path.exists(stats):
+        rows = _rows(stats)
+        cols = [c for c in ("Name", "Calls", "TotalDurationNs", "AverageNs", "Percentage", "MinNs", "MaxNs")
+                if rows and c in rows[0]]
+        lines = [f"# rocprofv3 --stats: {workload}", "", "The kernel statistics rocprofv3 wrote for", "",
+                 "    " + PROFILE_COMMANDS["trace"], "",
+                 "over the whole process (warm-up context, autotuner, profiled replay and timed run); the timed "
+                 f"run alone is in {workload}_kernels.md.", "",
+                 "| " + " | ".join(cols) + " |", "|" + "---|" * len(cols)]
+        for r in rows[:40]:
+            lines.append("| " + " | ".join(_short(r[c]) if c == "Name" else r[c] for c in cols) + " |")
+        open(os.path.join(prof, f"{workload}_kernel_stats.md"), "w").write("\n".join(lines) + "\n")
 
     fetch = _pmc(os.path.join(d, "pmc_fetch", "c2_counter_collection.csv"), "FETCH_SIZE")
     write = _pmc(os.path.join(d, "pmc_write", "c2_counter_collection.csv"), "WRITE_SIZE")
@@ -604,12 +618,14 @@ def main(argv=None):
                     help="warm-start comparison with PBF_OPT_WARM_STRICT (fallback when either error grows)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch settings instead of running pbf_autotune first")
+    ap.add_argument("--summary-dir", default="", metavar="DIR",
+                    help="with --summarize: write the summaries here instead of profiles/")
     ap.add_argument("--summarize", default="", metavar="DIR",
                     help="only write the profiles/ summaries from the rocprofv3 outputs under DIR")
     args = ap.parse_args(argv)
 
     if args.summarize:
-        summarize(args.summarize, "pbf_step_dambreak_1M")
+        summarize(args.summarize, "pbf_step_dambreak_1M", args.summary_dir)
         return 0
 
     if args.calibrate_copy:

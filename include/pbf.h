@@ -98,6 +98,12 @@ typedef struct pbf_params {
  * measure grows over the warm update (the max and the mean), not only the one the
  * tolerance applies to. */
 #define PBF_OPT_WARM_STRICT 16u
+/* The fused step walks every neighbour list in full.  By default, after building the
+ * lists it also keeps their entries within 1.1 h (the skin lists, in list order) and its
+ * Jacobi passes walk those while no particle has moved 0.045 h since the lists were built:
+ * an entry outside them is then farther than h and adds nothing, so the sums are the same,
+ * bit for bit.  This option turns the skin lists off (for comparison). */
+#define PBF_OPT_FULL_LISTS 32u
 
 typedef struct pbf_context pbf_context;
 

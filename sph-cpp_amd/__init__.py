@@ -98,6 +98,7 @@ PBF_OPT_SOLVE_MEAN_ERROR = 2
 PBF_OPT_KEEP_WARM_FEATURES = 4
 PBF_OPT_FAST_MATH = 8
 PBF_OPT_WARM_STRICT = 16
+PBF_OPT_FULL_LISTS = 32
 PBF_WARM_FEATURES = 8
 PBF_WARM_HIDDEN = 64
 PBF_WARM_PARAMS = 4801

@@ -180,6 +180,9 @@ static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
   K.stride = (K.cap + 7) / 8 * 8;
   K.count_overflow = (P.options & PBF_OPT_COUNT_OVERFLOW) ? 1 : 0;
   K.fast = (P.options & PBF_OPT_FAST_MATH) ? 1 : 0;
+  K.skin2 = (1.1f * h) * (1.1f * h);
+  K.disp_lim2 = (0.045f * h) * (0.045f * h);
+  K.skin_mode = (P.options & PBF_OPT_FULL_LISTS) ? 0 : 1;
   K.batched = 0;
   K.scenes = nullptr;
   K.sid = nullptr;
@@ -234,7 +237,7 @@ static void free_all(pbf_context* c) {
                   c->rhist, c->roffs, c->rpart, c->heavy, c->cnb, c->cnb_count, c->moved, c->dirty,
                   c->ehist, c->lsum, c->lsuma, c->wlam, c->wgiven, c->xsave, c->esum, c->feat, c->mw1,
                   c->mw2, c->mp, c->scenes, c->sid, c->sida, c->cbox, c->chbox, c->wmask, c->mflag,
-                  c->slog, c->slog_count};
+                  c->slog, c->slog_count, c->snb, c->snb_count, c->xs0, c->skin_hint};
   for (void* p : ptrs) dfree(p);
 }
 
@@ -274,7 +277,7 @@ __global__ void k_compact_perm(uint32_t n, const uint32_t* __restrict__ oid, uin
 }
 
 // neighbour search body shared by pbf_find_neighbors and pbf_step
-static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe);
+static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe, bool skin = false);
 
 void drop_graph(pbf_context* c) {
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
@@ -395,6 +398,8 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
       (rc = dalloc(&c->rpart, scan_partials_size(radix_hist_size(N) + 1))) || (rc = dalloc(&c->heavy, N)) ||
       (rc = dalloc(&c->chbox, 2 * ((N + 63) / 64))) || (rc = dalloc(&c->wmask, 2 * N)) ||
       (rc = dalloc(&c->cnb, (N + 63) / 64 * 64 * (uint64_t)c->K.stride)) || (rc = dalloc(&c->cnb_count, N)) ||
+      (rc = dalloc(&c->snb, (N + 63) / 64 * 64 * (uint64_t)c->K.stride)) || (rc = dalloc(&c->snb_count, N)) ||
+      (rc = dalloc(&c->xs0, NP)) || (rc = dalloc(&c->skin_hint, 1)) ||
       (rc = dalloc(&c->moved, N)) || (rc = dalloc(&c->dirty, N)) || (rc = dalloc(&c->mflag, N)) ||
       (rc = dalloc(&c->ehist, (uint64_t)kMaxSolveSlots + 1)) || (rc = dalloc(&c->lsum, N)) ||
       (rc = dalloc(&c->lsuma, N)) || (rc = dalloc(&c->wlam, N)) || (rc = dalloc(&c->wgiven, N)) ||
@@ -422,6 +427,7 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   launch_fill_u32(c->ehist, 0u, (uint64_t)kMaxSolveSlots + 1, c->stream);
   launch_fill_u64(c->esum, 0ull, (uint64_t)kMaxSolveSlots + 1, c->stream);
   launch_fill_u64(c->slog_count, 0ull, 1, c->stream);
+  launch_fill_u64(c->skin_hint, 1ull, 1, c->stream);
   if ((rc = check_hip(hipStreamSynchronize(c->stream), "pbf_create"))) {
     pbf_destroy(c);
     return rc;
@@ -794,7 +800,7 @@ static void step_body(pbf_context* c) {
       launch_fill_u32(c->mflag, 0u, n, c->stream);
     }
     launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
-    find_neighbors_impl(c, n, 0, n);
+    find_neighbors_impl(c, n, 0, n, true);
     if (c->track_lsum) launch_fill_u32((uint32_t*)c->lsum, 0u, n, c->stream);
     const SolveCtl S{-1.0f, 0, 0, nullptr, nullptr, c->track_lsum ? c->lsum : nullptr, nullptr};
     const uint32_t I = c->P.solve_step_count;
@@ -894,7 +900,7 @@ static int solve_body(pbf_context* c, float tolerance, uint32_t max_iterations, 
   launch_fill_u32(c->ehist, 0u, (uint64_t)max_iterations + 1, c->stream);
   launch_fill_u64(c->esum, 0ull, (uint64_t)max_iterations + 1, c->stream);
   launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
-  find_neighbors_impl(c, n, 0, n);  // carries the previous sums to the new order
+  find_neighbors_impl(c, n, 0, n, true);  // carries the previous sums to the new order
   // lsuma: the previous sums; lsum: this solve's
   std::swap(c->lsum, c->lsuma);
   if (!had_lsum) launch_fill_u32((uint32_t*)c->lsuma, 0u, n, c->stream);
@@ -1523,13 +1529,13 @@ int pbf_region_end(pbf_context* c, double* ms) {
 
 namespace pbf {
 
-static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe) {
+static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe, bool skin) {
   launch_fill_u64(c->counters + C_HEAVY, 0ull, 1, c->stream);
   launch_bin(c, bin_end);
   launch_sort(c);
   launch_reorder(c);
   launch_cell_box(c);
-  launch_fill_neighbors(c, fb, fe);
+  launch_fill_neighbors(c, fb, fe, skin);
   c->lists_valid = true;
 }
 

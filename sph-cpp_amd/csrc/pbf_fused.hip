@@ -60,6 +60,40 @@ __device__ __forceinline__ void flush_compact(const int32_t* __restrict__ s_list
   for (int k = 0; k < nc; ++k) cnb[list_at(k, p, stride)] = s_list[k * kStageBlock + lane];
 }
 
+// Skin lists (built with the lists, pbf_grid.hip): the entries within
+// 1.1 h of the positions the lists were built from (x*_0), in list order,
+// each slot packed with its place in the full list (slot | place << 25).  A
+// pass may walk the skin list instead of the full one while no particle has
+// moved 0.045 h or more from x*_0: an entry outside the skin list was at
+// least 1.1 h away, so it is still farther than 1.01 h, well past h and any
+// rounding of the within-h test, and adds nothing to any sum.  The entries
+// within h come in the same order, so the sums are the same bit for bit.
+// Every position a pressure update writes is checked against that move; a
+// larger one (or a NaN position) sets a flag: C_DISP by iteration parity
+// (read by the next iteration's passes) and C_DISP_ALL for the rest of the
+// step.  The flags are written only when a move is too large, so a step at
+// rest issues no atomics.
+constexpr uint32_t kSkinSlotBits = 25;
+constexpr uint32_t kSkinSlotMask = (1u << kSkinSlotBits) - 1u;
+
+__device__ __forceinline__ bool skin_usable(const unsigned long long* __restrict__ counters, int flag) {
+  return counters[C_SKIN] != 0ull && counters[flag] == 0ull;
+}
+
+// the squared move of a new position star from x*_0
+__device__ __forceinline__ float move2(f3 nw, float4 x0) { return magsq3(sub3(nw, xyz(x0))); }
+
+// Sets counters[a] and counters[C_DISP_ALL] when a lane's squared move m is
+// not below the limit (NaN included); every lane of the wavefront calls it.
+__device__ __forceinline__ void record_move(float m, unsigned long long* __restrict__ counters, int a,
+                                            const KParams& K) {
+  const uint64_t over = __ballot(!(m < K.disp_lim2));
+  if (over != 0ull && (threadIdx.x & 63) == 0) {
+    if (counters[a] == 0ull) atomicExch(&counters[a], 1ull);
+    if (counters[C_DISP_ALL] == 0ull) atomicExch(&counters[C_DISP_ALL], 1ull);
+  }
+}
+
 // x limited to [lo, hi] (NaN gives lo)
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return x > lo ? (x < hi ? x : hi) : lo; }
 
@@ -77,9 +111,14 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
                                                     float* __restrict__ lam, const int32_t* __restrict__ nbr,
                                                     const int32_t* __restrict__ nbr_count,
                                                     uint64_t* __restrict__ wmask, const FeatCtl& F,
-                                                    uint32_t& hacks_out, float (&fo)[8]) {
+                                                    uint32_t& hacks_out, float (&fo)[8],
+                                                    const int32_t* __restrict__ snb,
+                                                    const int32_t* __restrict__ snb_count, bool skin) {
   f3 xp = ld3(xs, p);
-  const int cnt = nbr_count[p];
+  // the walk goes over the skin list when it may (skin lists, above); the
+  // plain walk after a density fix always takes the full list
+  const int32_t* __restrict__ walk = skin ? snb : nbr;
+  const int cnt = skin ? snb_count[p] : nbr_count[p];
   float density = K.mass_W0;
   float sum_C = 0.0f;
   f3 sg = {0.f, 0.f, 0.f};
@@ -91,11 +130,11 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   uint64_t m0 = 0, m1 = 0;
   int hack_at = -1;
   for (int k0 = 0; k0 < cnt && hack_at < 0; k0 += kBatch) {
-    uint32_t q[kBatch];
-    load_batch(nbr, K.stride, k0, cnt, p, q);
+    uint32_t q[kBatch];  // skin entries keep their place in the full list in the top bits
+    load_batch(walk, K.stride, k0, cnt, p, q);
     float4 t[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
+    for (int u = 0; u < kBatch; ++u) t[u] = xs[skin ? q[u] & kSkinSlotMask : q[u]];
     uint32_t bits = 0;
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -105,14 +144,14 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
       const bool hk = w & (d2 < K.T_hack);  // particles.h:304: handled by the plain walk below
       const bool acc = w & !hk;
       bits |= (uint32_t)acc << u;
-      hack_at = hk ? k0 + u : hack_at;
+      hack_at = hk ? (skin ? (int)(q[u] >> kSkinSlotBits) : k0 + u) : hack_at;
       if (acc) {
         const float r = nb_sqrt<FAST>(d2);
         density += K.mass * poly6_in(r, K);
         const f3 grad = smul3(K.neg_inv_rho0, nb_spiky<FAST>(d, r, K));
         dec3(sg, grad);
         sum_C += magsq3(grad);
-        if (FEAT) {
+        if (FEAT) {  // (feature passes walk the full lists)
           dy_sum -= d.y;
           const f3 dv = sub3(vp, ld3(v, q[u]));
           div -= (dv.x * grad.x + dv.y * grad.y) + dv.z * grad.z;
@@ -130,7 +169,8 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   int within = __popcll(m0) + __popcll(m1);
   if (hack_at >= 0) {
     // particles.h:304-305: move x*_p, then finish this particle with a plain walk
-    for (int k = hack_at; k < cnt; ++k) {
+    const int full = nbr_count[p];
+    for (int k = hack_at; k < full; ++k) {
       const uint32_t qk = (uint32_t)nbr[list_at(k, p, K.stride)];
       const f3 xq = ld3(xs, qk);
       f3 d = sub3(xp, xq);
@@ -241,13 +281,18 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
                                                              uint64_t* __restrict__ wmask,
                                                              uint32_t* __restrict__ moved, uint32_t stamp,
                                                              uint32_t* __restrict__ mflag,
-                                                             unsigned long long* __restrict__ counters) {
+                                                             unsigned long long* __restrict__ counters,
+                                                             const int32_t* __restrict__ snb,
+                                                             const int32_t* __restrict__ snb_count) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  // this iteration's pressure update records its moves in the other slot
+  const bool skin = FM == 0 && skin_usable(counters, C_DISP + (it & 1));
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_DISP + ((it + 1) & 1)] = 0ull;
   float e = 0.0f;
   uint32_t hacks = 0;
   float fo[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (p < n) e = density_lambda_one<FM != 0, FAST>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, F, hacks, fo);
+  if (p < n) e = density_lambda_one<FM != 0, FAST>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, F, hacks, fo, snb, snb_count, skin);
   if (FM != 0 && F.feat && p < n) {
     float4* o = reinterpret_cast<float4*>(F.feat + (uint64_t)p * 8);
     o[0] = make_float4(fo[0], fo[1], fo[2], fo[3]);
@@ -426,14 +471,17 @@ __device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) 
 // before the warm start) restores xsave; an error <= tol ends the solve with
 // x* = xt.  Neither applies an update.
 template <bool FAST>
-__global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, SolveCtl S,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, SolveCtl S,
                                                         KParams K, const float4* __restrict__ xt,
                                                         const float4* __restrict__ x, float4* __restrict__ xs,
                                                         float4* __restrict__ v, const int32_t* __restrict__ nbr,
                                                         const int32_t* __restrict__ nbr_count,
                                                         const uint64_t* __restrict__ wmask,
                                                         const uint32_t* __restrict__ dirty,
-                                                        unsigned long long* __restrict__ counters) {
+                                                        unsigned long long* __restrict__ counters,
+                                                        const int32_t* __restrict__ snb,
+                                                        const int32_t* __restrict__ snb_count,
+                                                        const float4* __restrict__ xs0) {
   // C_DONE holds the slot that ended the solve, plus one.  Only a solve that
   // ended at an earlier slot skips this launch: blocks of the ending slot that
   // start after block 0 has written the flag must still copy xt to x*.
@@ -449,69 +497,87 @@ __global__ void __launch_bounds__(kBlock) k_dp_update_c(uint32_t n, int it, int 
     counters[C_MOVED_ALL + ((it + 1) & 1)] = 0ull;
   }
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  // what this slot does, the same for every thread: 0 the update, 1 the
+  // rejected warm start's restore, 2 the end of the solve (x* = xt)
+  int mode = 0;
   if (S.ehist) {
     const float e = S.mean ? mean_error(S.esum[it + 1], n) : __uint_as_float(S.ehist[it + 1]);
     const float e_pre = S.mean ? mean_error(S.esum[0], n) : __uint_as_float(S.ehist[0]);
+    bool grew = false;
     if (S.warm && it == 0) {
-      bool grew = e > e_pre;
+      grew = e > e_pre;
       if (S.strict) {  // the other error measure must not grow either
         const float o = S.mean ? __uint_as_float(S.ehist[1]) : mean_error(S.esum[1], n);
         const float o_pre = S.mean ? __uint_as_float(S.ehist[0]) : mean_error(S.esum[0], n);
         grew = grew || o > o_pre;
       }
-      if (grew) {
-        if (first) counters[C_WARM_KEPT] = 0ull;
-        if (p < n) xs[p] = S.xsave[p];
-        return;
-      }
     }
-    if (S.tol >= 0.0f && e <= S.tol) {
+    if (grew) {
+      mode = 1;
+      if (first) counters[C_WARM_KEPT] = 0ull;
+    } else if (S.tol >= 0.0f && e <= S.tol) {
+      mode = 2;
       if (first) {
         counters[C_DONE] = (unsigned long long)(it + 1);
         counters[C_ITERS] = (unsigned long long)(it + 1);
       }
-      if (p < n) xs[p] = xt[p];
-      return;
     }
   }
-  if (p >= n) return;
-  // a dirty particle (moved by the density fix, or next to one) walks its whole list with the test
-  const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull && dirty[p] == stamp;
-  // x* and lambda of a particle come in one load (xt.w holds lambda)
-  const float4 xp4 = xt[p];
-  const f3 xp = xyz(xp4);
-  const float lp = xp4.w;
-  f3 dp = {0.f, 0.f, 0.f};
-  // the entries the density pass summed (its within-h mask), or for a dirty
-  // particle every entry with the within-h test
-  const int cnt = nbr_count[p];
-  const uint64_t m0 = plain ? ~0ull : wmask[p];
-  const uint64_t m1 = (plain || K.cap <= 64) ? ~0ull : wmask[n + p];
-  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
-    uint32_t q[kBatch];
-    load_batch(nbr, K.stride, k0, cnt, p, q);
-    const uint32_t bits = (uint32_t)((k0 < 64 ? m0 >> k0 : m1 >> (k0 - 64)) & 0xFFu);
-    float4 t[kBatch];
+  const bool track = K.skin_mode && n < (1u << kSkinSlotBits);  // x*_0 is kept: check the moves
+  float mv = 0.0f;
+  if (p < n && mode == 1) {
+    const float4 sv = S.xsave[p];
+    xs[p] = sv;
+    if (track) mv = move2(xyz(sv), xs0[p]);
+  } else if (p < n && mode == 2) {
+    const float4 t = xt[p];
+    xs[p] = t;
+    if (track) mv = move2(xyz(t), xs0[p]);
+  } else if (p < n) {
+    // a dirty particle (moved by the density fix, or next to one) walks its
+    // whole list with the test; the others walk the list the density pass
+    // walked (the skin list when it could) under its within-h mask
+    const bool plain = counters[C_HACK_FLAG + (it & 1)] != 0ull && dirty[p] == stamp;
+    const bool skin = !plain && skin_usable(counters, C_DISP + (it & 1));
+    const int32_t* __restrict__ walk = skin ? snb : nbr;
+    // x* and lambda of a particle come in one load (xt.w holds lambda)
+    const float4 xp4 = xt[p];
+    const f3 xp = xyz(xp4);
+    const float lp = xp4.w;
+    f3 dp = {0.f, 0.f, 0.f};
+    const int cnt = skin ? snb_count[p] : nbr_count[p];
+    const uint64_t m0 = plain ? ~0ull : wmask[p];
+    const uint64_t m1 = (plain || K.cap <= 64) ? ~0ull : wmask[n + p];
+    for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+      uint32_t q[kBatch];
+      load_batch(walk, K.stride, k0, cnt, p, q);
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xt[(bits >> u) & 1u ? q[u] : p];  // masked entries read p itself
+      for (int u = 0; u < kBatch; ++u) q[u] = skin ? q[u] & kSkinSlotMask : q[u];
+      const uint32_t bits = (uint32_t)((k0 < 64 ? m0 >> k0 : m1 >> (k0 - 64)) & 0xFFu);
+      float4 t[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const f3 d = sub3(xp, xyz(t[u]));
-      const float d2 = magsq3(d);
-      if (k0 + u >= cnt || !((bits >> u) & 1u) || (plain && !within_h(d2, K))) continue;
-      const float r = nb_sqrt<FAST>(d2);
-      acc3(dp, smul3(lp + t[u].w, nb_spiky<FAST>(d, r, K)));
+      for (int u = 0; u < kBatch; ++u) t[u] = xt[(bits >> u) & 1u ? q[u] : p];  // masked entries read p itself
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const f3 d = sub3(xp, xyz(t[u]));
+        const float d2 = magsq3(d);
+        if (k0 + u >= cnt || !((bits >> u) & 1u) || (plain && !within_h(d2, K))) continue;
+        const float r = nb_sqrt<FAST>(d2);
+        acc3(dp, smul3(lp + t[u].w, nb_spiky<FAST>(d, r, K)));
+      }
+    }
+    f3 nw = add3(xp, smul3(K.inv_rho0, dp));
+    boundary(nw, K, p);
+    st3(xs, p, nw);
+    if (track) mv = move2(nw, xs0[p]);
+    if (S.lsum) S.lsum[p] += lp;
+    if (last) {
+      f3 vel = div3(sub3(nw, ld3(x, p)), K.dt);
+      if ((double)magsq3(vel) < K.vthr) vel = {0.f, 0.f, 0.f};
+      st3(v, p, vel);
     }
   }
-  f3 nw = add3(xp, smul3(K.inv_rho0, dp));
-  boundary(nw, K, p);
-  st3(xs, p, nw);
-  if (S.lsum) S.lsum[p] += lp;
-  if (last) {
-    f3 vel = div3(sub3(nw, ld3(x, p)), K.dt);
-    if ((double)magsq3(vel) < K.vthr) vel = {0.f, 0.f, 0.f};
-    st3(v, p, vel);
-  }
+  if (track) record_move(mv, counters, C_DISP + ((it + 1) & 1), K);
 }
 
 // Warm start of an iterate-to-tolerance solve: one compute_pressure_dps +
@@ -524,37 +590,46 @@ __global__ void __launch_bounds__(kBlock) k_warm_dp(uint32_t n, KParams K, const
                                                     const float* __restrict__ wlam,
                                                     const int32_t* __restrict__ nbr,
                                                     const int32_t* __restrict__ nbr_count, float4* __restrict__ xs,
-                                                    float4* __restrict__ xsave) {
+                                                    float4* __restrict__ xsave,
+                                                    unsigned long long* __restrict__ counters,
+                                                    const float4* __restrict__ xs0) {
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const float4 xp4 = xt[p];
-  const f3 xp = xyz(xp4);
-  const float lp = wlam[p];
-  f3 dp = {0.f, 0.f, 0.f};
-  const int cnt = nbr_count[p];
-  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
-    uint32_t q[kBatch];
-    load_batch(nbr, K.stride, k0, cnt, p, q);
-    float4 t[kBatch];
-    float lq[kBatch];
+  const bool track = K.skin_mode && n < (1u << kSkinSlotBits);
+  float mv = 0.0f;
+  if (p < n) {
+    const float4 xp4 = xt[p];
+    const f3 xp = xyz(xp4);
+    const float lp = wlam[p];
+    f3 dp = {0.f, 0.f, 0.f};
+    const int cnt = nbr_count[p];
+    for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+      uint32_t q[kBatch];
+      load_batch(nbr, K.stride, k0, cnt, p, q);
+      float4 t[kBatch];
+      float lq[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      t[u] = xt[q[u]];
-      lq[u] = wlam[q[u]];
-    }
+      for (int u = 0; u < kBatch; ++u) {
+        t[u] = xt[q[u]];
+        lq[u] = wlam[q[u]];
+      }
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const f3 d = sub3(xp, xyz(t[u]));
-      const float d2 = magsq3(d);
-      if (k0 + u >= cnt || !within_h(d2, K)) continue;
-      const float r = nb_sqrt<FAST>(d2);
-      acc3(dp, smul3(lp + lq[u], nb_spiky<FAST>(d, r, K)));
+      for (int u = 0; u < kBatch; ++u) {
+        const f3 d = sub3(xp, xyz(t[u]));
+        const float d2 = magsq3(d);
+        if (k0 + u >= cnt || !within_h(d2, K)) continue;
+        const float r = nb_sqrt<FAST>(d2);
+        acc3(dp, smul3(lp + lq[u], nb_spiky<FAST>(d, r, K)));
+      }
     }
+    f3 nw = add3(xp, smul3(K.inv_rho0, dp));
+    boundary(nw, K, p);
+    st3(xs, p, nw);
+    xsave[p] = xp4;
+    // both x* (kept) and xt (restored on a rejection) count as moves
+    if (track) mv = fmaxf(move2(nw, xs0[p]), move2(xp, xs0[p]));
   }
-  f3 nw = add3(xp, smul3(K.inv_rho0, dp));
-  boundary(nw, K, p);
-  st3(xs, p, nw);
-  xsave[p] = xp4;
+  // slot 0 reads parity 0
+  if (track) record_move(mv, counters, C_DISP + 0, K);
 }
 
 // After the Jacobi loop: the kept warm-start multipliers join the sum.
@@ -606,18 +681,30 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
                                                             const int32_t* __restrict__ nbr,
                                                             const int32_t* __restrict__ nbr_count,
                                                             int32_t* __restrict__ cnb,
-                                                            int32_t* __restrict__ cnb_count) {
+                                                            int32_t* __restrict__ cnb_count,
+                                                            const unsigned long long* __restrict__ counters,
+                                                            const int32_t* __restrict__ snb,
+                                                            const int32_t* __restrict__ snb_count,
+                                                            unsigned long long* __restrict__ skin_hint) {
   extern __shared__ int32_t s_list[];  // [cap + 1][kStageBlock]
   const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  // the next step builds skin lists only if this one's moves kept them usable to the end
+  if (K.skin_mode && blockIdx.x == 0 && lane == 0) *skin_hint = counters[C_DISP_ALL] == 0ull ? 1ull : 0ull;
   if (p >= n) return;
   const f3 xp = ld3(xs, p);
   const f3 vp = ld3(vin, p);
-  const int cnt = nbr_count[p];
+  // the skin list holds every entry within h while no particle has moved
+  // 0.045 h over the step (skin lists, above)
+  const bool skin = skin_usable(counters, C_DISP_ALL);
+  const int32_t* __restrict__ walk = skin ? snb : nbr;
+  const int cnt = skin ? snb_count[p] : nbr_count[p];
   int nc = 0;
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
-    load_batch(nbr, K.stride, k0, cnt, p, q);
+    load_batch(walk, K.stride, k0, cnt, p, q);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) q[u] = skin ? q[u] & kSkinSlotMask : q[u];
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
@@ -756,11 +843,13 @@ static void launch_density_fm(pbf_context* c, const char* name, int it, int eslo
   if (c->K.fast)
     PBF_LAUNCH(c, name, (k_density_lambda_c<FM, true>), it, eslot, S, F, c->K, (const float4*)c->xs,
                (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
-               c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters);
+               c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters, (const int32_t*)c->snb,
+               (const int32_t*)c->snb_count);
   else
     PBF_LAUNCH(c, name, (k_density_lambda_c<FM, false>), it, eslot, S, F, c->K, (const float4*)c->xs,
                (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
-               c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters);
+               c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters, (const int32_t*)c->snb,
+               (const int32_t*)c->snb_count);
 }
 
 void launch_density_lambda_masked(pbf_context* c, int it, int eslot, const SolveCtl& S, const FeatCtl& F) {
@@ -791,12 +880,13 @@ void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S) {
 void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S) {
   PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
              (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
-             (const int32_t*)c->nbr_count, (const uint64_t*)c->wmask, (const uint32_t*)c->dirty, c->counters);
+             (const int32_t*)c->nbr_count, (const uint64_t*)c->wmask, (const uint32_t*)c->dirty, c->counters,
+             (const int32_t*)c->snb, (const int32_t*)c->snb_count, (const float4*)c->xs0);
 }
 
 void launch_warm_dp(pbf_context* c) {
   PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "warm_dp", k_warm_dp, c->K, (const float4*)c->xt, (const float*)c->wlam, (const int32_t*)c->nbr,
-             (const int32_t*)c->nbr_count, c->xs, c->xsave);
+             (const int32_t*)c->nbr_count, c->xs, c->xsave, c->counters, (const float4*)c->xs0);
 }
 
 // The record of the solve that just ran (pbf_get_solve_log): slots used,
@@ -835,7 +925,8 @@ void launch_add_warm(pbf_context* c) {
 void launch_viscosity_first(pbf_context* c) {
   PBF_LAUNCH_STAGED(c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
              (const float4*)c->v, c->va, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb,
-             c->cnb_count);
+             c->cnb_count, (const unsigned long long*)c->counters, (const int32_t*)c->snb,
+             (const int32_t*)c->snb_count, c->skin_hint);
 }
 
 void launch_vorticity_masked(pbf_context* c) {

@@ -475,13 +475,22 @@ __device__ __forceinline__ WalkCells walk_cells(f3 ps, float s) {
 // column (entry k of the 64 lanes is contiguous) and writes them out at the
 // end, so that every global store of entry k covers the wavefront's 64
 // consecutive slots instead of 64 scattered 4-byte writes.  A staged entry is
-// 16 bits: the cell's place in the 27-cell walk (c, 5 bits) and the slot's
-// offset in that cell (11 bits), which halves the staging and doubles the
-// wavefronts the LDS holds.  The offset stays below kMaxCellOffset because a
-// lane walks a cell in full only when its 27 cells hold at most that many
-// candidates, and otherwise stops after a budget below it.
+// 16 bits: the slot's offset in its cell (10 bits), the cell's place in the
+// 27-cell walk (c, 5 bits) and whether the candidate lies within the skin
+// radius (bit 15), which halves the staging and doubles the wavefronts the
+// LDS holds.  The offset stays below kMaxCellOffset because a lane walks a
+// cell in full only when its 27 cells hold at most that many candidates, and
+// otherwise stops after a budget below it.
+//
+// Skin lists (with `skin`): the entries within 1.1 h (K.skin2) of the
+// positions the lists are built from, in list order, each slot packed with
+// its place in the full list (slot | place << 25; n < 2^25); x*_0 = x*.
+// The Jacobi passes walk them instead of the full lists while no particle has
+// moved far enough for an entry outside them to come within h
+// (pbf_fused.hip).
 constexpr int kFillBlock = 64;
-constexpr uint32_t kMaxCellOffset = 2048u;
+constexpr uint32_t kMaxCellOffset = 1024u;
+constexpr uint32_t kSkinPlaceShift = 25;
 
 // The lane walk of particle p: writes its list and count and returns true,
 // or returns false when the particle goes to the wavefront builder.  ovf
@@ -489,7 +498,8 @@ constexpr uint32_t kMaxCellOffset = 2048u;
 __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane, uint16_t* __restrict__ s_list,
                                           const float4* __restrict__ xs, const uint32_t* __restrict__ start,
                                           const KParams& K, int32_t* __restrict__ nbr,
-                                          int32_t* __restrict__ nbr_count, uint32_t& ovf) {
+                                          int32_t* __restrict__ nbr_count, uint32_t& ovf, int skin,
+                                          int32_t* __restrict__ snb, int32_t* __restrict__ snb_count) {
   const f3 ps = ld3(xs, p);
   const WalkCells w = walk_cells(ps, K.s);
   const GridView G = grid_of(K, p);
@@ -509,7 +519,7 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
 #pragma unroll
   for (int c = 0; c < 27; ++c) total += ce[c] - cb[c];
   // the lane's budget; below kMaxCellOffset - 8 so that a staged offset fits
-  // in 11 bits (a batch of 8 may pass the budget before the test)
+  // in 10 bits (a batch of 8 may pass the budget before the test)
   const uint32_t hb = min(K.heavy_budget, kMaxCellOffset - 8u);
   // every candidate must be seen to count the overflow: route by the total
   if (K.count_overflow && total > hb) return false;
@@ -525,8 +535,8 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
 #pragma unroll
   for (int c = 0; c < 27; ++c) {
     const uint32_t b = cb[c], e = ce[c];
-    // staged form of slot q: (c << 11) + (q - b), computed as q + eb in 16 bits
-    const uint32_t eb = ((uint32_t)c << 11) - b;
+    // staged form of slot q: (c << 10) + (q - b), computed as q + eb in 16 bits
+    const uint32_t eb = ((uint32_t)c << 10) - b;
     // candidates of a cell are contiguous: load 8 at a time, test in order
     // (position buffers carry 8 rows of padding, so the loads need no clamp)
     for (uint32_t q0 = b; q0 < e; q0 += 8) {
@@ -544,7 +554,7 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
         const uint32_t q = q0 + (uint32_t)u;
         const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
         const bool acc = (q < e) & (q != p) & (d2 < K.ss);
-        s_list[min(count, cap) * kFillBlock + lane] = (uint16_t)(q + eb);
+        s_list[min(count, cap) * kFillBlock + lane] = (uint16_t)((q + eb) | ((uint32_t)(d2 < K.skin2) << 15));
         count += acc ? 1 : 0;
       }
       // a full list cannot change any more; only the overflow count could
@@ -559,16 +569,19 @@ done:
   nbr_count[p] = count;
   // staged entries come in walk order, so the cells' first slots are taken
   // from registers cell by cell
-  int k = 0;
+  int k = 0, j = 0;
 #pragma unroll
   for (int c = 0; c < 27; ++c) {
     const uint32_t b = cb[c];
     for (; k < count; ++k) {
       const uint32_t e16 = s_list[k * kFillBlock + lane];
-      if ((e16 >> 11) != (uint32_t)c) break;
-      nbr[list_at(k, p, K.stride)] = (int32_t)(b + (e16 & (kMaxCellOffset - 1u)));
+      if (((e16 >> 10) & 31u) != (uint32_t)c) break;
+      const uint32_t slot = b + (e16 & (kMaxCellOffset - 1u));
+      nbr[list_at(k, p, K.stride)] = (int32_t)slot;
+      if (skin && (e16 >> 15)) snb[list_at(j++, p, K.stride)] = (int32_t)(slot | ((uint32_t)k << kSkinPlaceShift));
     }
   }
+  if (skin) snb_count[p] = j;
   return true;
 }
 
@@ -583,18 +596,29 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
                                                                int32_t* __restrict__ nbr,
                                                                int32_t* __restrict__ nbr_count,
                                                                uint32_t* __restrict__ heavy,
-                                                               unsigned long long* __restrict__ counters) {
+                                                               unsigned long long* __restrict__ counters,
+                                                               int skin_mode,
+                                                               const unsigned long long* __restrict__ hint,
+                                                               int32_t* __restrict__ snb,
+                                                               int32_t* __restrict__ snb_count,
+                                                               float4* __restrict__ xs0) {
   extern __shared__ uint16_t s_list[];  // [cap + 1][kFillBlock]
   const uint32_t lane = threadIdx.x;
   const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  // skin lists only when the previous step kept them usable to its end
+  const int skin = skin_mode && *hint != 0ull;
+  if (skin && blockIdx.x == 0 && lane == 0) counters[C_SKIN] = 1ull;
   bool to_heavy = false;
   uint32_t ovf = 0;
   if (p < n) {
+    if (skin_mode) xs0[p] = xs[p];
     const uint32_t o = oid[p];
-    if (o < fb || o >= fe)
+    if (o < fb || o >= fe) {
       nbr_count[p] = 0;
-    else
-      to_heavy = !walk_list(p, n, lane, s_list, xs, start, K, nbr, nbr_count, ovf);
+      if (skin) snb_count[p] = 0;
+    } else {
+      to_heavy = !walk_list(p, n, lane, s_list, xs, start, K, nbr, nbr_count, ovf, skin, snb, snb_count);
+    }
   }
   // every lane of the wavefront (one per block) is here
   const uint64_t hm = __ballot(to_heavy);
@@ -618,12 +642,18 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
 // list is full and the walk may end.
 __device__ __forceinline__ bool take_group(uint32_t q, bool valid, float4 t, f3 ps, const KParams& K, uint32_t n,
                                            uint32_t p, uint64_t below, uint32_t cap, int32_t* __restrict__ nbr,
-                                           uint32_t& count) {
-  const bool acc = valid && magsq3(sub3(ps, f3{t.x, t.y, t.z})) < K.ss;
+                                           uint32_t& count, int skin, int32_t* __restrict__ snb, uint32_t& scount) {
+  const float d2 = magsq3(sub3(ps, f3{t.x, t.y, t.z}));
+  const bool acc = valid && d2 < K.ss;
   const uint64_t m = __ballot(acc);
-  if (acc) {
-    const uint32_t pos = count + (uint32_t)__popcll(m & below);
-    if (pos < cap) nbr[list_at(pos, p, K.stride)] = (int32_t)q;
+  const uint32_t pos = count + (uint32_t)__popcll(m & below);
+  if (acc && pos < cap) nbr[list_at(pos, p, K.stride)] = (int32_t)q;
+  if (skin) {  // entries of the list (pos < cap) within the skin radius, in order
+    const bool in = acc && pos < cap && d2 < K.skin2;
+    const uint64_t ms = __ballot(in);
+    if (in)
+      snb[list_at(scount + (uint32_t)__popcll(ms & below), p, K.stride)] = (int32_t)(q | (pos << kSkinPlaceShift));
+    scount += (uint32_t)__popcll(ms);
   }
   count += (uint32_t)__popcll(m);
   return count >= cap && !K.count_overflow;
@@ -637,7 +667,10 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
                                                        const float4* __restrict__ chbox, KParams K,
                                                        int32_t* __restrict__ nbr, int32_t* __restrict__ nbr_count,
                                                        const uint32_t* __restrict__ heavy,
-                                                       unsigned long long* __restrict__ counters) {
+                                                       unsigned long long* __restrict__ counters, int skin_mode,
+                                                       const unsigned long long* __restrict__ hint,
+                                                       int32_t* __restrict__ snb, int32_t* __restrict__ snb_count) {
+  const int skin = skin_mode && *hint != 0ull;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -649,7 +682,8 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
     const f3 ps = ld3(xs, p);
     const WalkCells w = walk_cells(ps, K.s);
     const GridView G = grid_of(K, p);
-    uint32_t count = 0;  // accepted so far, uncapped (uniform across the wave)
+    uint32_t count = 0;   // accepted so far, uncapped (uniform across the wave)
+    uint32_t scount = 0;  // skin entries so far
     // lane c < 27 fetches cell c's range and box test, all in flight at once;
     // the wavefront then walks the cells in order from registers
     uint32_t lb = 0, le = 0;
@@ -675,8 +709,8 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
           float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = ta;
           if (va) ta = xs[qa];
           if (vb) tb = xs[qb];
-          if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count)) goto done;
-          if (base + 64 < e && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count)) goto done;
+          if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count, skin, snb, scount)) goto done;
+          if (base + 64 < e && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count, skin, snb, scount)) goto done;
         }
         continue;
       }
@@ -702,34 +736,41 @@ __global__ void __launch_bounds__(kBlock) k_fill_heavy(uint32_t n, const float4*
           float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = ta;
           if (va) ta = xs[qa];
           if (vb) tb = xs[qb];
-          if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count)) goto done;
-          if (two && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count)) goto done;
+          if (take_group(qa, va, ta, ps, K, n, p, below, cap, nbr, count, skin, snb, scount)) goto done;
+          if (two && take_group(qb, vb, tb, ps, K, n, p, below, cap, nbr, count, skin, snb, scount)) goto done;
         }
       }
     }
   done:
     if (lane == 0) {
       nbr_count[p] = (int32_t)(count < cap ? count : cap);
+      if (skin) snb_count[p] = (int32_t)scount;
       if (count > cap && K.count_overflow) atomicAdd(&counters[C_OVERFLOW], (unsigned long long)(count - cap));
     }
   }
 }
 
-void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe) {
+void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe, bool want_skin) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
+  // skin lists for the fused step (their packed form needs slots below 2^25);
+  // x*_0 is kept whenever they may be used, the lists themselves when the
+  // previous step's moves stayed below their limit (c->skin_hint)
+  const int skin = want_skin && c->K.skin_mode && c->n < (1ull << kSkinPlaceShift);
   KernelTimer* t = timer_begin(c, "fill_neighbors");
   hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n, kFillBlock)), dim3(kFillBlock),
                      (uint32_t)(kFillBlock * (c->K.cap + 1) * sizeof(uint16_t)), c->stream, n, (uint32_t)fb,
                      (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
-                     (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count, c->heavy, c->counters);
+                     (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count, c->heavy, c->counters, skin,
+                     (const unsigned long long*)c->skin_hint, c->snb, c->snb_count, c->xs0);
   timer_end(c, t);
   // enough waves to cover the chip; each loops over the heavy list (usually empty)
   t = timer_begin(c, "fill_heavy");
   const uint32_t blocks = std::min<uint32_t>(nblocks((uint64_t)n * 64), c->tune.heavy_blocks);
   hipLaunchKernelGGL(k_fill_heavy, dim3(blocks), dim3(kBlock), 0, c->stream, n, (const float4*)c->xs,
                      (const uint32_t*)c->cell_start, (const uint32_t*)c->cbox, (const float4*)c->chbox, c->K,
-                     c->nbr, c->nbr_count, (const uint32_t*)c->heavy, c->counters);
+                     c->nbr, c->nbr_count, (const uint32_t*)c->heavy, c->counters, skin,
+                     (const unsigned long long*)c->skin_hint, c->snb, c->snb_count);
   timer_end(c, t);
 }
 

@@ -21,6 +21,9 @@
 //   nbr_count        int32[n]
 //   cnb, cnb_count   int32[n/64][K][64], int32[n]  compact lists of the entries within h
 //                                (same tiling; built by the first XSPH pass)
+//   snb, snb_count   int32[n/64][K][64], int32[n]  skin lists: the entries within 1.1 h of the
+//                                positions the lists were built from (xs0), in list order, each
+//                                slot packed with its place in the full list (bits 25-31)
 //   counters         u64[16]     overflow, outside, hacks, hack flags, solve state, ...
 //   ehist            uint32[kMaxSolveSlots+1]  max density error per solve slot (float bits)
 //   lsum, lsuma      float[n]    multipliers summed over the updates of the last solve
@@ -69,6 +72,9 @@ struct KParams {
   int cap;
   int stride;                    // list row length: cap rounded up to a multiple of 8
   int count_overflow;            // PBF_OPT_COUNT_OVERFLOW
+  float skin2;                   // (1.1 h)^2: the radius of the skin lists
+  float disp_lim2;               // (0.045 h)^2: squared moves below it keep the skin lists usable
+  int skin_mode;                 // skin lists allowed (no PBF_OPT_FULL_LISTS)
   int fast;                      // PBF_OPT_FAST_MATH: hardware sqrt and reciprocal in the fused step
   uint32_t heavy_budget;         // candidates one lane scans before a wavefront takes over (see walk_list)
   // batched scenes: ncells is the total over the scenes, sid the scene of each slot
@@ -89,7 +95,11 @@ enum Counter {
   C_DONE = 11,      // iterate-to-tolerance: 0, or the slot whose error fell below the tolerance plus one
   C_ITERS = 12,     // iterate-to-tolerance: slots used
   C_WARM_KEPT = 13, // the warm-start positions were kept
-  C_NUM = 16
+  C_DISP = 14,      // two flags (14, 15): some particle has moved too far for the skin lists by the
+                    // end of the previous Jacobi iteration (parity of the iteration that reads it)
+  C_SKIN = 16,      // 1: this step's skin lists are built
+  C_DISP_ALL = 17,  // the same over the whole step so far
+  C_NUM = 24
 };
 
 constexpr uint32_t kMaxSolveSlots = PBF_MAX_SOLVE_ITERATIONS;
@@ -178,6 +188,9 @@ struct pbf_context {
   uint32_t* scan_partials = nullptr;
   int32_t *nbr = nullptr, *nbr_count = nullptr;
   int32_t *cnb = nullptr, *cnb_count = nullptr;
+  int32_t *snb = nullptr, *snb_count = nullptr;  // skin lists (entries within 1.1 h when built)
+  float4* xs0 = nullptr;                          // position stars the lists were built from
+  unsigned long long* skin_hint = nullptr;  // 1: the last step's moves kept the skin lists usable
   uint64_t* wmask = nullptr;  // [2][n] within-h masks of the last fused density pass
   unsigned long long* counters = nullptr;
   uint32_t* ehist = nullptr;
@@ -256,7 +269,7 @@ void launch_reorder(pbf_context* c);         // gather state by perm
 uint64_t radix_hist_size(uint64_t n);
 int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint64_t n, uint32_t key_bits,
                      uint32_t* hist, uint32_t* offs, uint32_t* partials, hipStream_t s);
-void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe);
+void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe, bool want_skin);
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits, hipStream_t s);
 void launch_bounds_counting(const uint32_t* sorted, uint64_t n, uint64_t b, uint64_t e, uint32_t* out,
                             int upper, hipStream_t s);

@@ -437,14 +437,15 @@ def test_crowded_cells_chunk_walk_bit_exact(pbf, scenes, budget):
     _compare_state(g, o, "crowded cells step")
 
 
-@pytest.mark.parametrize("budget", [768, 4096])
+@pytest.mark.parametrize("budget", [4096, 384])
 def test_staged_offsets_near_the_limit(pbf, scenes, budget):
-    """One cell of 2,046 particles whose first 1,986 slots lie beyond s from the last 60: those 60
-    lanes reject 1,986 candidates before they accept, so the offsets they stage reach 2,045 (the
-    lane builder stages 11-bit offsets within a cell).  With budget 4096 (used as 2040) the lanes
-    walk the cell themselves; with 768 they hand it to the wavefront builder.  Lists bit for bit."""
+    """One cell of 1,022 particles whose first 962 slots lie beyond s from the last 60: those 60
+    lanes reject 962 candidates before they accept, so the offsets they stage reach 1,021 (the
+    lane builder stages 10-bit offsets within a cell, next to the skin bit).  With budget 4096
+    (used as 1016) the lanes walk the cell themselves; with 384 they hand it to the wavefront
+    builder.  Lists bit for bit; the step after them (through the skin lists) too."""
     rng = np.random.default_rng(5)
-    far = 0.235 + rng.uniform(-0.002, 0.002, size=(1986, 3))
+    far = 0.235 + rng.uniform(-0.002, 0.002, size=(962, 3))
     near = 0.125 + rng.uniform(-0.002, 0.002, size=(60, 3))
     pos = np.concatenate([far, near]).astype(np.float32)
     p = scenes.pbf_params((1.0, 1.0, 1.0), pos.shape[0])
@@ -455,15 +456,15 @@ def test_staged_offsets_near_the_limit(pbf, scenes, budget):
     o.pass_(O.FIND)
     nb = o.neighbors()
     b, e = g.bin_bounds()
-    assert (e - b).max() == 2046
+    assert (e - b).max() == 1022
     lists, counts = g.neighbor_lists()
     assert np.array_equal(counts, nb["counts"])
-    assert np.all(counts[1986:] == 59)
+    assert np.all(counts[962:] == 59)
     mask = np.arange(p.max_neighbors)[None, :] < counts[:, None]
     assert np.array_equal(np.where(mask, lists, 0), np.where(mask, nb["lists"], 0))
     g.step(1)
     o.step(1)
-    _compare_state(g, o, "cell of 2,046 step")
+    _compare_state(g, o, "cell of 1,022 step")
 
 
 def test_partial_bin_span(pbf, scenes):

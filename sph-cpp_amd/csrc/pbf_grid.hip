@@ -495,10 +495,11 @@ constexpr uint32_t kSkinPlaceShift = 25;
 // The lane walk of particle p: writes its list and count and returns true,
 // or returns false when the particle goes to the wavefront builder.  ovf
 // receives the candidates dropped by the cap.
+template <bool SKIN>
 __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane, uint16_t* __restrict__ s_list,
                                           const float4* __restrict__ xs, const uint32_t* __restrict__ start,
                                           const KParams& K, int32_t* __restrict__ nbr,
-                                          int32_t* __restrict__ nbr_count, uint32_t& ovf, int skin,
+                                          int32_t* __restrict__ nbr_count, uint32_t& ovf,
                                           int32_t* __restrict__ snb, int32_t* __restrict__ snb_count) {
   const f3 ps = ld3(xs, p);
   const WalkCells w = walk_cells(ps, K.s);
@@ -554,7 +555,8 @@ __device__ __forceinline__ bool walk_list(uint32_t p, uint32_t n, uint32_t lane,
         const uint32_t q = q0 + (uint32_t)u;
         const float d2 = magsq3(sub3(ps, f3{t[u].x, t[u].y, t[u].z}));
         const bool acc = (q < e) & (q != p) & (d2 < K.ss);
-        s_list[min(count, cap) * kFillBlock + lane] = (uint16_t)((q + eb) | ((uint32_t)(d2 < K.skin2) << 15));
+        const uint32_t sb = SKIN ? (uint32_t)(d2 < K.skin2) << 15 : 0u;
+        s_list[min(count, cap) * kFillBlock + lane] = (uint16_t)((q + eb) | sb);
         count += acc ? 1 : 0;
       }
       // a full list cannot change any more; only the overflow count could
@@ -578,10 +580,10 @@ done:
       if (((e16 >> 10) & 31u) != (uint32_t)c) break;
       const uint32_t slot = b + (e16 & (kMaxCellOffset - 1u));
       nbr[list_at(k, p, K.stride)] = (int32_t)slot;
-      if (skin && (e16 >> 15)) snb[list_at(j++, p, K.stride)] = (int32_t)(slot | ((uint32_t)k << kSkinPlaceShift));
+      if (SKIN && (e16 >> 15)) snb[list_at(j++, p, K.stride)] = (int32_t)(slot | ((uint32_t)k << kSkinPlaceShift));
     }
   }
-  if (skin) snb_count[p] = j;
+  if (SKIN) snb_count[p] = j;
   return true;
 }
 
@@ -617,7 +619,8 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
       nbr_count[p] = 0;
       if (skin) snb_count[p] = 0;
     } else {
-      to_heavy = !walk_list(p, n, lane, s_list, xs, start, K, nbr, nbr_count, ovf, skin, snb, snb_count);
+      to_heavy = skin ? !walk_list<true>(p, n, lane, s_list, xs, start, K, nbr, nbr_count, ovf, snb, snb_count)
+                      : !walk_list<false>(p, n, lane, s_list, xs, start, K, nbr, nbr_count, ovf, snb, snb_count);
     }
   }
   // every lane of the wavefront (one per block) is here

@@ -355,10 +355,11 @@ def summarize(d: str, workload: str, out_dir: str = "") -> None:
                  "Means over the timed run's launches.  `VALU issue` = wave64 VALU instructions per SIMD per",
                  f"cycle: VALU instructions / ({VALU_SIMDS} SIMDs x the kernel's mean duration from the kernel trace",
                  f"x {VALU_CLOCK_GHZ} GHz).  A back-to-back stream of wave64 instructions that each take 4 cycles on a",
-                 "16-lane SIMD issues 0.25; the density, pressure-update and wavefront list-builder kernels",
-                 "measure 0.26-0.33, so gfx950 runs part of their FP32 instructions in fewer than 4 cycles and",
-                 "those kernels are bound by VALU issue.  The lane list builder and the first XSPH pass, at",
-                 "0.10-0.13, wait on gathers.", "",
+                 "16-lane SIMD issues 0.25.  The density, pressure-update and wavefront list-builder kernels",
+                 "measure 0.27-0.31, but issue is not what bounds them: with PBF_OPT_FAST_MATH, which halves",
+                 "their instructions per neighbour (hardware square root and reciprocal), they run only ~5%",
+                 "faster.  They and the lane list builder and the XSPH passes (0.10-0.18) wait on their",
+                 "neighbour gathers.", "",
                  "| kernel | launches | waves | VALU instr / wave | vector loads / wave | LDS instr / wave | "
                  "mean us | VALU issue |",
                  "|---|---:|---:|---:|---:|---:|---:|---:|"]
@@ -674,8 +675,13 @@ def main(argv=None):
     avg_launch_ms = kd["total_ms"] / max(1, kd["launches"])
     alg = ALG_BYTES.get(dom, 0) * n
     achieved = alg / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    vinfo = _profile_valu(HIP_KERNEL.get(dom, dom)) or {}
+    issue = vinfo.get("valu_issue_per_simd_cycle")
     roof = {
         "bound": "hbm",
+        "limiter": ("neighbour-gather latency (measured: VALU issue "
+                    f"{issue} per SIMD-cycle, {vinfo.get('vector_loads_per_wave')} vector loads per wave; "
+                    "HBM traffic far below peak)") if issue is not None else None,
         "kernel": dom,
         "hip_kernel": HIP_KERNEL.get(dom, ""),
         "achieved": round(achieved, 2),

@@ -104,6 +104,12 @@ typedef struct pbf_params {
  * an entry outside them is then farther than h and adds nothing, so the sums are the same,
  * bit for bit.  This option turns the skin lists off (for comparison). */
 #define PBF_OPT_FULL_LISTS 32u
+/* The list builder walks every particle's 27 cells with one lane of its own.  By default
+ * the particles of ordinary cells are listed by a walk shared by all particles of a cell
+ * (each candidate tested by one lane against the cell's particles in turn); the lists are
+ * the same, bit for bit.  This option keeps the per-lane walk for every particle (for
+ * comparison). */
+#define PBF_OPT_LANE_LISTS 64u
 
 typedef struct pbf_context pbf_context;
 
@@ -118,6 +124,8 @@ typedef struct pbf_stats {
   uint64_t density_hacks;      /* triggers of the near-coincidence fix (particles.h:303-305) */
   uint64_t iterations;         /* Jacobi iterations run in the last step */
   double max_density_error;    /* max_p max(rho_p/rho0 - 1, 0) of the last density pass */
+  uint64_t tile_listed;        /* particles the last list build listed with the cell-tile walk
+                                  (PBF_OPT_LANE_LISTS); counted only with PBF_OPT_COUNT_OVERFLOW */
 } pbf_stats;
 
 /* Thread-local message for the last failing call. */

@@ -89,6 +89,7 @@ class Stats(C.Structure):
         ("density_hacks", C.c_uint64),
         ("iterations", C.c_uint64),
         ("max_density_error", C.c_double),
+        ("tile_listed", C.c_uint64),
     ]
 
 
@@ -99,6 +100,7 @@ PBF_OPT_KEEP_WARM_FEATURES = 4
 PBF_OPT_FAST_MATH = 8
 PBF_OPT_WARM_STRICT = 16
 PBF_OPT_FULL_LISTS = 32
+PBF_OPT_LANE_LISTS = 64
 PBF_WARM_FEATURES = 8
 PBF_WARM_HIDDEN = 64
 PBF_WARM_PARAMS = 4801

@@ -183,6 +183,7 @@ static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
   K.skin2 = (1.1f * h) * (1.1f * h);
   K.disp_lim2 = (0.045f * h) * (0.045f * h);
   K.skin_mode = (P.options & PBF_OPT_FULL_LISTS) ? 0 : 1;
+  K.tile_lists = (P.options & PBF_OPT_LANE_LISTS) ? 0 : 1;
   K.batched = 0;
   K.scenes = nullptr;
   K.sid = nullptr;
@@ -1351,6 +1352,7 @@ int pbf_get_stats(pbf_context* c, pbf_stats* out) {
   out->neighbor_overflow = h[C_OVERFLOW];
   out->outside_grid = h[C_OUTSIDE];
   out->density_hacks = h[C_HACKS];
+  out->tile_listed = h[C_TILE];
   out->iterations = c->last_was_solve ? h[C_ITERS] : c->last_iterations;
   double m = 0.0;
   for (float r : rho) {

@@ -147,6 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(const float4* __restrict__ xs, u
                                                 uint32_t* __restrict__ key, uint32_t* __restrict__ inv,
                                                 unsigned long long* __restrict__ counters) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) counters[C_TILE] = 0ull;  // counted by the list build that follows
   if (p >= n) return;
   const uint32_t o = oid[p];
   uint32_t k = K.ncells;  // particles past the bin span are placed in no cell
@@ -587,12 +588,245 @@ done:
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// The cell-tile walk: the default for particles of ordinary cells.  A
+// wavefront's 64 consecutive slots fall into runs of one cell each
+// (segments).  All particles of a segment walk the same candidate stream:
+// the 27 cells around their cell in walk order, each cell's slots in order,
+// i.e. the sequence the lane walk visits.  The segment stages the positions
+// of the stream's first kTileMaxCand candidates in LDS and takes them 64 at a
+// time, one per lane; for each particle of the segment the wavefront tests
+// its 64 candidates at once, and the ballot of the accepted ones is that
+// particle's next 64 acceptance bits, kept in the particle's own lane of two
+// registers (the particle's position comes from its lane by readlane).
+// Afterwards every lane stages its accepted stream positions in order up to
+// the cap, then turns them into slots through the segment's table of cell
+// starts.  The lists, counts, overflow counts and skin lists are the lane
+// walk's, bit for bit: the same candidates in the same order under the same
+// test.  Each candidate is tested by one lane for the whole segment instead
+// of by every particle's lane for itself, with no divergence between lanes
+// whose cells differ in size.
+//
+// A particle is not covered when its cell coordinates, computed in floating
+// point like the walk's (x + i*s), are not exactly its cell's neighbours,
+// when its cell is the outside bucket or one of its neighbour cells wraps
+// onto its own, when its 27 cells hold more than kTileMaxCand candidates, or
+// when its wavefront has more than kTileSegs segments; a wavefront with a
+// particle that is not covered takes the lane walk for all its particles.
+constexpr uint32_t kTileRounds = 8;
+constexpr uint32_t kTileMaxCand = 64u * kTileRounds;
+constexpr uint32_t kTileSegs = 16;
+constexpr uint32_t kTileTab = 28;  // per segment: 27 cell starts; 28 stream offsets (the last is the total)
+
+// v_writelane_b32: lane `lane` of v takes the uniform value `val`
+__device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t val, uint32_t lane) {
+  asm("v_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(val), "{m0}"(lane));
+  return v;
+}
+__device__ __forceinline__ float read_lane(float v, uint32_t lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+
+template <bool SKIN>
+__device__ __forceinline__ bool tile_walk(uint32_t tb, uint32_t lane, uint32_t n, bool query, f3 ps,
+                                          const uint32_t* __restrict__ key, const float4* __restrict__ xs,
+                                          const uint32_t* __restrict__ start, const KParams& K,
+                                          int32_t* __restrict__ nbr, int32_t* __restrict__ nbr_count,
+                                          int32_t* __restrict__ snb, int32_t* __restrict__ snb_count,
+                                          uint16_t* __restrict__ s_list, uint32_t* __restrict__ s_tab,
+                                          uint16_t* __restrict__ s_off, uint64_t* __restrict__ s_qm,
+                                          uint32_t& ovf) {
+  const uint32_t p = tb + lane;
+  const bool valid = p < n;
+  const uint32_t kk = valid ? key[p] : 0xFFFFFFFFu;
+  const WalkCells w = walk_cells(ps, K.s);
+  const uint32_t prev = __shfl_up(kk, 1, 64);
+  uint64_t heads = __builtin_amdgcn_ballot_w64(valid && (lane == 0 || kk != prev));
+  const uint32_t last = min(64u, n - tb);
+  // the current segment's stream (the slot, then the position, of every
+  // stream position) lives in the staging area, which the lists use only
+  // after the segments
+  uint32_t* __restrict__ s_stream = reinterpret_cast<uint32_t*>(s_list);
+  float* __restrict__ s_cand = reinterpret_cast<float*>(s_list);
+  uint32_t wa[2 * kTileRounds], ws[2 * kTileRounds];
+#pragma unroll
+  for (uint32_t i = 0; i < 2 * kTileRounds; ++i) wa[i] = ws[i] = 0u;
+  // pass 1: every segment's table of cells, and which particles it covers
+  bool mine = false;
+  uint32_t my_seg = 0, seg = 0;
+  while (heads != 0ull) {
+    const uint32_t s0 = (uint32_t)__builtin_ctzll(heads);
+    heads &= heads - 1ull;
+    const uint32_t s1 = heads ? (uint32_t)__builtin_ctzll(heads) : last;
+    const uint32_t ck = (uint32_t)__builtin_amdgcn_readlane((int)kk, (int)s0);
+    if (ck >= K.ncells) continue;  // the outside bucket (its particles are not covered)
+    if (seg >= kTileSegs) continue;
+    // the segment cell's coordinates in its grid (its scene's)
+    const GridView G = grid_of(K, tb + s0);
+    const uint32_t local = ck - G.off;
+    const uint32_t dxy = (uint32_t)G.dxy;
+    const uint32_t bz = local / dxy, rem = local - bz * dxy;
+    const uint32_t by = rem / G.dimx, bx = rem - by * G.dimx;
+    const float fx = (float)bx, fy = (float)by, fz = (float)bz;
+    // lanes 0..26: cell `lane` of the walk order (x outer, z inner)
+    uint32_t cb = 0, cm = 0, cellk = K.ncells;
+    if (lane < 27) {
+      const float ox = (float)((int)(lane / 9u) - 1), oy = (float)((int)((lane / 3u) % 3u) - 1),
+                  oz = (float)((int)(lane % 3u) - 1);
+      cellk = cell_key(fx + ox, fy + oy, fz + oz, G, K);
+      if (cellk < K.ncells) {
+        cb = start[cellk];
+        cm = start[cellk + 1] - cb;
+      }
+    }
+    // exclusive scan of the cell sizes: lane c gets the stream offset of cell c, lane 27 the total
+    uint32_t incl = cm;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+      const uint32_t t = __shfl_up(incl, d, 64);
+      if ((int)lane >= d) incl += t;
+    }
+    const uint32_t off = incl - cm;
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 26);
+    // the segment's particles whose own walk cells are the segment's
+    const bool canon = lane >= s0 && lane < s1 && query && w.cx[0] == fx - 1.0f && w.cx[1] == fx &&
+                       w.cx[2] == fx + 1.0f && w.cy[0] == fy - 1.0f && w.cy[1] == fy && w.cy[2] == fy + 1.0f &&
+                       w.cz[0] == fz - 1.0f && w.cz[1] == fz && w.cz[2] == fz + 1.0f;
+    const uint64_t qm = __builtin_amdgcn_ballot_w64(canon);
+    // a grid so narrow that a neighbour cell wraps onto the centre cell lists
+    // a particle's own slot twice
+    const bool dup = __builtin_amdgcn_ballot_w64(lane < 27 && lane != 13 && cellk == ck) != 0ull;
+    if (total > kTileMaxCand || dup || qm == 0ull) continue;
+    if (lane < 27) s_tab[seg * kTileTab + lane] = cb;
+    if (lane < kTileTab) s_off[seg * kTileTab + lane] = (uint16_t)off;
+    if (lane == 0) s_qm[seg] = qm;
+    if (canon) {
+      mine = true;
+      my_seg = seg;
+    }
+    ++seg;
+  }
+  // a wavefront takes the lane walk throughout unless every particle that
+  // needs a list is covered: the lane walk costs nearly as much for a few
+  // lanes as for all of them (measured: leaving the uncovered lanes alone to
+  // it was no faster at rest and 1-6% slower over the collapse)
+  if (__builtin_amdgcn_ballot_w64(query && !mine) != 0ull) return false;
+  __syncthreads();  // one wavefront per block
+  // pass 2: the segments' streams
+  for (uint32_t sg = 0; sg < seg; ++sg) {
+    // (the same in every lane: read into scalar registers)
+    const uint64_t qv = s_qm[sg];
+    const uint64_t qm = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)qv) |
+                        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(qv >> 32)) << 32);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_off[sg * kTileTab + 27]);
+    uint32_t cb = 0, off = 0, cm = 0;
+    if (lane < 27) {
+      cb = s_tab[sg * kTileTab + lane];
+      off = s_off[sg * kTileTab + lane];
+      cm = (uint32_t)s_off[sg * kTileTab + lane + 1] - off;
+    }
+    // stage the candidates' positions (SoA, stream order) through a table of
+    // their slots, both in the staging area the lists use only afterwards
+    for (uint32_t u = 0; u < cm; ++u) s_stream[off + u] = cb + u;
+    __syncthreads();
+    uint32_t sl[kTileRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kTileRounds; ++r) {
+      const uint32_t j = r * 64u + lane;
+      sl[r] = j < total ? s_stream[j] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kTileRounds; ++r) {
+      const uint32_t j = r * 64u + lane;
+      if (j < total) {
+        const f3 c3 = ld3(xs, sl[r]);
+        s_cand[j] = c3.x;
+        s_cand[kTileMaxCand + j] = c3.y;
+        s_cand[2 * kTileMaxCand + j] = c3.z;
+      }
+    }
+    __syncthreads();
+    // a particle's own stream position: its slot in the centre cell (place 13)
+    const uint32_t jself0 = (uint32_t)__builtin_amdgcn_readlane((int)off, 13) + tb -
+                            (uint32_t)__builtin_amdgcn_readlane((int)cb, 13);
+#pragma unroll
+    for (uint32_t r = 0; r < kTileRounds; ++r) {
+      if (r * 64u >= total) break;
+      const uint32_t j = r * 64u + lane;
+      f3 t = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+      if (j < total) t = f3{s_cand[j], s_cand[kTileMaxCand + j], s_cand[2 * kTileMaxCand + j]};
+      uint64_t qq = qm;
+      while (qq != 0ull) {
+        const uint32_t i = (uint32_t)__builtin_ctzll(qq);
+        qq &= qq - 1ull;
+        const f3 pi = {read_lane(ps.x, i), read_lane(ps.y, i), read_lane(ps.z, i)};
+        const float d2 = magsq3(sub3(pi, t));
+        const uint32_t js = jself0 + i;  // the particle itself is no candidate
+        const uint64_t self = (js >> 6) == r ? 1ull << (js & 63u) : 0ull;
+        const uint64_t am = __builtin_amdgcn_ballot_w64(d2 < K.ss) & ~self;
+        wa[2 * r] = write_lane(wa[2 * r], (uint32_t)am, i);
+        wa[2 * r + 1] = write_lane(wa[2 * r + 1], (uint32_t)(am >> 32), i);
+        if (SKIN) {
+          const uint64_t sm = __builtin_amdgcn_ballot_w64(d2 < K.skin2) & am;
+          ws[2 * r] = write_lane(ws[2 * r], (uint32_t)sm, i);
+          ws[2 * r + 1] = write_lane(ws[2 * r + 1], (uint32_t)(sm >> 32), i);
+        }
+      }
+    }
+    __syncthreads();  // the next segment rewrites the stream
+  }
+  if (!mine) return false;
+  // stage the accepted stream positions (with the skin bit) in order, up to the cap
+  const uint32_t base = my_seg * kTileTab;
+  const uint32_t total = s_off[base + 27];
+  const uint32_t cap = (uint32_t)K.cap;
+  uint32_t k = 0, acc = 0;
+#pragma unroll
+  for (uint32_t wi = 0; wi < 2 * kTileRounds; ++wi) {
+    if (32u * wi < total) {
+      uint32_t bits = wa[wi];
+      const uint32_t rest = total - 32u * wi;
+      if (rest < 32u) bits &= (1u << rest) - 1u;
+      acc += (uint32_t)__builtin_popcount(bits);
+      const uint32_t sb = SKIN ? ws[wi] : 0u;
+      while (bits != 0u && k < cap) {
+        const uint32_t b = (uint32_t)__builtin_ctz(bits);
+        bits &= bits - 1u;
+        s_list[k * kFillBlock + lane] = (uint16_t)((32u * wi + b) | (((sb >> b) & 1u) << 15));
+        ++k;
+      }
+    }
+  }
+  if (acc > k) ovf = acc - k;
+  nbr_count[p] = (int32_t)k;
+  // flush: stream positions to slots, cell by cell
+  uint32_t c = 0, co = 0, nxt = s_off[base + 1], cs = s_tab[base];
+  int js = 0;
+  for (uint32_t e = 0; e < k; ++e) {
+    const uint32_t e16 = s_list[e * kFillBlock + lane];
+    const uint32_t j = e16 & 1023u;
+    while (j >= nxt) {
+      ++c;
+      co = nxt;
+      cs = s_tab[base + c];
+      nxt = s_off[base + c + 1];
+    }
+    const uint32_t slot = cs + (j - co);
+    nbr[list_at(e, p, K.stride)] = (int32_t)slot;
+    if (SKIN && (e16 >> 15)) snb[list_at(js++, p, K.stride)] = (int32_t)(slot | (e << kSkinPlaceShift));
+  }
+  if (SKIN) snb_count[p] = js;
+  return true;
+}
+
 // Hand-offs and overflow counts go to the shared counters once per
 // wavefront (a same-address atomic per lane serialises in L2 when many
 // particles of a collapsed region hand off together).
 __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint32_t fb, uint32_t fe,
                                                                const float4* __restrict__ xs,
                                                                const uint32_t* __restrict__ oid,
+                                                               const uint32_t* __restrict__ key,
                                                                const uint32_t* __restrict__ start,
                                                                const uint32_t* __restrict__ box, KParams K,
                                                                int32_t* __restrict__ nbr,
@@ -604,18 +838,33 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
                                                                int32_t* __restrict__ snb,
                                                                int32_t* __restrict__ snb_count,
                                                                float4* __restrict__ xs0) {
-  extern __shared__ uint16_t s_list[];  // [cap + 1][kFillBlock]
+  extern __shared__ uint16_t s_list[];  // [cap + 1][kFillBlock], or the cell-tile walk's stream
+  __shared__ uint32_t s_tab[kTileSegs * kTileTab];
+  __shared__ uint16_t s_off[kTileSegs * kTileTab];
+  __shared__ uint64_t s_qm[kTileSegs];
   const uint32_t lane = threadIdx.x;
-  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  const uint32_t tb = xcd_block() * blockDim.x;
+  const uint32_t p = tb + threadIdx.x;
   // skin lists only when the previous step kept them usable to its end
   const int skin = skin_mode && *hint != 0ull;
   if (skin && blockIdx.x == 0 && lane == 0) counters[C_SKIN] = 1ull;
   bool to_heavy = false;
   uint32_t ovf = 0;
+  const uint32_t o = p < n ? oid[p] : 0u;
+  const bool query = p < n && o >= fb && o < fe;
+  bool built = false;
+  if (K.tile_lists) {  // every lane of the wavefront takes part
+    const f3 ps = p < n ? ld3(xs, p) : f3{0.f, 0.f, 0.f};
+    built = skin ? tile_walk<true>(tb, lane, n, query, ps, key, xs, start, K, nbr, nbr_count, snb, snb_count,
+                                   s_list, s_tab, s_off, s_qm, ovf)
+                 : tile_walk<false>(tb, lane, n, query, ps, key, xs, start, K, nbr, nbr_count, snb, snb_count,
+                                    s_list, s_tab, s_off, s_qm, ovf);
+  }
   if (p < n) {
     if (skin_mode) xs0[p] = xs[p];
-    const uint32_t o = oid[p];
-    if (o < fb || o >= fe) {
+    if (built) {
+      // listed by the cell-tile walk
+    } else if (!query) {
       nbr_count[p] = 0;
       if (skin) snb_count[p] = 0;
     } else {
@@ -632,10 +881,12 @@ __global__ void __launch_bounds__(kFillBlock) k_fill_neighbors(uint32_t n, uint3
     if (to_heavy) heavy[base + (unsigned long long)__popcll(hm & ((1ull << lane) - 1ull))] = p;
   }
   if (K.count_overflow) {
-    unsigned long long o = ovf;
+    unsigned long long ov = ovf;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) o += __shfl_xor(o, d, 64);
-    if (lane == 0 && o) atomicAdd(&counters[C_OVERFLOW], o);
+    for (int d = 32; d >= 1; d >>= 1) ov += __shfl_xor(ov, d, 64);
+    const unsigned long long tiles = (unsigned long long)__popcll(__ballot(built && p < n));
+    if (lane == 0 && ov) atomicAdd(&counters[C_OVERFLOW], ov);
+    if (lane == 0 && tiles) atomicAdd(&counters[C_TILE], tiles);
   }
 }
 
@@ -761,9 +1012,13 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe, bool want_s
   // previous step's moves stayed below their limit (c->skin_hint)
   const int skin = want_skin && c->K.skin_mode && c->n < (1ull << kSkinPlaceShift);
   KernelTimer* t = timer_begin(c, "fill_neighbors");
-  hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n, kFillBlock)), dim3(kFillBlock),
-                     (uint32_t)(kFillBlock * (c->K.cap + 1) * sizeof(uint16_t)), c->stream, n, (uint32_t)fb,
-                     (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->cell_start,
+  // the staging area holds a wavefront's lists, or the cell-tile walk's candidate positions
+  const uint32_t stage = std::max<uint32_t>(kFillBlock * (c->K.cap + 1) * sizeof(uint16_t),
+                                            3u * kTileMaxCand * sizeof(float));
+  hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n, kFillBlock)), dim3(kFillBlock), stage, c->stream, n,
+                     (uint32_t)fb,
+                     (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->key,
+                     (const uint32_t*)c->cell_start,
                      (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count, c->heavy, c->counters, skin,
                      (const unsigned long long*)c->skin_hint, c->snb, c->snb_count, c->xs0);
   timer_end(c, t);

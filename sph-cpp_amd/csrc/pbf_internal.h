@@ -75,6 +75,7 @@ struct KParams {
   float skin2;                   // (1.1 h)^2: the radius of the skin lists
   float disp_lim2;               // (0.045 h)^2: squared moves below it keep the skin lists usable
   int skin_mode;                 // skin lists allowed (no PBF_OPT_FULL_LISTS)
+  int tile_lists;                // the cell-tile list walk (no PBF_OPT_LANE_LISTS)
   int fast;                      // PBF_OPT_FAST_MATH: hardware sqrt and reciprocal in the fused step
   uint32_t heavy_budget;         // candidates one lane scans before a wavefront takes over (see walk_list)
   // batched scenes: ncells is the total over the scenes, sid the scene of each slot
@@ -99,6 +100,7 @@ enum Counter {
                     // end of the previous Jacobi iteration (parity of the iteration that reads it)
   C_SKIN = 16,      // 1: this step's skin lists are built
   C_DISP_ALL = 17,  // the same over the whole step so far
+  C_TILE = 18,      // particles listed by the cell-tile walk (counted with PBF_OPT_COUNT_OVERFLOW)
   C_NUM = 24
 };
 

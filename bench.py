@@ -231,13 +231,17 @@ def timed_run(pbf, params, pos, vel, steps, profile=(), tuning=None):
 
 # ---------------------------------------------------------------------------
 # rocprofv3 output -> the text summaries kept under profiles/
-PROFILED = "python3 bench.py --no-cpu --no-variants --no-warm"
+# the driver's timed run (substeps 1-20), and the configuration's 200 substeps for the kernel times
+PROFILED = "python3 bench.py --no-cpu --no-variants --no-warm --steps 20 --warmup 5"
+PROFILED_200 = "python3 bench.py --no-cpu --no-variants --no-warm --steps 200"
 PROFILE_COMMANDS = {
     "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d /tmp/pbf_prof/trace -o c2 -- " + PROFILED,
+    "trace200": "rocprofv3 --kernel-trace --stats --output-format csv -d /tmp/pbf_prof/trace200 -o c2 -- "
+                + PROFILED_200,
     "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d /tmp/pbf_prof/pmc_fetch -o c2 -- " + PROFILED,
     "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d /tmp/pbf_prof/pmc_write -o c2 -- " + PROFILED,
-    "sq": "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES "
-          "--output-format csv -d /tmp/pbf_prof/pmc_sq -o c2 -- " + PROFILED,
+    "sq": "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_ACTIVE_INST_VALU "
+          "SQ_THREAD_CYCLES_VALU --output-format csv -d /tmp/pbf_prof/pmc_sq -o c2 -- " + PROFILED,
     "cal_fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d /tmp/pbf_prof/cal_fetch -o cp -- "
                  "python3 bench.py --calibrate-copy 512",
     "cal_write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d /tmp/pbf_prof/cal_write -o cp -- "
@@ -272,26 +276,15 @@ def _pmc(path: str, counter: str) -> dict:
     return out
 
 
-def summarize(d: str, workload: str, out_dir: str = "") -> None:
-    """Write profiles/<workload>_kernels.md, <workload>_hbm_traffic.md and
-    pmc_traffic.json from the rocprofv3 runs of PROFILE_COMMANDS under d."""
-    prof = out_dir or os.path.join(ROOT, "profiles")
-    os.makedirs(prof, exist_ok=True)
-    timed = _timed(_rows(os.path.join(d, "trace", "c2_kernel_trace.csv")), "Start_Timestamp")
+def _kernel_table(trace_csv: str, title: str, command: str, about: str):
+    timed = _timed(_rows(trace_csv), "Start_Timestamp")
     per: dict = {}
     for r in timed:
         per.setdefault(_short(r["Kernel_Name"]), []).append(
             (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     total_us = sum(sum(v) for v in per.values())
-    lines = [f"# Kernel times: {workload}", "",
-             "Made by (from the repository root, on one MI355X):", "",
-             "    " + PROFILE_COMMANDS["trace"], "",
-             "then `python3 bench.py --summarize /tmp/pbf_prof`.  The command is the benchmark without the CPU",
-             "baseline, the variants and the warm-start comparison: a warm-up context advanced 100 substeps,",
-             "the autotuner on it, then the timed run of 200 substeps of the 1,048,576-particle dam break from",
-             "the initial scene (neighbour cap 60).  The table covers the timed run only (the launches after",
-             "its particle upload); the benchmark's own HIP-event times of the same launches are in its",
-             "`kernels_ms_per_step` and `roofline.avg_launch_ms`.", "",
+    lines = [f"# Kernel times: {title}", "", "Made by (from the repository root, on one MI355X):", "",
+             "    " + command, "", about, "",
              "| kernel | calls | avg us | min us | max us | total ms | % of timed run |",
              "|---|---:|---:|---:|---:|---:|---:|"]
     for k in sorted(per, key=lambda k: -sum(per[k])):
@@ -299,7 +292,32 @@ def summarize(d: str, workload: str, out_dir: str = "") -> None:
         lines.append(f"| `{k}` | {len(v)} | {statistics.mean(v):.1f} | {min(v):.1f} | {max(v):.1f} | "
                      f"{sum(v) / 1e3:.2f} | {100 * sum(v) / total_us:.1f} |")
     lines += ["", f"Sum of kernel time in the timed run: {total_us / 1e3:.1f} ms."]
+    return lines, per
+
+
+def summarize(d: str, workload: str, out_dir: str = "") -> None:
+    """Write profiles/<workload>_kernels.md (and _200_substeps_kernels.md),
+    _kernel_stats.md, _hbm_traffic.md, _valu.md, pmc_traffic.json and
+    pmc_valu.json from the rocprofv3 runs of PROFILE_COMMANDS under d."""
+    prof = out_dir or os.path.join(ROOT, "profiles")
+    os.makedirs(prof, exist_ok=True)
+    lines, per = _kernel_table(
+        os.path.join(d, "trace", "c2_kernel_trace.csv"), workload, PROFILE_COMMANDS["trace"],
+        "then `python3 bench.py --summarize /tmp/pbf_prof`.  The command is the benchmark's timed run as the\n"
+        "driver runs it (`--steps 20 --warmup 5`) without the CPU baseline, the variants and the warm-start\n"
+        "comparison: a warm-up context advanced 10 substeps, the autotuner on it, a profiled replay, then the\n"
+        "timed run of substeps 1-20 of the 1,048,576-particle dam break from the initial scene (neighbour cap\n"
+        "60).  The table covers the timed run only (the launches after its particle upload); the benchmark's\n"
+        "own HIP-event times of the same launches are its `kernels_ms_per_step` and `roofline.avg_launch_ms`.")
     open(os.path.join(prof, f"{workload}_kernels.md"), "w").write("\n".join(lines) + "\n")
+    t200 = os.path.join(d, "trace200", "c2_kernel_trace.csv")
+    if os.path.exists(t200):
+        lines200, _ = _kernel_table(
+            t200, f"{workload}, 200 substeps", PROFILE_COMMANDS["trace200"],
+            "The same benchmark over the configuration's 200 substeps (`--steps 200`): a warm-up context\n"
+            "advanced 100 substeps, the autotuner, then the timed run from the initial scene into the\n"
+            "neighbour-cap collapse.  The table covers the timed run only.")
+        open(os.path.join(prof, f"{workload}_200_substeps_kernels.md"), "w").write("\n".join(lines200) + "\n")
     # rocprofv3's own --stats table of the same command (every launch of the process)
     stats = os.path.join(d, "trace", "c2_kernel_stats.csv")
     if os.path.exists(stats):
@@ -348,21 +366,24 @@ def summarize(d: str, workload: str, out_dir: str = "") -> None:
 
     sq_path = os.path.join(d, "pmc_sq", "c2_counter_collection.csv")
     if os.path.exists(sq_path):
-        sq = {c: _pmc(sq_path, c) for c in ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS")}
+        sq = {c: _pmc(sq_path, c) for c in ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS",
+                                             "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU")}
         lines = [f"# VALU issue per launch: {workload}", "",
                  "One counter run of the benchmark command (six SQ counters):", "",
                  "    " + PROFILE_COMMANDS["sq"], "",
                  "Means over the timed run's launches.  `VALU issue` = wave64 VALU instructions per SIMD per",
                  f"cycle: VALU instructions / ({VALU_SIMDS} SIMDs x the kernel's mean duration from the kernel trace",
                  f"x {VALU_CLOCK_GHZ} GHz).  A back-to-back stream of wave64 instructions that each take 4 cycles on a",
-                 "16-lane SIMD issues 0.25.  The density, pressure-update and wavefront list-builder kernels",
-                 "measure 0.27-0.31, but issue is not what bounds them: with PBF_OPT_FAST_MATH, which halves",
-                 "their instructions per neighbour (hardware square root and reciprocal), they run only ~5%",
-                 "faster.  They and the lane list builder and the XSPH passes (0.10-0.18) wait on their",
-                 "neighbour gathers.", "",
+                 "16-lane SIMD issues 0.25.  `lanes busy` = SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU), the",
+                 "share of lanes active in the VALU instructions issued.  The density and pressure-update",
+                 "passes issue at that ceiling: they are bound by VALU issue (~70 instructions per accepted",
+                 "neighbour, half of them in the correctly rounded divide and square root).  With",
+                 "PBF_OPT_FAST_MATH (hardware square root and reciprocal) the density pass issues 43% fewer",
+                 "VALU instructions and runs 12% faster at rest, where it then waits on its neighbour gathers.",
+                 "The XSPH and vorticity passes (0.10-0.18) wait on their gathers.", "",
                  "| kernel | launches | waves | VALU instr / wave | vector loads / wave | LDS instr / wave | "
-                 "mean us | VALU issue |",
-                 "|---|---:|---:|---:|---:|---:|---:|---:|"]
+                 "mean us | VALU issue | lanes busy |",
+                 "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
         issue = {}
         for k in sorted(per, key=lambda k: -sum(per[k])):
             if k not in sq["SQ_INSTS_VALU"] or not sq["SQ_WAVES"].get(k):
@@ -375,10 +396,13 @@ def summarize(d: str, workload: str, out_dir: str = "") -> None:
             rate = valu / (VALU_SIMDS * us * 1e-6 * VALU_CLOCK_GHZ * 1e9) if us > 0 else 0.0
             if waves <= 0:
                 continue
+            active = statistics.mean(sq["SQ_ACTIVE_INST_VALU"].get(k, [0.0]))
+            threads = statistics.mean(sq["SQ_THREAD_CYCLES_VALU"].get(k, [0.0]))
+            lanes = threads / (64.0 * active) if active > 0 else 0.0
             lines.append(f"| `{k}` | {len(sq['SQ_INSTS_VALU'][k])} | {waves:.0f} | {valu / waves:.0f} | "
-                         f"{vmem / waves:.1f} | {lds / waves:.1f} | {us:.1f} | {rate:.3f} |")
+                         f"{vmem / waves:.1f} | {lds / waves:.1f} | {us:.1f} | {rate:.3f} | {lanes:.2f} |")
             issue[k] = {"valu_issue_per_simd_cycle": round(rate, 4), "valu_instr_per_wave": round(valu / waves),
-                        "vector_loads_per_wave": round(vmem / waves, 1)}
+                        "vector_loads_per_wave": round(vmem / waves, 1), "valu_lanes_busy": round(lanes, 3)}
         open(os.path.join(prof, f"{workload}_valu.md"), "w").write("\n".join(lines) + "\n")
         with open(os.path.join(prof, "pmc_valu.json"), "w") as f:
             json.dump({"workload": workload, "note": "wave64 VALU instructions per SIMD per cycle; 0.25 is one "
@@ -679,9 +703,10 @@ def main(argv=None):
     issue = vinfo.get("valu_issue_per_simd_cycle")
     roof = {
         "bound": "hbm",
-        "limiter": ("neighbour-gather latency (measured: VALU issue "
-                    f"{issue} per SIMD-cycle, {vinfo.get('vector_loads_per_wave')} vector loads per wave; "
-                    "HBM traffic far below peak)") if issue is not None else None,
+        "limiter": (f"VALU issue {issue} per SIMD-cycle (0.25: one 4-cycle wave64 instruction after another), "
+                    f"{vinfo.get('valu_instr_per_wave')} VALU instructions and "
+                    f"{vinfo.get('vector_loads_per_wave')} vector loads per wave; HBM traffic far below peak")
+                   if issue is not None else None,
         "kernel": dom,
         "hip_kernel": HIP_KERNEL.get(dom, ""),
         "achieved": round(achieved, 2),

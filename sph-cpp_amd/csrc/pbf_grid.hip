@@ -593,12 +593,12 @@ done:
 // wavefront's 64 consecutive slots fall into runs of one cell each
 // (segments).  All particles of a segment walk the same candidate stream:
 // the 27 cells around their cell in walk order, each cell's slots in order,
-// i.e. the sequence the lane walk visits.  The segment stages the positions
-// of the stream's first kTileMaxCand candidates in LDS and takes them 64 at a
-// time, one per lane; for each particle of the segment the wavefront tests
-// its 64 candidates at once, and the ballot of the accepted ones is that
-// particle's next 64 acceptance bits, kept in the particle's own lane of two
-// registers (the particle's position comes from its lane by readlane).
+// i.e. the sequence the lane walk visits.  The stream (at most kTileMaxCand
+// candidates) is held in registers, 64 candidates per round, one per lane;
+// for each particle of the segment the wavefront tests every round's 64
+// candidates at once, and the ballot of the accepted ones is that particle's
+// next 64 acceptance bits, kept in the particle's own lane of two registers
+// (the particle's position comes from its lane by readlane).
 // Afterwards every lane stages its accepted stream positions in order up to
 // the cap, then turns them into slots through the segment's table of cell
 // starts.  The lists, counts, overflow counts and skin lists are the lane
@@ -643,11 +643,9 @@ __device__ __forceinline__ bool tile_walk(uint32_t tb, uint32_t lane, uint32_t n
   const uint32_t prev = __shfl_up(kk, 1, 64);
   uint64_t heads = __builtin_amdgcn_ballot_w64(valid && (lane == 0 || kk != prev));
   const uint32_t last = min(64u, n - tb);
-  // the current segment's stream (the slot, then the position, of every
-  // stream position) lives in the staging area, which the lists use only
-  // after the segments
+  // the current segment's stream (the slot of every stream position) lives in
+  // the staging area, which the lists use only after the segments
   uint32_t* __restrict__ s_stream = reinterpret_cast<uint32_t*>(s_list);
-  float* __restrict__ s_cand = reinterpret_cast<float*>(s_list);
   uint32_t wa[2 * kTileRounds], ws[2 * kTileRounds];
 #pragma unroll
   for (uint32_t i = 0; i < 2 * kTileRounds; ++i) wa[i] = ws[i] = 0u;
@@ -725,44 +723,31 @@ __device__ __forceinline__ bool tile_walk(uint32_t tb, uint32_t lane, uint32_t n
       off = s_off[sg * kTileTab + lane];
       cm = (uint32_t)s_off[sg * kTileTab + lane + 1] - off;
     }
-    // stage the candidates' positions (SoA, stream order) through a table of
-    // their slots, both in the staging area the lists use only afterwards
+    // the candidates' slots through a table in the staging area (which the
+    // lists use only afterwards), then their positions into registers: every
+    // round's candidates stay there while the segment's particles go by
     for (uint32_t u = 0; u < cm; ++u) s_stream[off + u] = cb + u;
     __syncthreads();
-    uint32_t sl[kTileRounds];
+    f3 t[kTileRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kTileRounds; ++r) {
       const uint32_t j = r * 64u + lane;
-      sl[r] = j < total ? s_stream[j] : 0u;
+      t[r] = f3{__builtin_inff(), __builtin_inff(), __builtin_inff()};
+      if (j < total) t[r] = ld3(xs, s_stream[j]);
     }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t r = 0; r < kTileRounds; ++r) {
-      const uint32_t j = r * 64u + lane;
-      if (j < total) {
-        const f3 c3 = ld3(xs, sl[r]);
-        s_cand[j] = c3.x;
-        s_cand[kTileMaxCand + j] = c3.y;
-        s_cand[2 * kTileMaxCand + j] = c3.z;
-      }
-    }
-    __syncthreads();
     // a particle's own stream position: its slot in the centre cell (place 13)
     const uint32_t jself0 = (uint32_t)__builtin_amdgcn_readlane((int)off, 13) + tb -
                             (uint32_t)__builtin_amdgcn_readlane((int)cb, 13);
+    uint64_t qq = qm;
+    while (qq != 0ull) {
+      const uint32_t i = (uint32_t)__builtin_ctzll(qq);
+      qq &= qq - 1ull;
+      const f3 pi = {read_lane(ps.x, i), read_lane(ps.y, i), read_lane(ps.z, i)};
+      const uint32_t js = jself0 + i;  // the particle itself is no candidate
 #pragma unroll
-    for (uint32_t r = 0; r < kTileRounds; ++r) {
-      if (r * 64u >= total) break;
-      const uint32_t j = r * 64u + lane;
-      f3 t = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-      if (j < total) t = f3{s_cand[j], s_cand[kTileMaxCand + j], s_cand[2 * kTileMaxCand + j]};
-      uint64_t qq = qm;
-      while (qq != 0ull) {
-        const uint32_t i = (uint32_t)__builtin_ctzll(qq);
-        qq &= qq - 1ull;
-        const f3 pi = {read_lane(ps.x, i), read_lane(ps.y, i), read_lane(ps.z, i)};
-        const float d2 = magsq3(sub3(pi, t));
-        const uint32_t js = jself0 + i;  // the particle itself is no candidate
+      for (uint32_t r = 0; r < kTileRounds; ++r) {
+        if (r * 64u >= total) break;
+        const float d2 = magsq3(sub3(pi, t[r]));
         const uint64_t self = (js >> 6) == r ? 1ull << (js & 63u) : 0ull;
         const uint64_t am = __builtin_amdgcn_ballot_w64(d2 < K.ss) & ~self;
         wa[2 * r] = write_lane(wa[2 * r], (uint32_t)am, i);
@@ -1012,9 +997,9 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe, bool want_s
   // previous step's moves stayed below their limit (c->skin_hint)
   const int skin = want_skin && c->K.skin_mode && c->n < (1ull << kSkinPlaceShift);
   KernelTimer* t = timer_begin(c, "fill_neighbors");
-  // the staging area holds a wavefront's lists, or the cell-tile walk's candidate positions
+  // the staging area holds a wavefront's lists, or the cell-tile walk's stream of slots
   const uint32_t stage = std::max<uint32_t>(kFillBlock * (c->K.cap + 1) * sizeof(uint16_t),
-                                            3u * kTileMaxCand * sizeof(float));
+                                            kTileMaxCand * sizeof(uint32_t));
   hipLaunchKernelGGL(k_fill_neighbors, dim3(nblocks(n, kFillBlock)), dim3(kFillBlock), stage, c->stream, n,
                      (uint32_t)fb,
                      (uint32_t)fe, (const float4*)c->xs, (const uint32_t*)c->oid, (const uint32_t*)c->key,

@@ -481,7 +481,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
                                                         unsigned long long* __restrict__ counters,
                                                         const int32_t* __restrict__ snb,
                                                         const int32_t* __restrict__ snb_count,
-                                                        const float4* __restrict__ xs0) {
+                                                        const float4* __restrict__ xs0,
+                                                        const float* __restrict__ rho) {
   // C_DONE holds the slot that ended the solve, plus one.  Only a solve that
   // ended at an earlier slot skips this launch: blocks of the ending slot that
   // start after block 0 has written the flag must still copy xt to x*.
@@ -571,10 +572,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     st3(xs, p, nw);
     if (track) mv = move2(nw, xs0[p]);
     if (S.lsum) S.lsum[p] += lp;
-    if (last) {
+    if (last) {  // the density rides in v.w for the XSPH passes' gathers
       f3 vel = div3(sub3(nw, ld3(x, p)), K.dt);
       if ((double)magsq3(vel) < K.vthr) vel = {0.f, 0.f, 0.f};
-      st3(v, p, vel);
+      v[p] = make_float4(vel.x, vel.y, vel.z, rho[p]);
     }
   }
   if (track) record_move(mv, counters, C_DISP + ((it + 1) & 1), K);
@@ -645,7 +646,7 @@ __global__ void k_add_warm(uint32_t n, const float* __restrict__ wlam, float* __
 template <bool FROM_LDS>
 __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_t n, uint32_t lane, int nc,
                                        uint32_t p, f3 xp, f3 vp, const KParams& K, const float4* __restrict__ xs,
-                                       const float* __restrict__ rho, const float4* __restrict__ vin) {
+                                       const float4* __restrict__ vin) {
   f3 dv = {0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < nc; k0 += kBatch) {
     uint32_t q[kBatch];
@@ -653,19 +654,17 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
       load_batch_lds(list, lane, k0, nc, p, q);
     else
       load_batch(list, K.stride, k0, nc, p, q);
-    float4 t[kBatch], vq[kBatch];
-    float rq[kBatch];
+    float4 t[kBatch], vq[kBatch];  // vq.w: the neighbour's density
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       t[u] = xs[q[u]];
       vq[u] = vin[q[u]];
-      rq[u] = rho[q[u]];
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= nc) continue;
       const float r = sqrt_cr(magsq3(sub3(xp, xyz(t[u]))));
-      acc3(dv, div3(mul3(sub3(xyz(vq[u]), vp), poly6_in(r, K)), rq[u]));
+      acc3(dv, div3(mul3(sub3(xyz(vq[u]), vp), poly6_in(r, K)), vq[u].w));
     }
   }
   return dv;
@@ -675,7 +674,6 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
 // apply_viscosity (particles.h:436-447), first call: walks the whole list and
 // leaves the compact rows of the final positions for the passes that follow.
 __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KParams K, const float4* __restrict__ xs,
-                                                            const float* __restrict__ rho,
                                                             const float4* __restrict__ vin,
                                                             float4* __restrict__ vout,
                                                             const int32_t* __restrict__ nbr,
@@ -718,15 +716,14 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
   }
   cnb_count[p] = nc;
   flush_compact(s_list, lane, nc, cnb, K.stride, p);
-  const f3 dv = xsph_sum<true>(s_list, n, lane, nc, p, xp, vp, K, xs, rho, vin);
+  const f3 dv = xsph_sum<true>(s_list, n, lane, nc, p, xp, vp, K, xs, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
-  st3(vout, p, nv);
+  vout[p] = make_float4(nv.x, nv.y, nv.z, vin[p].w);  // the density stays in w
 }
 
 // apply_viscosity, second call, fused with update_positions (particles.h:390-394)
 __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K, const float4* __restrict__ xs,
-                                                           const float* __restrict__ rho,
                                                            const float4* __restrict__ vin, float4* __restrict__ vout,
                                                            float4* __restrict__ x, const int32_t* __restrict__ cnb,
                                                            const int32_t* __restrict__ cnb_count) {
@@ -734,7 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K
   if (p >= n) return;
   const f3 xp = ld3(xs, p);
   const f3 vp = ld3(vin, p);
-  const f3 dv = xsph_sum<false>(cnb, n, 0, cnb_count[p], p, xp, vp, K, xs, rho, vin);
+  const f3 dv = xsph_sum<false>(cnb, n, 0, cnb_count[p], p, xp, vp, K, xs, vin);
   f3 nv = vp;
   acc3(nv, smul3(K.visc, dv));
   st3(vout, p, nv);
@@ -802,9 +799,10 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     }
   }
   const f3 N = div3(eta, sqrtf(magsq3(eta)) + K.r_eps);
-  f3 vp = ld3(v, p);
+  const float4 v4 = v[p];
+  f3 vp = xyz(v4);
   acc3(vp, mul3(mul3(cross3(N, ld3(scr, p)), K.vort), K.dt));
-  st3(v, p, vp);
+  v[p] = make_float4(vp.x, vp.y, vp.z, v4.w);  // the density stays in w for the second XSPH pass
 }
 
 // ---- launchers ----
@@ -881,7 +879,7 @@ void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S
   PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
              (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
              (const int32_t*)c->nbr_count, (const uint64_t*)c->wmask, (const uint32_t*)c->dirty, c->counters,
-             (const int32_t*)c->snb, (const int32_t*)c->snb_count, (const float4*)c->xs0);
+             (const int32_t*)c->snb, (const int32_t*)c->snb_count, (const float4*)c->xs0, (const float*)c->rho);
 }
 
 void launch_warm_dp(pbf_context* c) {
@@ -923,7 +921,7 @@ void launch_add_warm(pbf_context* c) {
 }
 
 void launch_viscosity_first(pbf_context* c) {
-  PBF_LAUNCH_STAGED(c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs, (const float*)c->rho,
+  PBF_LAUNCH_STAGED(c, "viscosity", k_viscosity_first, c->K, (const float4*)c->xs,
              (const float4*)c->v, c->va, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->cnb,
              c->cnb_count, (const unsigned long long*)c->counters, (const int32_t*)c->snb,
              (const int32_t*)c->snb_count, c->skin_hint);
@@ -940,7 +938,7 @@ void launch_apply_vorticity_masked(pbf_context* c) {
 }
 
 void launch_viscosity_last(pbf_context* c) {
-  PBF_LAUNCH(c, "viscosity_positions", k_viscosity_last, c->K, (const float4*)c->xs, (const float*)c->rho,
+  PBF_LAUNCH(c, "viscosity_positions", k_viscosity_last, c->K, (const float4*)c->xs,
              (const float4*)c->va, c->v, c->x, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 

@@ -1,8 +1,9 @@
 // pbf_internal.h -- shared declarations of the HIP PBF library (not part of the ABI).
 //
 // Device data layout (one context, n particles, C cells, K = list cap):
-//   x, xs, v         float4[n]   position, position star, velocity (w unused),
-//                                stored in cell order (sorted by (cell key, original index))
+//   x, xs, v         float4[n]   position, position star, velocity, stored in cell order (sorted by
+//                                (cell key, original index)); from the velocity update on, v.w holds the
+//                                density of the last density pass (gathered by the XSPH passes)
 //   xa, xsa, va      float4[n]   reorder targets, swapped with the above
 //   xt               float4[n]   position stars after the density pass (hack applied)
 //   scr              float4[n]   scratch vector: dp, vorticity or colour field

@@ -197,15 +197,17 @@ void launch_update_position_stars(pbf_context* c, uint64_t b, uint64_t e) {
 }
 
 // update_velocities (particles.h:375-385)
+// (the density rides in v.w, where the fused step's XSPH passes gather it)
 __global__ void __launch_bounds__(kBlock) k_update_velocities(uint32_t n, uint32_t b, uint32_t e, KParams K,
                                                               const uint32_t* __restrict__ oid,
                                                               const float4* __restrict__ x,
-                                                              const float4* __restrict__ xs, float4* __restrict__ v) {
+                                                              const float4* __restrict__ xs,
+                                                              const float* __restrict__ rho, float4* __restrict__ v) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n || !in_span(oid[p], b, e)) return;
   f3 vel = div3(sub3(ld3(xs, p), ld3(x, p)), K.dt);
   if ((double)magsq3(vel) < K.vthr) vel = {0.f, 0.f, 0.f};
-  st3(v, p, vel);
+  v[p] = make_float4(vel.x, vel.y, vel.z, rho[p]);
 }
 
 void launch_update_velocities(pbf_context* c, uint64_t b, uint64_t e) {
@@ -213,7 +215,8 @@ void launch_update_velocities(pbf_context* c, uint64_t b, uint64_t e) {
   if (!n) return;
   KernelTimer* t = timer_begin(c, "update_velocities");
   hipLaunchKernelGGL(k_update_velocities, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)b,
-                     (uint32_t)e, c->K, (const uint32_t*)c->oid, (const float4*)c->x, (const float4*)c->xs, c->v);
+                     (uint32_t)e, c->K, (const uint32_t*)c->oid, (const float4*)c->x, (const float4*)c->xs,
+                     (const float*)c->rho, c->v);
   timer_end(c, t);
 }
 
@@ -397,7 +400,7 @@ __global__ void __launch_bounds__(kBlock) k_tension(uint32_t n, uint32_t b, uint
   }
   f3 vp = ld3(v, p);
   acc3(vp, mul3(div3(f, rp), K.dt));
-  st3(v, p, vp);
+  v[p] = make_float4(vp.x, vp.y, vp.z, rp);  // the density stays in v.w for the XSPH passes
 }
 
 void launch_surface_tension(pbf_context* c, uint64_t cb, uint64_t ce, uint64_t b, uint64_t e) {

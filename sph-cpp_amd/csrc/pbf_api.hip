@@ -1535,8 +1535,9 @@ static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, u
   launch_fill_u64(c->counters + C_HEAVY, 0ull, 1, c->stream);
   launch_bin(c, bin_end);
   launch_sort(c);
-  launch_reorder(c);
-  launch_cell_box(c);
+  // a fused step recomputes densities, multipliers and the scratch vector
+  // before reading them (with at least one Jacobi iteration): they stay unsorted
+  launch_reorder(c, !(skin && c->P.solve_step_count > 0));
   launch_fill_neighbors(c, fb, fe, skin);
   c->lists_valid = true;
 }

@@ -738,7 +738,7 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K
   x[p] = make_float4(xp.x, xp.y, xp.z, 0.f);
 }
 
-// compute_vorticity (particles.h:452-466) -> scr
+// compute_vorticity (particles.h:452-466) -> scr (omega, and |omega| in w)
 __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                         const float4* __restrict__ vin, float4* __restrict__ scr,
                                                         const int32_t* __restrict__ cnb,
@@ -766,7 +766,7 @@ __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, c
       acc3(w, cross3(sub3(xyz(vq[u]), vp), spiky_in(d, r, K)));
     }
   }
-  st3(scr, p, w);
+  scr[p] = make_float4(w.x, w.y, w.z, sqrtf(magsq3(w)));  // |omega| for apply_vorticity's gathers
 }
 
 // apply_vorticity (particles.h:471-487), v in place (own particle only)
@@ -794,7 +794,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
       if (k0 + u >= cnt) continue;
       const f3 d = sub3(xp, xyz(t[u]));
       const float r = sqrtf(magsq3(d));
-      const float wm = sqrtf(magsq3(xyz(wq[u])));
+      const float wm = wq[u].w;  // |omega_q|, computed by the vorticity pass
       acc3(eta, smul3(wm, spiky_in(d, r, K)));
     }
   }

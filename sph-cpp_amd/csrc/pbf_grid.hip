@@ -234,19 +234,22 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
   xo[s] = x[p];
   xso[s] = xs[p];
   vo[s] = v[p];
-  rhoo[s] = rho[p];
-  lamo[s] = lam[p];
-  scro[s] = scr[p];
+  if (rho) {
+    rhoo[s] = rho[p];
+    lamo[s] = lam[p];
+    scro[s] = scr[p];
+  }
   oido[s] = oid[p];
   keyo[s] = key[p];
 }
 
-void launch_reorder(pbf_context* c) {
+void launch_reorder(pbf_context* c, bool all_fields) {
   const uint32_t n = (uint32_t)c->n;
   if (!n) return;
   KernelTimer* t = timer_begin(c, "reorder");
   hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->perm,
-                     (const float4*)c->x, (const float4*)c->xs, (const float4*)c->v, (const float*)c->rho,
+                     (const float4*)c->x, (const float4*)c->xs, (const float4*)c->v,
+                     all_fields ? (const float*)c->rho : nullptr,
                      (const float*)c->lam, (const float4*)c->scr, (const uint32_t*)c->oid,
                      (const uint32_t*)c->key, c->track_lsum ? (const float*)c->lsum : nullptr,
                      c->K.batched ? (const uint32_t*)c->sid : nullptr, c->xa, c->xsa, c->va, c->rhoa, c->lama,
@@ -260,9 +263,11 @@ void launch_reorder(pbf_context* c) {
   std::swap(c->x, c->xa);
   std::swap(c->xs, c->xsa);
   std::swap(c->v, c->va);
-  std::swap(c->rho, c->rhoa);
-  std::swap(c->lam, c->lama);
-  std::swap(c->scr, c->scra);
+  if (all_fields) {
+    std::swap(c->rho, c->rhoa);
+    std::swap(c->lam, c->lama);
+    std::swap(c->scr, c->scra);
+  }
   std::swap(c->oid, c->oida);
   std::swap(c->key, c->keya);
 }
@@ -293,7 +298,8 @@ __device__ __forceinline__ float ord_dec(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
-__global__ void k_box_init(uint32_t* __restrict__ box, uint64_t count) {
+__global__ void k_box_init(uint32_t* __restrict__ box, uint64_t count, const unsigned long long* __restrict__ counters) {
+  if (counters[C_HEAVY] == 0ull) return;  // the boxes serve only the wavefront builder
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) box[i] = (i % 6 < 3) ? 0xFFFFFFFFu : 0u;
 }
@@ -302,7 +308,9 @@ __global__ void k_box_init(uint32_t* __restrict__ box, uint64_t count) {
 // the first lane of a run updates the cell's box.
 __global__ void __launch_bounds__(kBlock) k_cell_box(uint32_t n, uint32_t ncells, const uint32_t* __restrict__ key,
                                                      const float4* __restrict__ xs, uint32_t* __restrict__ box,
-                                                     float4* __restrict__ chbox) {
+                                                     float4* __restrict__ chbox,
+                                                     const unsigned long long* __restrict__ counters) {
+  if (counters[C_HEAVY] == 0ull) return;
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool valid = p < n;
@@ -371,7 +379,9 @@ __global__ void __launch_bounds__(kBlock) k_cell_box(uint32_t n, uint32_t ncells
 __global__ void __launch_bounds__(kBlock) k_cell_box_chunks(uint32_t n, uint32_t ncells,
                                                             const uint32_t* __restrict__ key,
                                                             const float4* __restrict__ chbox,
-                                                            uint32_t* __restrict__ box) {
+                                                            uint32_t* __restrict__ box,
+                                                            const unsigned long long* __restrict__ counters) {
+  if (counters[C_HEAVY] == 0ull) return;
   const uint32_t ch = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const uint32_t nch = (n + 63) / 64;
@@ -419,11 +429,12 @@ void launch_cell_box(pbf_context* c) {
   if (!n) return;
   KernelTimer* t = timer_begin(c, "cell_box");
   const uint64_t count = 6ull * c->K.ncells;
-  hipLaunchKernelGGL(k_box_init, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, c->cbox, count);
+  const unsigned long long* heavy = c->counters;
+  hipLaunchKernelGGL(k_box_init, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, c->cbox, count, heavy);
   hipLaunchKernelGGL(k_cell_box, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, c->K.ncells,
-                     (const uint32_t*)c->key, (const float4*)c->xs, c->cbox, c->chbox);
+                     (const uint32_t*)c->key, (const float4*)c->xs, c->cbox, c->chbox, heavy);
   hipLaunchKernelGGL(k_cell_box_chunks, dim3(nblocks((n + 63) / 64)), dim3(kBlock), 0, c->stream, n, c->K.ncells,
-                     (const uint32_t*)c->key, (const float4*)c->chbox, c->cbox);
+                     (const uint32_t*)c->key, (const float4*)c->chbox, c->cbox, heavy);
   timer_end(c, t);
 }
 
@@ -1007,6 +1018,8 @@ void launch_fill_neighbors(pbf_context* c, uint64_t fb, uint64_t fe, bool want_s
                      (const uint32_t*)c->cbox, c->K, c->nbr, c->nbr_count, c->heavy, c->counters, skin,
                      (const unsigned long long*)c->skin_hint, c->snb, c->snb_count, c->xs0);
   timer_end(c, t);
+  // the cell and chunk boxes of the wavefront builder, built only when some particle went to it
+  launch_cell_box(c);
   // enough waves to cover the chip; each loops over the heavy list (usually empty)
   t = timer_begin(c, "fill_heavy");
   const uint32_t blocks = std::min<uint32_t>(nblocks((uint64_t)n * 64), c->tune.heavy_blocks);

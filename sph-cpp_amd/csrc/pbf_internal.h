@@ -268,7 +268,9 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32
 void launch_bin(pbf_context* c, uint64_t bin_end);  // key, inverse permutation, keys by original index
 void launch_sort(pbf_context* c);            // radix sort -> perm, cell_start
 void launch_cell_box(pbf_context* c);        // bounding box of each cell's particles
-void launch_reorder(pbf_context* c);         // gather state by perm
+// gather state by perm (without all_fields: densities, multipliers and the
+// scratch vector stay where they are)
+void launch_reorder(pbf_context* c, bool all_fields = true);
 uint64_t radix_hist_size(uint64_t n);
 int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint64_t n, uint32_t key_bits,
                      uint32_t* hist, uint32_t* offs, uint32_t* partials, hipStream_t s);

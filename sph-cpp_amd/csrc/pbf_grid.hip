@@ -672,6 +672,7 @@ __device__ __forceinline__ bool tile_walk(uint32_t tb, uint32_t lane, uint32_t n
     if (seg >= kTileSegs) continue;
     // the segment cell's coordinates in its grid (its scene's)
     const GridView G = grid_of(K, tb + s0);
+    if (G.dxy > 0xFFFFFFFFull) continue;  // (a layer of 2^32 cells: coordinates need 64-bit division)
     const uint32_t local = ck - G.off;
     const uint32_t dxy = (uint32_t)G.dxy;
     const uint32_t bz = local / dxy, rem = local - bz * dxy;

@@ -626,7 +626,10 @@ done:
 // particle that is not covered takes the lane walk for all its particles.
 constexpr uint32_t kTileRounds = 8;
 constexpr uint32_t kTileMaxCand = 64u * kTileRounds;
-constexpr uint32_t kTileSegs = 16;
+// (12 segment tables keep the wavefront's LDS under 10 KB, so that 16
+// wavefronts share a CU, as many as the registers allow; 16 tables cover a few
+// more particles at rest but fit only 14)
+constexpr uint32_t kTileSegs = 12;
 constexpr uint32_t kTileTab = 28;  // per segment: 27 cell starts; 28 stream offsets (the last is the total)
 
 // v_writelane_b32: lane `lane` of v takes the uniform value `val`

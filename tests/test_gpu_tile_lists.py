@@ -99,11 +99,12 @@ def test_tile_walk_caps_and_overflow(pbf, scenes, cap):
 
 
 def test_tile_walk_mixed_with_crowded_cells(pbf, scenes):
-    """Dense clusters (cells past the tile walk's candidate limit, listed by
-    the lane and wavefront walks) next to an ordinary lattice."""
+    """A dense cluster (cells past the tile walk's candidate limit, listed by
+    the lane and wavefront walks) below an ordinary lattice (cell layers of
+    their own in the z-major cell order, listed by the tile walk)."""
     rng = np.random.default_rng(21)
     dense = rng.uniform(0.30, 0.40, size=(3000, 3)).astype(np.float32)
-    sparse = scenes.lattice((8, 8, 8), origin=(0.5, 0.05, 0.05))
+    sparse = scenes.lattice((16, 16, 4), origin=(0.05, 0.05, 0.6))
     pos = np.concatenate([dense, sparse]).astype(np.float32)
     p = scenes.pbf_params((1.0, 1.0, 1.0), pos.shape[0], count_overflow=True)
     t, l = _pair(pbf, p, pos, np.zeros_like(pos))

@@ -795,12 +795,12 @@ static void step_body(pbf_context* c) {
   const uint64_t n = c->n;
   {
     // the multiplier sums move with the reorder inside find_neighbors
-    launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
     if (c->capturing) {  // stamps repeat on every replay
       launch_fill_u32(c->dirty, 0u, n, c->stream);
       launch_fill_u32(c->mflag, 0u, n, c->stream);
     }
-    launch_forces_predict(c, F_FORCES | F_PREDICT, 0, n);
+    if (!n) launch_fill_u64(c->counters, 0ull, C_NUM, c->stream);
+    launch_forces_predict(c, F_FORCES | F_PREDICT | F_CLEAR, 0, n);  // (counters zeroed there)
     find_neighbors_impl(c, n, 0, n, true);
     if (c->track_lsum) launch_fill_u32((uint32_t*)c->lsum, 0u, n, c->stream);
     const SolveCtl S{-1.0f, 0, 0, nullptr, nullptr, c->track_lsum ? c->lsum : nullptr, nullptr};
@@ -1532,8 +1532,7 @@ int pbf_region_end(pbf_context* c, double* ms) {
 namespace pbf {
 
 static void find_neighbors_impl(pbf_context* c, uint64_t bin_end, uint64_t fb, uint64_t fe, bool skin) {
-  launch_fill_u64(c->counters + C_HEAVY, 0ull, 1, c->stream);
-  launch_bin(c, bin_end);
+  launch_bin(c, bin_end);  // (clears the list build's counters)
   launch_sort(c);
   // a fused step recomputes densities, multipliers and the scratch vector
   // before reading them (with at least one Jacobi iteration): they stay unsorted

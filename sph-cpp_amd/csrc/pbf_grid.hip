@@ -147,7 +147,10 @@ __global__ void __launch_bounds__(kBlock) k_bin(const float4* __restrict__ xs, u
                                                 uint32_t* __restrict__ key, uint32_t* __restrict__ inv,
                                                 unsigned long long* __restrict__ counters) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) counters[C_TILE] = 0ull;  // counted by the list build that follows
+  if (p == 0) {  // counted by the list build that follows
+    counters[C_HEAVY] = 0ull;
+    counters[C_TILE] = 0ull;
+  }
   if (p >= n) return;
   const uint32_t o = oid[p];
   uint32_t k = K.ncells;  // particles past the bin span are placed in no cell

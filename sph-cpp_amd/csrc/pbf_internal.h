@@ -281,7 +281,7 @@ void launch_bounds_counting(const uint32_t* sorted, uint64_t n, uint64_t b, uint
 
 // ---- launchers (pbf_passes.hip) ----
 // flags for the particle-local kernel
-enum { F_FORCES = 1, F_PREDICT = 2 };
+enum { F_FORCES = 1, F_PREDICT = 2, F_CLEAR = 4 };  // F_CLEAR: zero the counters first
 void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e);
 void launch_density(pbf_context* c, uint64_t b, uint64_t e);  // xs -> xt, rho
 // ---- fused-step launchers (pbf_fused.hip): they build and use the compact lists ----

@@ -27,8 +27,10 @@ __device__ __forceinline__ bool in_span(uint32_t o, uint32_t b, uint32_t e) { re
 __global__ void __launch_bounds__(kBlock) k_forces_predict(uint32_t n, uint32_t b, uint32_t e, int flags, KParams K,
                                                            const uint32_t* __restrict__ oid,
                                                            const float4* __restrict__ x, float4* __restrict__ v,
-                                                           float4* __restrict__ xs) {
+                                                           float4* __restrict__ xs,
+                                                           unsigned long long* __restrict__ counters) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((flags & F_CLEAR) && p < (uint32_t)C_NUM) counters[p] = 0ull;  // a fused step's counters start at zero
   if (p >= n || !in_span(oid[p], b, e)) return;
   f3 vp = ld3(v, p);
   if (flags & F_FORCES) {
@@ -47,7 +49,7 @@ void launch_forces_predict(pbf_context* c, int flags, uint64_t b, uint64_t e) {
   if (!n) return;
   KernelTimer* t = timer_begin(c, "forces_predict");
   hipLaunchKernelGGL(k_forces_predict, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (uint32_t)b, (uint32_t)e,
-                     flags, c->K, (const uint32_t*)c->oid, (const float4*)c->x, c->v, c->xs);
+                     flags, c->K, (const uint32_t*)c->oid, (const float4*)c->x, c->v, c->xs, c->counters);
   timer_end(c, t);
 }
 

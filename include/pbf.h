@@ -110,6 +110,15 @@ typedef struct pbf_params {
  * the same, bit for bit.  This option keeps the per-lane walk for every particle (for
  * comparison). */
 #define PBF_OPT_LANE_LISTS 64u
+/* pbf_step and pbf_solve run their Jacobi iterations (the density, dirty-marking,
+ * lambda-repair and pressure-update passes of every iteration or solve slot) in one
+ * cooperative launch with grid-wide barriers between the passes, instead of one launch per
+ * pass; a solve stops on the device.  Same results, bit for bit.  Used for up to 256
+ * particles per compute unit (65,536 on a 256-CU device: one block of the launch per CU);
+ * larger contexts, and graphs being captured (pbf_step_graph, pbf_solve_graph), keep the
+ * launches per pass (pbf_stats.persistent_grid tells which ran).  A device without
+ * cooperative launches makes pbf_step and pbf_solve fail with PBF_ERR_UNSUPPORTED. */
+#define PBF_OPT_PERSISTENT 128u
 
 typedef struct pbf_context pbf_context;
 
@@ -126,6 +135,8 @@ typedef struct pbf_stats {
   double max_density_error;    /* max_p max(rho_p/rho0 - 1, 0) of the last density pass */
   uint64_t tile_listed;        /* particles the last list build listed with the cell-tile walk
                                   (PBF_OPT_LANE_LISTS); counted only with PBF_OPT_COUNT_OVERFLOW */
+  uint64_t persistent_grid;    /* blocks of the last cooperative Jacobi launch (PBF_OPT_PERSISTENT; 0: none) */
+  uint64_t persistent_capacity; /* blocks of that kernel the device runs at once (0 without the option) */
 } pbf_stats;
 
 /* Thread-local message for the last failing call. */

@@ -90,6 +90,8 @@ class Stats(C.Structure):
         ("iterations", C.c_uint64),
         ("max_density_error", C.c_double),
         ("tile_listed", C.c_uint64),
+        ("persistent_grid", C.c_uint64),
+        ("persistent_capacity", C.c_uint64),
     ]
 
 
@@ -101,6 +103,7 @@ PBF_OPT_FAST_MATH = 8
 PBF_OPT_WARM_STRICT = 16
 PBF_OPT_FULL_LISTS = 32
 PBF_OPT_LANE_LISTS = 64
+PBF_OPT_PERSISTENT = 128
 PBF_WARM_FEATURES = 8
 PBF_WARM_HIDDEN = 64
 PBF_WARM_PARAMS = 4801

@@ -790,6 +790,20 @@ static void step_tail(pbf_context* c) {
   launch_viscosity_last(c);          // XSPH 2: va -> v, x = x*
 }
 
+// Whether the Jacobi iterations go to one cooperative launch: asked for, not
+// while a graph is being captured, and the particles fit one launch.
+static bool persistent_jacobi(pbf_context* c) {
+  return (c->P.options & PBF_OPT_PERSISTENT) && !c->capturing && c->n && persistent_fits(c);
+}
+
+// PBF_ERR_UNSUPPORTED when the persistent Jacobi kernel is asked for and the
+// device cannot launch it.
+static int check_persistent(pbf_context* c) {
+  if ((c->P.options & PBF_OPT_PERSISTENT) && !persistent_available(c))
+    return set_error(PBF_ERR_UNSUPPORTED, "PBF_OPT_PERSISTENT: no cooperative launches on this device");
+  return PBF_OK;
+}
+
 // Simulation/Source/main.cpp:71-124 on one domain, fused kernels.
 static void step_body(pbf_context* c) {
   const uint64_t n = c->n;
@@ -805,7 +819,9 @@ static void step_body(pbf_context* c) {
     if (c->track_lsum) launch_fill_u32((uint32_t*)c->lsum, 0u, n, c->stream);
     const SolveCtl S{-1.0f, 0, 0, nullptr, nullptr, c->track_lsum ? c->lsum : nullptr, nullptr};
     const uint32_t I = c->P.solve_step_count;
-    for (uint32_t it = 0; it < I; ++it) {
+    c->coop_grid = 0;
+    if (I && persistent_jacobi(c)) launch_jacobi_persistent(c, I, false, true, S);
+    else for (uint32_t it = 0; it < I; ++it) {
       ++c->dirty_stamp;
       launch_density_lambda_masked(c, (int)it, 0, S);       // xs -> xt, rho, lam, compact lists
       launch_mark_dirty(c, (int)it, S);                     // only if the density fix fired
@@ -822,6 +838,7 @@ static void step_body(pbf_context* c) {
 int pbf_step(pbf_context* c, uint32_t nsteps) {
   PBF_TRACE("pbf_step");
   if (!c) return set_error(PBF_ERR_ARG, "null context");
+  if (int rc = check_persistent(c)) return rc;
   for (uint32_t st = 0; st < nsteps; ++st) step_body(c);
   return check_hip(hipGetLastError(), "pbf_step");
 }
@@ -834,6 +851,7 @@ int pbf_step(pbf_context* c, uint32_t nsteps) {
 int pbf_step_graph(pbf_context* c, uint32_t nsteps) {
   PBF_TRACE("pbf_step_graph");
   if (!c) return set_error(PBF_ERR_ARG, "null context");
+  if (int rc = check_persistent(c)) return rc;  // an odd count's last step
   int rc;
   if (nsteps >= 2 && (!c->graph_exec || c->graph_n != c->n || c->graph_version != c->version ||
                       c->graph_sig != buffer_sig(c))) {
@@ -932,7 +950,9 @@ static int solve_body(pbf_context* c, float tolerance, uint32_t max_iterations, 
     launch_warm_dp(c);
   }
   constexpr uint32_t kChunk = 8;
-  for (uint32_t it = 0; it < max_iterations; ++it) {
+  c->coop_grid = 0;
+  if (persistent_jacobi(c)) launch_jacobi_persistent(c, max_iterations, true, false, S);
+  else for (uint32_t it = 0; it < max_iterations; ++it) {
     ++c->dirty_stamp;
     launch_density_lambda_masked(c, (int)it, (int)it + 1, S);
     launch_mark_dirty(c, (int)it, S);
@@ -961,6 +981,7 @@ int pbf_solve(pbf_context* c, float tolerance, uint32_t max_iterations, int warm
   PBF_TRACE("pbf_solve");
   int rc = solve_check_args(c, tolerance, max_iterations, warm_start);
   if (rc) return rc;
+  if ((rc = check_persistent(c))) return rc;
   if ((rc = solve_body(c, tolerance, max_iterations, warm_start, true))) return rc;
   return check_hip(hipGetLastError(), "pbf_solve");
 }
@@ -973,6 +994,7 @@ int pbf_solve_graph(pbf_context* c, float tolerance, uint32_t max_iterations, in
   PBF_TRACE("pbf_solve_graph");
   int rc = solve_check_args(c, tolerance, max_iterations, warm_start);
   if (rc) return rc;
+  if ((rc = check_persistent(c))) return rc;  // an odd count's last step
   if (!c->track_lsum) {
     // the captured steps read the previous sums: start them at zero here, so
     // that both captured steps see kept sums
@@ -1201,6 +1223,7 @@ struct SavedState {
 // Mean ms of `steps` fused steps from the saved state (one untimed step first).
 static int time_steps(pbf_context* c, SavedState& S, uint32_t steps, hipEvent_t a, hipEvent_t b, double* ms_out) {
   int rc;
+  if ((rc = check_persistent(c))) return rc;
   if ((rc = S.restore())) return rc;
   step_body(c);  // first launches of this setting, untimed
   if ((rc = S.restore())) return rc;
@@ -1353,6 +1376,8 @@ int pbf_get_stats(pbf_context* c, pbf_stats* out) {
   out->outside_grid = h[C_OUTSIDE];
   out->density_hacks = h[C_HACKS];
   out->tile_listed = h[C_TILE];
+  out->persistent_grid = c->coop_grid;
+  out->persistent_capacity = (c->P.options & PBF_OPT_PERSISTENT) ? persistent_capacity(c) : 0;
   out->iterations = c->last_was_solve ? h[C_ITERS] : c->last_iterations;
   double m = 0.0;
   for (float r : rho) {

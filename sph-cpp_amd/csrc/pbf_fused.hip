@@ -23,6 +23,8 @@
 //    x*_p in the middle of the walk; when it fires, the density pass finishes
 //    that particle with a plain walk and raises the iteration's flag, and the
 //    lambda and dp passes of that iteration fall back to plain walks.
+#include <hip/hip_cooperative_groups.h>
+
 #include "pbf_mlp_device.h"
 
 namespace pbf {
@@ -271,9 +273,14 @@ __device__ __forceinline__ void warm_network(const FeatCtl& F, uint32_t wave_fir
 // FM = 0: density and lambda only; 1: also the warm start's features (stored
 // when F.feat is set); 2: also the network on them (F.wlam).  FAST: the
 // PBF_OPT_FAST_MATH arithmetic.
+// The pass bodies below take their block of particles (vb) and whether the
+// calling thread is the one that clears the pass's flags (first): the same
+// bodies run as their own kernels (vb from xcd_block()) and inside the
+// persistent Jacobi kernel (vb from the calling block's share).
 template <int FM, bool FAST>
-__global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S, FeatCtl F,
-                                                             KParams K, const float4* __restrict__ xs,
+__device__ __forceinline__ void density_pass(uint32_t vb, bool first, uint32_t n, int it, int eslot,
+                                             const SolveCtl& S, const FeatCtl& F,
+                                                             const KParams& K, const float4* __restrict__ xs,
                                                              const float4* __restrict__ v, float4* __restrict__ xt,
                                                              float* __restrict__ rho, float* __restrict__ lam,
                                                              const int32_t* __restrict__ nbr,
@@ -285,10 +292,10 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
                                                              const int32_t* __restrict__ snb,
                                                              const int32_t* __restrict__ snb_count) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
-  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  const uint32_t p = vb * blockDim.x + threadIdx.x;
   // this iteration's pressure update records its moves in the other slot
   const bool skin = FM == 0 && skin_usable(counters, C_DISP + (it & 1));
-  if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_DISP + ((it + 1) & 1)] = 0ull;
+  if (first) counters[C_DISP + ((it + 1) & 1)] = 0ull;
   float e = 0.0f;
   uint32_t hacks = 0;
   float fo[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -330,6 +337,23 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
   }
 }
 
+template <int FM, bool FAST>
+__global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S, FeatCtl F,
+                                                             KParams K, const float4* __restrict__ xs,
+                                                             const float4* __restrict__ v, float4* __restrict__ xt,
+                                                             float* __restrict__ rho, float* __restrict__ lam,
+                                                             const int32_t* __restrict__ nbr,
+                                                             const int32_t* __restrict__ nbr_count,
+                                                             uint64_t* __restrict__ wmask,
+                                                             uint32_t* __restrict__ moved, uint32_t stamp,
+                                                             uint32_t* __restrict__ mflag,
+                                                             unsigned long long* __restrict__ counters,
+                                                             const int32_t* __restrict__ snb,
+                                                             const int32_t* __restrict__ snb_count) {
+  density_pass<FM, FAST>(xcd_block(), blockIdx.x == 0 && threadIdx.x == 0, n, it, eslot, S, F, K, xs, v, xt, rho,
+                         lam, nbr, nbr_count, wmask, moved, stamp, mflag, counters, snb, snb_count);
+}
+
 // A particle is dirty when it or a particle in its list was moved by the
 // density fix: its fused lambda and its within-h mask may be stale.  A list
 // walk visits cells computed from floating-point coordinates (x + i*s,
@@ -343,21 +367,18 @@ constexpr uint32_t kMaxMovedMarked = 4096;
 constexpr uint32_t kMaxMarkedPerMoved = 8192;
 constexpr uint32_t kMarkBlocks = 64;
 
-__global__ void __launch_bounds__(kBlock) k_mark_dirty(int it, uint32_t stamp, SolveCtl S, KParams K,
-                                                       const uint32_t* __restrict__ key,
-                                                       const uint32_t* __restrict__ start,
-                                                       const uint32_t* __restrict__ moved,
-                                                       uint32_t* __restrict__ dirty,
-                                                       unsigned long long* __restrict__ counters) {
+// (wave, nwaves: the calling wavefront among all wavefronts that share the pass)
+__device__ __forceinline__ void mark_pass(uint32_t wave, uint32_t nwaves, bool first, int it, uint32_t stamp,
+                                          const SolveCtl& S, const KParams& K, const uint32_t* __restrict__ key,
+                                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ moved,
+                                          uint32_t* __restrict__ dirty, unsigned long long* __restrict__ counters) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   const uint32_t nmoved = (uint32_t)counters[C_MOVED_N + (it & 1)];
   if (nmoved > kMaxMovedMarked) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_MOVED_ALL + (it & 1)] = 1ull;
+    if (first) counters[C_MOVED_ALL + (it & 1)] = 1ull;
     return;
   }
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   for (uint32_t i = wave; i < nmoved; i += nwaves) {
     const uint32_t q = moved[i];
     const uint32_t kq = key[q];
@@ -392,6 +413,16 @@ __global__ void __launch_bounds__(kBlock) k_mark_dirty(int it, uint32_t stamp, S
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_mark_dirty(int it, uint32_t stamp, SolveCtl S, KParams K,
+                                                       const uint32_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ start,
+                                                       const uint32_t* __restrict__ moved,
+                                                       uint32_t* __restrict__ dirty,
+                                                       unsigned long long* __restrict__ counters) {
+  mark_pass((blockIdx.x * blockDim.x + threadIdx.x) >> 6, (gridDim.x * blockDim.x) >> 6,
+            blockIdx.x == 0 && threadIdx.x == 0, it, stamp, S, K, key, start, moved, dirty, counters);
+}
+
 // compute_pressure_lambdas (particles.h:318-342) from xt for the dirty
 // particles of this iteration (plain walks).  The new lambda goes to lam and
 // to the particle's xt.w; positions are read as three words, so no lane reads
@@ -405,8 +436,8 @@ __device__ __forceinline__ f3 ld3_words(const float4* a, uint32_t i) {
 }
 
 template <bool FAST>
-__global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, uint32_t stamp, SolveCtl S,
-                                                          KParams K, float4* __restrict__ xt,
+__device__ __forceinline__ void repair_pass(uint32_t vb, uint32_t n, int it, uint32_t stamp, const SolveCtl& S,
+                                                          const KParams& K, float4* __restrict__ xt,
                                                           const float* __restrict__ rho, float* __restrict__ lam,
                                                           const int32_t* __restrict__ nbr,
                                                           const int32_t* __restrict__ nbr_count,
@@ -416,7 +447,7 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   if (counters[C_HACK_FLAG + (it & 1)] == 0ull) return;
   const bool scan = counters[C_MOVED_ALL + (it & 1)] != 0ull;
-  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  const uint32_t p = vb * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int cnt = nbr_count[p];
   if (scan) {
@@ -460,6 +491,18 @@ __global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, ui
   reinterpret_cast<float*>(xt + p)[3] = lp;
 }
 
+template <bool FAST>
+__global__ void __launch_bounds__(kBlock) k_lambda_repair(uint32_t n, int it, uint32_t stamp, SolveCtl S,
+                                                          KParams K, float4* __restrict__ xt,
+                                                          const float* __restrict__ rho, float* __restrict__ lam,
+                                                          const int32_t* __restrict__ nbr,
+                                                          const int32_t* __restrict__ nbr_count,
+                                                          const uint32_t* __restrict__ mflag,
+                                                          uint32_t* __restrict__ dirty,
+                                                          const unsigned long long* __restrict__ counters) {
+  repair_pass<FAST>(xcd_block(), n, it, stamp, S, K, xt, rho, lam, nbr, nbr_count, mflag, dirty, counters);
+}
+
 __device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) {
   return (float)((double)sum / 4294967296.0 / (double)n);
 }
@@ -471,8 +514,9 @@ __device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) 
 // before the warm start) restores xsave; an error <= tol ends the solve with
 // x* = xt.  Neither applies an update.
 template <bool FAST>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, SolveCtl S,
-                                                        KParams K, const float4* __restrict__ xt,
+__device__ __forceinline__ void dp_pass(uint32_t vb, bool first, uint32_t n, int it, int last, uint32_t stamp,
+                                        const SolveCtl& S,
+                                                        const KParams& K, const float4* __restrict__ xt,
                                                         const float4* __restrict__ x, float4* __restrict__ xs,
                                                         float4* __restrict__ v, const int32_t* __restrict__ nbr,
                                                         const int32_t* __restrict__ nbr_count,
@@ -490,14 +534,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     const unsigned long long done = counters[C_DONE];
     if (done != 0ull && done <= (unsigned long long)it) return;
   }
-  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   // clear the next iteration's flags (nobody reads them until the next density pass)
   if (first) {
     counters[C_HACK_FLAG + ((it + 1) & 1)] = 0ull;
     counters[C_MOVED_N + ((it + 1) & 1)] = 0ull;
     counters[C_MOVED_ALL + ((it + 1) & 1)] = 0ull;
   }
-  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  const uint32_t p = vb * blockDim.x + threadIdx.x;
   // what this slot does, the same for every thread: 0 the update, 1 the
   // rejected warm start's restore, 2 the end of the solve (x* = xt)
   int mode = 0;
@@ -579,6 +622,23 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
     }
   }
   if (track) record_move(mv, counters, C_DISP + ((it + 1) & 1), K);
+}
+
+template <bool FAST>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, SolveCtl S,
+                                                        KParams K, const float4* __restrict__ xt,
+                                                        const float4* __restrict__ x, float4* __restrict__ xs,
+                                                        float4* __restrict__ v, const int32_t* __restrict__ nbr,
+                                                        const int32_t* __restrict__ nbr_count,
+                                                        const uint64_t* __restrict__ wmask,
+                                                        const uint32_t* __restrict__ dirty,
+                                                        unsigned long long* __restrict__ counters,
+                                                        const int32_t* __restrict__ snb,
+                                                        const int32_t* __restrict__ snb_count,
+                                                        const float4* __restrict__ xs0,
+                                                        const float* __restrict__ rho) {
+  dp_pass<FAST>(xcd_block(), blockIdx.x == 0 && threadIdx.x == 0, n, it, last, stamp, S, K, xt, x, xs, v, nbr,
+                nbr_count, wmask, dirty, counters, snb, snb_count, xs0, rho);
 }
 
 // Warm start of an iterate-to-tolerance solve: one compute_pressure_dps +
@@ -885,6 +945,116 @@ void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S
 void launch_warm_dp(pbf_context* c) {
   PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "warm_dp", k_warm_dp, c->K, (const float4*)c->xt, (const float*)c->wlam, (const int32_t*)c->nbr,
              (const int32_t*)c->nbr_count, c->xs, c->xsave, c->counters, (const float4*)c->xs0);
+}
+
+// ---------------------------------------------------------------------------
+// The Jacobi iterations of a step or the slots of a solve in one cooperative
+// launch (PBF_OPT_PERSISTENT): each iteration runs the density, dirty-marking,
+// lambda-repair and pressure-update bodies above over the whole particle
+// range, separated by grid-wide barriers instead of kernel boundaries, and a
+// solve leaves the loop on the device once its slot met the tolerance (every
+// block reads the done flag after the same barrier).  Each block takes a
+// contiguous share of the 256-particle blocks, its share placed on its XCD
+// (xcd_block()).  The host launches it with one 256-particle block per block
+// and at most one block per CU (persistent_fits), far inside what the chip
+// holds at once, so every block is resident whatever else the device runs.
+struct JacobiArgs {
+  uint32_t n, iters;
+  int solve, last_vel;  // solve: slot it's errors go to slot it + 1; last_vel: velocities in the last iteration
+  uint32_t stamp0;      // stamp of the first iteration (then one more per iteration)
+  SolveCtl S;
+  KParams K;
+  float4 *xs, *v, *xt;
+  const float4 *x, *xs0;
+  float *rho, *lam;
+  const int32_t *nbr, *nbr_count, *snb, *snb_count;
+  uint64_t* wmask;
+  uint32_t *moved, *mflag, *dirty;
+  const uint32_t *key, *start;
+  unsigned long long* counters;
+};
+
+template <bool FAST>
+__global__ void __launch_bounds__(kBlock) k_jacobi_persistent(JacobiArgs A) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const uint32_t nvb = (A.n + blockDim.x - 1) / blockDim.x;
+  const uint32_t per = (nvb + gridDim.x - 1) / gridDim.x;
+  const uint32_t vb0 = xcd_block() * per, vb1 = min(nvb, vb0 + per);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+  const FeatCtl F0{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0f, nullptr};
+  for (uint32_t it = 0; it < A.iters; ++it) {
+    if (A.S.tol >= 0.0f && __atomic_load_n(&A.counters[C_DONE], __ATOMIC_RELAXED) != 0ull) break;
+    const uint32_t stamp = A.stamp0 + it;
+    const int eslot = A.solve ? (int)it + 1 : 0;
+    for (uint32_t vb = vb0; vb < vb1; ++vb)
+      density_pass<0, FAST>(vb, vb == 0 && threadIdx.x == 0, A.n, (int)it, eslot, A.S, F0, A.K, A.xs, A.v, A.xt,
+                            A.rho, A.lam, A.nbr, A.nbr_count, A.wmask, A.moved, stamp, A.mflag, A.counters, A.snb,
+                            A.snb_count);
+    grid.sync();
+    mark_pass(wave, nwaves, wave == 0 && threadIdx.x == 0, (int)it, stamp, A.S, A.K, A.key, A.start, A.moved,
+              A.dirty, A.counters);
+    grid.sync();
+    for (uint32_t vb = vb0; vb < vb1; ++vb)
+      repair_pass<FAST>(vb, A.n, (int)it, stamp, A.S, A.K, A.xt, A.rho, A.lam, A.nbr, A.nbr_count, A.mflag,
+                        A.dirty, A.counters);
+    grid.sync();
+    const int last = A.last_vel && it + 1 == A.iters;
+    for (uint32_t vb = vb0; vb < vb1; ++vb)
+      dp_pass<FAST>(vb, vb == 0 && threadIdx.x == 0, A.n, (int)it, last, stamp, A.S, A.K, A.xt, A.x, A.xs, A.v,
+                    A.nbr, A.nbr_count, A.wmask, A.dirty, A.counters, A.snb, A.snb_count, A.xs0, A.rho);
+    grid.sync();
+  }
+}
+
+// Blocks of one cooperative launch of the persistent kernel on the context's
+// device (0: cooperative launches unavailable), found once per context.
+static uint32_t persistent_blocks(pbf_context* c) {
+  int& blocks = c->coop_blocks[c->K.fast ? 1 : 0];
+  if (blocks >= 0) return (uint32_t)blocks;
+  blocks = 0;
+  int coop = 0, cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) != hipSuccess || !coop)
+    return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) return 0;
+  c->coop_cus = cus;
+  const void* kern = c->K.fast ? (const void*)k_jacobi_persistent<true> : (const void*)k_jacobi_persistent<false>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess) return 0;
+  blocks = std::max(0, per_cu * cus);
+  return (uint32_t)blocks;
+}
+
+uint32_t persistent_capacity(pbf_context* c) { return persistent_blocks(c); }
+
+// The cooperative launch is used for at most one block per CU (particles up
+// to 256 x CUs): a launch of the full co-resident capacity (5 blocks per CU at
+// this kernel's registers) did not finish on the 1M-particle dam break.
+bool persistent_fits(pbf_context* c) {
+  const uint32_t cap = persistent_blocks(c);
+  const uint64_t nvb = (c->n + kBlock - 1) / kBlock;
+  return cap > 0 && nvb <= (uint64_t)std::min<uint32_t>((uint32_t)c->coop_cus, cap);
+}
+
+bool persistent_available(pbf_context* c) { return persistent_blocks(c) > 0; }
+
+int launch_jacobi_persistent(pbf_context* c, uint32_t iters, bool solve, bool last_vel, const SolveCtl& S) {
+  const uint32_t n = (uint32_t)c->n;
+  if (!n || !iters) return PBF_OK;
+  const uint32_t nvb = (n + kBlock - 1) / kBlock;
+  const uint32_t blocks = std::min<uint32_t>(persistent_blocks(c), nvb);
+  if (!blocks) return set_error(PBF_ERR_UNSUPPORTED, "cooperative launches are not available on this device");
+  JacobiArgs A{n, iters, solve ? 1 : 0, last_vel ? 1 : 0, c->dirty_stamp + 1, S, c->K, c->xs, c->v, c->xt,
+               (const float4*)c->x, (const float4*)c->xs0, c->rho, c->lam, (const int32_t*)c->nbr,
+               (const int32_t*)c->nbr_count, (const int32_t*)c->snb, (const int32_t*)c->snb_count, c->wmask,
+               c->moved, c->mflag, c->dirty, (const uint32_t*)c->key, (const uint32_t*)c->cell_start,
+               c->counters};
+  void* args[] = {&A};
+  KernelTimer* t = timer_begin(c, "jacobi_persistent");
+  const void* kern = c->K.fast ? (const void*)k_jacobi_persistent<true> : (const void*)k_jacobi_persistent<false>;
+  c->coop_grid = blocks;
+  const hipError_t e = hipLaunchCooperativeKernel(kern, dim3(blocks), dim3(kBlock), args, 0, c->stream);
+  timer_end(c, t);
+  c->dirty_stamp += iters;
+  return check_hip(e, "jacobi_persistent");
 }
 
 // The record of the solve that just ran (pbf_get_solve_log): slots used,

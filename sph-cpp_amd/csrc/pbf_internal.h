@@ -227,6 +227,10 @@ struct pbf_context {
   pbf_solve_record* slog = nullptr;
   unsigned long long* slog_count = nullptr;
   pbf_tuning tune{256u, 768u, 8192u};
+  int coop_blocks[2] = {-1, -1};  // blocks of a cooperative launch of the persistent Jacobi kernel, exact and
+                                  // fast arithmetic (-1: not yet found)
+  int coop_cus = 0;               // compute units of the device (found with coop_blocks)
+  uint32_t coop_grid = 0;         // blocks of the last step's cooperative launch (0: none)
   unsigned long long* done_host = nullptr;  // pinned copy of C_DONE, polled between slot chunks
   float *lsum = nullptr, *lsuma = nullptr, *wlam = nullptr, *wgiven = nullptr;
   float4* xsave = nullptr;
@@ -293,6 +297,11 @@ void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S
 void launch_warm_dp(pbf_context* c);      // xt, wlam -> xs (warm start), xsave = xt
 void launch_add_warm(pbf_context* c);     // lsum += wlam when the warm start was kept
 void launch_log_solve(pbf_context* c);    // one pbf_solve_record of the solve just queued
+// the Jacobi loop in one cooperative launch (PBF_OPT_PERSISTENT); advances dirty_stamp by iters
+bool persistent_available(pbf_context* c);
+uint32_t persistent_capacity(pbf_context* c);  // blocks of it the device holds at once (occupancy query)
+bool persistent_fits(pbf_context* c);          // the particles fit one launch of it (one block per CU at most)
+int launch_jacobi_persistent(pbf_context* c, uint32_t iters, bool solve, bool last_vel, const SolveCtl& S);
 void launch_viscosity_first(pbf_context* c);                  // v -> va, compact lists of the final positions
 void launch_vorticity_masked(pbf_context* c);                 // va -> scr
 void launch_apply_vorticity_masked(pbf_context* c);           // va += ...

@@ -191,8 +191,8 @@ KERNELS = ["forces_predict", "bin", "scan", "sort", "reorder", "cell_box", "fill
 HIP_KERNEL = {
     "forces_predict": "pbf::k_forces_predict", "bin": "pbf::k_bin", "reorder": "pbf::k_reorder",
     "cell_box": "pbf::k_cell_box", "fill_neighbors": "pbf::k_fill_neighbors", "fill_heavy": "pbf::k_fill_heavy",
-    "density_lambda": "pbf::k_density_lambda_c<0, false>", "mark_dirty": "pbf::k_mark_dirty",
-    "lambda_repair": "pbf::k_lambda_repair<false>", "dp_update": "pbf::k_dp_update_c<false>",
+    "density_lambda": "pbf::k_density_lambda_c<0, false, true>", "mark_dirty": "pbf::k_mark_dirty",
+    "lambda_repair": "pbf::k_lambda_repair<false>", "dp_update": "pbf::k_dp_update_c<false, true>",
     "viscosity": "pbf::k_viscosity_first", "vorticity": "pbf::k_vorticity_c",
     "apply_vorticity": "pbf::k_apply_vorticity_c", "viscosity_positions": "pbf::k_viscosity_last",
 }

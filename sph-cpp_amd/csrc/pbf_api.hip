@@ -139,6 +139,18 @@ static int compute_kparams(const pbf_params& P, KParams& K, uint64_t dims[3]) {
   K.spiky_norm = static_cast<float>(-45.0 / (M_PI * std::pow((double)h, 6.0)));
   K.cspline_norm = static_cast<float>(32.0 / (M_PI * std::pow((double)h, 9.0)));
   K.r_eps = std::numeric_limits<float>::epsilon();
+  {
+    // every pair the within-h sums accept has T_hack <= d2 < T_h, so
+    // sqrt_cr(d2) <= h, a divisor r + r_eps in [r_eps, h + r_eps] and
+    // numerators |spiky_norm (h - r)^2 d_i| <= |spiky_norm| h^3 (up to
+    // rounding): where those stay inside the exact operations' fast ranges
+    // the kernels skip the tests (div3_fast_static)
+    const double hd = h;
+    const bool sq = K.T_hack >= 0x1p-96f && K.T_h <= 0x1p96f;
+    const bool dv = K.r_eps >= 0x1p-40f && hd + K.r_eps <= 0x1p40;
+    const bool nm = std::fabs((double)K.spiky_norm) * hd * hd * hd * 1.001 <= 0x1p40;
+    K.static_ranges = (sq && dv && nm) ? 1 : 0;
+  }
   K.mass = P.rest_mass;
   {
     const float r = 0.0f;  // W(0), particles.h:294

@@ -88,12 +88,13 @@ __device__ __forceinline__ bool div3_fast(f3 a, float b) {  // one condition, ev
                       (unsigned)div_in_range(a.y) & (unsigned)div_in_range(a.z);
   return ok != 0u;
 }
+__device__ __forceinline__ f3 div3_markstein(f3 a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float r1 = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+  return {div_markstein(a.x, b, r1), div_markstein(a.y, b, r1), div_markstein(a.z, b, r1)};
+}
 __device__ __forceinline__ f3 div3_shared(f3 a, float b) {
-  if (__builtin_expect(div3_fast(a, b), 1)) {
-    const float r = __builtin_amdgcn_rcpf(b);
-    const float r1 = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
-    return {div_markstein(a.x, b, r1), div_markstein(a.y, b, r1), div_markstein(a.z, b, r1)};
-  }
+  if (__builtin_expect(div3_fast(a, b), 1)) return div3_markstein(a, b);
   return div3(a, b);
 }
 
@@ -104,15 +105,36 @@ __device__ __forceinline__ f3 div3_shared(f3 a, float b) {
 // without the wrapper; other x take the plain sqrtf, so every result equals
 // sqrtf(x).
 __device__ __forceinline__ bool sqrt_fast(float x) { return (x >= 0x1p-96f) & (x <= 0x1p96f); }
+__device__ __forceinline__ float sqrt_corrected(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __int_as_float(__float_as_int(s) - 1);
+  const float sp = __int_as_float(__float_as_int(s) + 1);
+  const float r = __builtin_fmaf(-sm, s, x) <= 0.0f ? sm : s;
+  return __builtin_fmaf(-sp, s, x) > 0.0f ? sp : r;
+}
 __device__ __forceinline__ float sqrt_cr(float x) {
-  if (__builtin_expect(sqrt_fast(x), 1)) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float sm = __int_as_float(__float_as_int(s) - 1);
-    const float sp = __int_as_float(__float_as_int(s) + 1);
-    const float r = __builtin_fmaf(-sm, s, x) <= 0.0f ? sm : s;
-    return __builtin_fmaf(-sp, s, x) > 0.0f ? sp : r;
-  }
+  if (__builtin_expect(sqrt_fast(x), 1)) return sqrt_corrected(x);
   return sqrtf(x);
+}
+
+// The same two operations inside the pressure solve's within-h sums when the
+// host has shown (KParams::static_ranges) that every pair those sums accept,
+// T_hack <= d2 < T_h, keeps the square root's operand in [2^-96, 2^96], the
+// divisor r + r_eps in [2^-40, 2^40] and each numerator |s d_i| at or below
+// 2^40.  Of the range tests only "|a_i| >= 2^-40 or a_i == 0" is then left,
+// one unsigned minimum over (bits << 1) - 1 of the three numerators (zero
+// wraps to the top).  The same branches are taken as with the full tests, so
+// the results are the same bit for bit.
+__device__ __forceinline__ bool div3_fast_static(f3 a) {
+  constexpr uint32_t lim = (0x2B800000u << 1) - 1u;  // the bits of 2^-40, shifted likewise
+  const uint32_t tx = (__float_as_uint(a.x) << 1) - 1u;
+  const uint32_t ty = (__float_as_uint(a.y) << 1) - 1u;
+  const uint32_t tz = (__float_as_uint(a.z) << 1) - 1u;
+  return min(min(tx, ty), tz) >= lim;
+}
+__device__ __forceinline__ f3 div3_shared_static(f3 a, float b) {
+  if (__builtin_expect(div3_fast_static(a), 1)) return div3_markstein(a, b);
+  return div3(a, b);
 }
 
 // Del_Spikey<float,3>::operator() for r <= h, with r = |d| (kernels.h:171)
@@ -127,16 +149,21 @@ __device__ __forceinline__ f3 spiky_in(f3 d, float r, const KParams& K) {
 // reference's.  (The XSPH and vorticity passes stay exact in both modes: with
 // the hardware operations they measured slower, 110 -> 132 us for the
 // vorticity pass at 1M, from a different register allocation.)
-template <bool FAST>
+// SR: the pair is one the within-h sums accept and KParams::static_ranges holds.
+template <bool FAST, bool SR = false>
 __device__ __forceinline__ float nb_sqrt(float d2) {
   if constexpr (FAST) return __builtin_amdgcn_sqrtf(d2);
+  else if constexpr (SR) return sqrt_corrected(d2);
   else return sqrt_cr(d2);
 }
-template <bool FAST>
+template <bool FAST, bool SR = false>
 __device__ __forceinline__ f3 nb_spiky(f3 d, float r, const KParams& K) {
   if constexpr (FAST) {
     const float s = K.spiky_norm * (K.h - r) * (K.h - r) * __builtin_amdgcn_rcpf(r + K.r_eps);
     return smul3(s, d);
+  } else if constexpr (SR) {
+    const float s = K.spiky_norm * (K.h - r) * (K.h - r);  // as spiky_in
+    return div3_shared_static(smul3(s, d), r + K.r_eps);
   } else {
     return spiky_in(d, r, K);
   }

@@ -106,7 +106,7 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return x 
 // and is in the sums).  With FEAT, also the learned warm start's features of
 // the particle (FeatCtl).
 // Returns the particle's density error max(rho/rho0 - 1, 0).
-template <bool FEAT, bool FAST>
+template <bool FEAT, bool FAST, bool SR>
 __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int it, const KParams& K,
                                                     const float4* __restrict__ xs, const float4* __restrict__ v,
                                                     float4* __restrict__ xt, float* __restrict__ rho,
@@ -131,12 +131,13 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   // order, and into the mask; the walk stops at a near-coincident entry
   uint64_t m0 = 0, m1 = 0;
   int hack_at = -1;
+  const uint32_t qmask = skin ? kSkinSlotMask : ~0u;  // the slot of a walked entry
   for (int k0 = 0; k0 < cnt && hack_at < 0; k0 += kBatch) {
     uint32_t q[kBatch];  // skin entries keep their place in the full list in the top bits
     load_batch(walk, K.stride, k0, cnt, p, q);
     float4 t[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xs[skin ? q[u] & kSkinSlotMask : q[u]];
+    for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u] & qmask];
     uint32_t bits = 0;
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -148,9 +149,9 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
       bits |= (uint32_t)acc << u;
       hack_at = hk ? (skin ? (int)(q[u] >> kSkinSlotBits) : k0 + u) : hack_at;
       if (acc) {
-        const float r = nb_sqrt<FAST>(d2);
+        const float r = nb_sqrt<FAST, SR>(d2);
         density += K.mass * poly6_in(r, K);
-        const f3 grad = smul3(K.neg_inv_rho0, nb_spiky<FAST>(d, r, K));
+        const f3 grad = smul3(K.neg_inv_rho0, nb_spiky<FAST, SR>(d, r, K));
         dec3(sg, grad);
         sum_C += magsq3(grad);
         if (FEAT) {  // (feature passes walk the full lists)
@@ -277,7 +278,7 @@ __device__ __forceinline__ void warm_network(const FeatCtl& F, uint32_t wave_fir
 // calling thread is the one that clears the pass's flags (first): the same
 // bodies run as their own kernels (vb from xcd_block()) and inside the
 // persistent Jacobi kernel (vb from the calling block's share).
-template <int FM, bool FAST>
+template <int FM, bool FAST, bool SR>
 __device__ __forceinline__ void density_pass(uint32_t vb, bool first, uint32_t n, int it, int eslot,
                                              const SolveCtl& S, const FeatCtl& F,
                                                              const KParams& K, const float4* __restrict__ xs,
@@ -299,7 +300,7 @@ __device__ __forceinline__ void density_pass(uint32_t vb, bool first, uint32_t n
   float e = 0.0f;
   uint32_t hacks = 0;
   float fo[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (p < n) e = density_lambda_one<FM != 0, FAST>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, F, hacks, fo, snb, snb_count, skin);
+  if (p < n) e = density_lambda_one<FM != 0, FAST, SR>(p, n, it, K, xs, v, xt, rho, lam, nbr, nbr_count, wmask, F, hacks, fo, snb, snb_count, skin);
   if (FM != 0 && F.feat && p < n) {
     float4* o = reinterpret_cast<float4*>(F.feat + (uint64_t)p * 8);
     o[0] = make_float4(fo[0], fo[1], fo[2], fo[3]);
@@ -337,8 +338,8 @@ __device__ __forceinline__ void density_pass(uint32_t vb, bool first, uint32_t n
   }
 }
 
-template <int FM, bool FAST>
-__global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S, FeatCtl F,
+template <int FM, bool FAST, bool SR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((FM == 0 && (SR || FAST)) ? 8 : 1))) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S, FeatCtl F,
                                                              KParams K, const float4* __restrict__ xs,
                                                              const float4* __restrict__ v, float4* __restrict__ xt,
                                                              float* __restrict__ rho, float* __restrict__ lam,
@@ -350,7 +351,7 @@ __global__ void __launch_bounds__(kBlock) k_density_lambda_c(uint32_t n, int it,
                                                              unsigned long long* __restrict__ counters,
                                                              const int32_t* __restrict__ snb,
                                                              const int32_t* __restrict__ snb_count) {
-  density_pass<FM, FAST>(xcd_block(), blockIdx.x == 0 && threadIdx.x == 0, n, it, eslot, S, F, K, xs, v, xt, rho,
+  density_pass<FM, FAST, SR>(xcd_block(), blockIdx.x == 0 && threadIdx.x == 0, n, it, eslot, S, F, K, xs, v, xt, rho,
                          lam, nbr, nbr_count, wmask, moved, stamp, mflag, counters, snb, snb_count);
 }
 
@@ -513,7 +514,36 @@ __device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) 
 // density pass measured: a rejected warm start (slot 0 error above the error
 // before the warm start) restores xsave; an error <= tol ends the solve with
 // x* = xt.  Neither applies an update.
-template <bool FAST>
+// The pressure update's sum over one particle's list (particles.h:355-366):
+// the entries under the within-h mask, or with plain every entry within h.
+template <bool FAST, bool SR>
+__device__ __forceinline__ f3 dp_walk(const int32_t* __restrict__ walk, int cnt, bool skin, bool plain, uint64_t m0,
+                                      uint64_t m1, uint32_t p, f3 xp, float lp, const KParams& K,
+                                      const float4* __restrict__ xt) {
+  f3 dp = {0.f, 0.f, 0.f};
+  const uint32_t qmask = skin ? kSkinSlotMask : ~0u;  // the slot of a walked entry
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    uint32_t q[kBatch];
+    load_batch(walk, K.stride, k0, cnt, p, q);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) q[u] &= qmask;
+    const uint32_t bits = (uint32_t)((k0 < 64 ? m0 >> k0 : m1 >> (k0 - 64)) & 0xFFu);
+    float4 t[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) t[u] = xt[(bits >> u) & 1u ? q[u] : p];  // masked entries read p itself
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const f3 d = sub3(xp, xyz(t[u]));
+      const float d2 = magsq3(d);
+      if (k0 + u >= cnt || !((bits >> u) & 1u) || (plain && !within_h(d2, K))) continue;
+      const float r = nb_sqrt<FAST, SR>(d2);
+      acc3(dp, smul3(lp + t[u].w, nb_spiky<FAST, SR>(d, r, K)));
+    }
+  }
+  return dp;
+}
+
+template <bool FAST, bool SR>
 __device__ __forceinline__ void dp_pass(uint32_t vb, bool first, uint32_t n, int it, int last, uint32_t stamp,
                                         const SolveCtl& S,
                                                         const KParams& K, const float4* __restrict__ xt,
@@ -588,28 +618,14 @@ __device__ __forceinline__ void dp_pass(uint32_t vb, bool first, uint32_t n, int
     const float4 xp4 = xt[p];
     const f3 xp = xyz(xp4);
     const float lp = xp4.w;
-    f3 dp = {0.f, 0.f, 0.f};
     const int cnt = skin ? snb_count[p] : nbr_count[p];
     const uint64_t m0 = plain ? ~0ull : wmask[p];
     const uint64_t m1 = (plain || K.cap <= 64) ? ~0ull : wmask[n + p];
-    for (int k0 = 0; k0 < cnt; k0 += kBatch) {
-      uint32_t q[kBatch];
-      load_batch(walk, K.stride, k0, cnt, p, q);
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) q[u] = skin ? q[u] & kSkinSlotMask : q[u];
-      const uint32_t bits = (uint32_t)((k0 < 64 ? m0 >> k0 : m1 >> (k0 - 64)) & 0xFFu);
-      float4 t[kBatch];
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) t[u] = xt[(bits >> u) & 1u ? q[u] : p];  // masked entries read p itself
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const f3 d = sub3(xp, xyz(t[u]));
-        const float d2 = magsq3(d);
-        if (k0 + u >= cnt || !((bits >> u) & 1u) || (plain && !within_h(d2, K))) continue;
-        const float r = nb_sqrt<FAST>(d2);
-        acc3(dp, smul3(lp + t[u].w, nb_spiky<FAST>(d, r, K)));
-      }
-    }
+    // with no dirty lane in the wavefront every pair the walk takes is one
+    // the density pass accepted at the same positions (static ranges)
+    const f3 dp = (SR && __ballot(plain) == 0ull)
+                      ? dp_walk<FAST, SR>(walk, cnt, skin, false, m0, m1, p, xp, lp, K, xt)
+                      : dp_walk<FAST, false>(walk, cnt, skin, plain, m0, m1, p, xp, lp, K, xt);
     f3 nw = add3(xp, smul3(K.inv_rho0, dp));
     boundary(nw, K, p);
     st3(xs, p, nw);
@@ -624,7 +640,7 @@ __device__ __forceinline__ void dp_pass(uint32_t vb, bool first, uint32_t n, int
   if (track) record_move(mv, counters, C_DISP + ((it + 1) & 1), K);
 }
 
-template <bool FAST>
+template <bool FAST, bool SR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_dp_update_c(uint32_t n, int it, int last, uint32_t stamp, SolveCtl S,
                                                         KParams K, const float4* __restrict__ xt,
                                                         const float4* __restrict__ x, float4* __restrict__ xs,
@@ -637,7 +653,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))
                                                         const int32_t* __restrict__ snb_count,
                                                         const float4* __restrict__ xs0,
                                                         const float* __restrict__ rho) {
-  dp_pass<FAST>(xcd_block(), blockIdx.x == 0 && threadIdx.x == 0, n, it, last, stamp, S, K, xt, x, xs, v, nbr,
+  dp_pass<FAST, SR>(xcd_block(), blockIdx.x == 0 && threadIdx.x == 0, n, it, last, stamp, S, K, xt, x, xs, v, nbr,
                 nbr_count, wmask, dirty, counters, snb, snb_count, xs0, rho);
 }
 
@@ -757,12 +773,13 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
   const bool skin = skin_usable(counters, C_DISP_ALL);
   const int32_t* __restrict__ walk = skin ? snb : nbr;
   const int cnt = skin ? snb_count[p] : nbr_count[p];
+  const uint32_t qmask = skin ? kSkinSlotMask : ~0u;  // the slot of a walked entry
   int nc = 0;
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     uint32_t q[kBatch];
     load_batch(walk, K.stride, k0, cnt, p, q);
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) q[u] = skin ? q[u] & kSkinSlotMask : q[u];
+    for (int u = 0; u < kBatch; ++u) q[u] &= qmask;
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
@@ -894,17 +911,32 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     else                                                  \
       launch(c, name, (KT<false>), __VA_ARGS__);          \
   } while (0)
+// the same with the static-range instance of the exact arithmetic (KParams::static_ranges)
+#define PBF_LAUNCH_ARITH_SR(launch, c, name, KT, ...)     \
+  do {                                                    \
+    if ((c)->K.fast)                                      \
+      launch(c, name, (KT<true, false>), __VA_ARGS__);    \
+    else if ((c)->K.static_ranges)                        \
+      launch(c, name, (KT<false, true>), __VA_ARGS__);    \
+    else                                                  \
+      launch(c, name, (KT<false, false>), __VA_ARGS__);   \
+  } while (0)
 
 template <int FM>
 static void launch_density_fm(pbf_context* c, const char* name, int it, int eslot, const SolveCtl& S,
                               const FeatCtl& F) {
   if (c->K.fast)
-    PBF_LAUNCH(c, name, (k_density_lambda_c<FM, true>), it, eslot, S, F, c->K, (const float4*)c->xs,
+    PBF_LAUNCH(c, name, (k_density_lambda_c<FM, true, false>), it, eslot, S, F, c->K, (const float4*)c->xs,
+               (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
+               c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters, (const int32_t*)c->snb,
+               (const int32_t*)c->snb_count);
+  else if (c->K.static_ranges)
+    PBF_LAUNCH(c, name, (k_density_lambda_c<FM, false, true>), it, eslot, S, F, c->K, (const float4*)c->xs,
                (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
                c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters, (const int32_t*)c->snb,
                (const int32_t*)c->snb_count);
   else
-    PBF_LAUNCH(c, name, (k_density_lambda_c<FM, false>), it, eslot, S, F, c->K, (const float4*)c->xs,
+    PBF_LAUNCH(c, name, (k_density_lambda_c<FM, false, false>), it, eslot, S, F, c->K, (const float4*)c->xs,
                (const float4*)c->v, c->xt, c->rho, c->lam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count,
                c->wmask, c->moved, c->dirty_stamp, c->mflag, c->counters, (const int32_t*)c->snb,
                (const int32_t*)c->snb_count);
@@ -936,7 +968,7 @@ void launch_lambda_repair(pbf_context* c, int it, const SolveCtl& S) {
 }
 
 void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S) {
-  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
+  PBF_LAUNCH_ARITH_SR(PBF_LAUNCH, c, "dp_update", k_dp_update_c, it, last, c->dirty_stamp, S, c->K, (const float4*)c->xt,
              (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
              (const int32_t*)c->nbr_count, (const uint64_t*)c->wmask, (const uint32_t*)c->dirty, c->counters,
              (const int32_t*)c->snb, (const int32_t*)c->snb_count, (const float4*)c->xs0, (const float*)c->rho);
@@ -974,7 +1006,7 @@ struct JacobiArgs {
   unsigned long long* counters;
 };
 
-template <bool FAST>
+template <bool FAST, bool SR>
 __global__ void __launch_bounds__(kBlock) k_jacobi_persistent(JacobiArgs A) {
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const uint32_t nvb = (A.n + blockDim.x - 1) / blockDim.x;
@@ -987,7 +1019,7 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_persistent(JacobiArgs A) {
     const uint32_t stamp = A.stamp0 + it;
     const int eslot = A.solve ? (int)it + 1 : 0;
     for (uint32_t vb = vb0; vb < vb1; ++vb)
-      density_pass<0, FAST>(vb, vb == 0 && threadIdx.x == 0, A.n, (int)it, eslot, A.S, F0, A.K, A.xs, A.v, A.xt,
+      density_pass<0, FAST, SR>(vb, vb == 0 && threadIdx.x == 0, A.n, (int)it, eslot, A.S, F0, A.K, A.xs, A.v, A.xt,
                             A.rho, A.lam, A.nbr, A.nbr_count, A.wmask, A.moved, stamp, A.mflag, A.counters, A.snb,
                             A.snb_count);
     grid.sync();
@@ -1000,7 +1032,7 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_persistent(JacobiArgs A) {
     grid.sync();
     const int last = A.last_vel && it + 1 == A.iters;
     for (uint32_t vb = vb0; vb < vb1; ++vb)
-      dp_pass<FAST>(vb, vb == 0 && threadIdx.x == 0, A.n, (int)it, last, stamp, A.S, A.K, A.xt, A.x, A.xs, A.v,
+      dp_pass<FAST, SR>(vb, vb == 0 && threadIdx.x == 0, A.n, (int)it, last, stamp, A.S, A.K, A.xt, A.x, A.xs, A.v,
                     A.nbr, A.nbr_count, A.wmask, A.dirty, A.counters, A.snb, A.snb_count, A.xs0, A.rho);
     grid.sync();
   }
@@ -1008,8 +1040,15 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_persistent(JacobiArgs A) {
 
 // Blocks of one cooperative launch of the persistent kernel on the context's
 // device (0: cooperative launches unavailable), found once per context.
+// the persistent kernel's instance for the context's arithmetic
+static const void* persistent_kernel(const pbf_context* c) {
+  if (c->K.fast) return (const void*)k_jacobi_persistent<true, false>;
+  if (c->K.static_ranges) return (const void*)k_jacobi_persistent<false, true>;
+  return (const void*)k_jacobi_persistent<false, false>;
+}
+
 static uint32_t persistent_blocks(pbf_context* c) {
-  int& blocks = c->coop_blocks[c->K.fast ? 1 : 0];
+  int& blocks = c->coop_blocks[c->K.fast ? 1 : (c->K.static_ranges ? 2 : 0)];
   if (blocks >= 0) return (uint32_t)blocks;
   blocks = 0;
   int coop = 0, cus = 0, per_cu = 0;
@@ -1017,7 +1056,7 @@ static uint32_t persistent_blocks(pbf_context* c) {
     return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) return 0;
   c->coop_cus = cus;
-  const void* kern = c->K.fast ? (const void*)k_jacobi_persistent<true> : (const void*)k_jacobi_persistent<false>;
+  const void* kern = persistent_kernel(c);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess) return 0;
   blocks = std::max(0, per_cu * cus);
   return (uint32_t)blocks;
@@ -1049,7 +1088,7 @@ int launch_jacobi_persistent(pbf_context* c, uint32_t iters, bool solve, bool la
                c->counters};
   void* args[] = {&A};
   KernelTimer* t = timer_begin(c, "jacobi_persistent");
-  const void* kern = c->K.fast ? (const void*)k_jacobi_persistent<true> : (const void*)k_jacobi_persistent<false>;
+  const void* kern = persistent_kernel(c);
   c->coop_grid = blocks;
   const hipError_t e = hipLaunchCooperativeKernel(kern, dim3(blocks), dim3(kBlock), args, 0, c->stream);
   timer_end(c, t);

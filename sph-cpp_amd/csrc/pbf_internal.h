@@ -78,6 +78,7 @@ struct KParams {
   int skin_mode;                 // skin lists allowed (no PBF_OPT_FULL_LISTS)
   int tile_lists;                // the cell-tile list walk (no PBF_OPT_LANE_LISTS)
   int fast;                      // PBF_OPT_FAST_MATH: hardware sqrt and reciprocal in the fused step
+  int static_ranges;             // the exact sums' range tests hold for every accepted pair (nb_sqrt, nb_spiky)
   uint32_t heavy_budget;         // candidates one lane scans before a wavefront takes over (see walk_list)
   // batched scenes: ncells is the total over the scenes, sid the scene of each slot
   int batched;
@@ -227,8 +228,8 @@ struct pbf_context {
   pbf_solve_record* slog = nullptr;
   unsigned long long* slog_count = nullptr;
   pbf_tuning tune{256u, 768u, 8192u};
-  int coop_blocks[2] = {-1, -1};  // blocks of a cooperative launch of the persistent Jacobi kernel, exact and
-                                  // fast arithmetic (-1: not yet found)
+  int coop_blocks[3] = {-1, -1, -1};  // blocks of a cooperative launch of the persistent Jacobi kernel: exact,
+                                      // fast and static-range exact arithmetic (-1: not yet found)
   int coop_cus = 0;               // compute units of the device (found with coop_blocks)
   uint32_t coop_grid = 0;         // blocks of the last step's cooperative launch (0: none)
   unsigned long long* done_host = nullptr;  // pinned copy of C_DONE, polled between slot chunks

@@ -8,8 +8,11 @@
 // patterns (every exponent, zeros, denormals, infinities, NaNs); operands
 // shaped like the passes' own (divisors r + FLT_EPSILON with r < 0.13,
 // numerators s d with |s| <= 1.5e5 and |d| <= 0.13, squared distances below
-// 0.0169); and log-uniform magnitudes over 2^-48 .. 2^48 (both sides of the
-// divide's range test).
+// 0.0169, with signed zero numerators among them); and log-uniform
+// magnitudes over 2^-48 .. 2^48 (both sides of the divide's range test).
+// The instances with the range tests discharged on the host
+// (div3_shared_static, sqrt_corrected) are compared too, on the operands
+// that meet their preconditions.
 #include "pbf_device.h"
 
 namespace pbf {
@@ -25,6 +28,12 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 __device__ __forceinline__ float unit(uint64_t h) { return (float)(h >> 40) * 0x1p-24f; }  // [0, 1)
 
 __device__ __forceinline__ bool same_bits(float x, float y) { return __float_as_uint(x) == __float_as_uint(y); }
+__device__ __forceinline__ bool same3(f3 a, f3 b) { return same_bits(a.x, b.x) && same_bits(a.y, b.y) && same_bits(a.z, b.z); }
+// what the host guarantees before it selects div3_shared_static
+__device__ __forceinline__ bool static_div_ok(f3 a, float b) {
+  return (b >= 0x1p-40f) & (b <= 0x1p40f) & (fabsf(a.x) <= 0x1p40f) & (fabsf(a.y) <= 0x1p40f) &
+         (fabsf(a.z) <= 0x1p40f);
+}
 
 __global__ void k_check_arithmetic(uint64_t samples, uint64_t seed, unsigned long long* __restrict__ out) {
   unsigned long long bad_div = 0, bad_sqrt = 0, fast_div = 0, fast_sqrt = 0, bad_shaped = 0;
@@ -52,17 +61,18 @@ __global__ void k_check_arithmetic(uint64_t samples, uint64_t seed, unsigned lon
       const float dx = 0.26f * unit(h2) - 0.13f, dy = 0.26f * unit(h2 << 24) - 0.13f;
       const float dz = (h3 & 7) == 0 ? 0.0f : 0.26f * unit(h3) - 0.13f;
       a = {s * dx, s * dy, s * dz};
+      if ((h3 & 0x70) == 0x10) a.x = (h3 & 0x80) ? -0.0f : 0.0f;  // signed zero numerators
       b = r + 1.1920929e-7f;
       x = (dx * dx + dy * dy) + dz * dz;
     }
     const f3 q = div3_shared(a, b);
     const f3 ref = div3(a, b);
     fast_div += div3_fast(a, b) ? 1 : 0;
-    const bool bad = !(same_bits(q.x, ref.x) && same_bits(q.y, ref.y) && same_bits(q.z, ref.z));
+    const bool bad = !same3(q, ref) || (static_div_ok(a, b) && !same3(div3_shared_static(a, b), ref));
     bad_div += bad ? 1 : 0;
     bad_shaped += (bad && kind == 0) ? 1 : 0;
     fast_sqrt += sqrt_fast(x) ? 1 : 0;
-    bad_sqrt += same_bits(sqrt_cr(x), sqrtf(x)) ? 0 : 1;
+    bad_sqrt += (same_bits(sqrt_cr(x), sqrtf(x)) && (!sqrt_fast(x) || same_bits(sqrt_corrected(x), sqrtf(x)))) ? 0 : 1;
   }
   // one update per wavefront
 #pragma unroll

@@ -77,11 +77,14 @@ __device__ __forceinline__ bool div_in_range(float a) {
   const float m = fabsf(a);
   return (a == 0.0f) | ((m >= 0x1p-40f) & (m <= 0x1p40f));
 }
+// Each correction adds (b q - a)(-r1), i.e. the residual (a - b q) r1 (round
+// to nearest is symmetric, so b q - a rounds to minus a - b q); written this
+// way a zero numerator keeps its sign through both steps (b > 0 here: the
+// quotient of +-0 is +-0), where a - b q would turn -0 into +0.
 __device__ __forceinline__ float div_markstein(float a, float b, float r1) {
   const float q = a * r1;
-  const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q, a), r1, q);
-  const float q2 = __builtin_fmaf(__builtin_fmaf(-b, q1, a), r1, q1);
-  return a == 0.0f ? a : q2;  // b > 0 here: the quotient of +-0 is +-0
+  const float q1 = __builtin_fmaf(__builtin_fmaf(b, q, -a), -r1, q);
+  return __builtin_fmaf(__builtin_fmaf(b, q1, -a), -r1, q1);
 }
 __device__ __forceinline__ bool div3_fast(f3 a, float b) {  // one condition, evaluated without branches
   const unsigned ok = (unsigned)(b >= 0x1p-40f) & (unsigned)(b <= 0x1p40f) & (unsigned)div_in_range(a.x) &

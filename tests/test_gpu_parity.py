@@ -647,8 +647,10 @@ def test_divide_and_sqrt_sequences_bit_exact(pbf):
     (pbf_device.h) give the bits of the correctly rounded operations: 2 x 48M
     operand sets, a third raw 32-bit patterns (every exponent, zeros,
     denormals, infinities, NaNs), a third shaped like the passes' own
-    operands, a third log-uniform over 2^-48..2^48 (both sides of the
-    divide's range test)."""
+    operands (signed zero numerators among them), a third log-uniform over
+    2^-48..2^48 (both sides of the divide's range test).  The instances with
+    the range tests discharged on the host are compared on the operands that
+    meet their preconditions."""
     for seed in (12345, 777):
         r = pbf.check_arithmetic(48 << 20, seed=seed)
         print(r, flush=True)

@@ -149,7 +149,9 @@ const char* pbf_last_error(void);
 int pbf_params_derive(pbf_params* p);
 
 /* Particles(parameters) (particles.h:47-57) + Neighbors(parameters)
- * (neighbors.h:67-76): allocates capacity max_particles_local on the device. */
+ * (neighbors.h:67-76): allocates capacity max_particles_local on the device
+ * (below 2^28: the neighbour passes address particle arrays with 32-bit byte
+ * offsets; PBF_ERR_ARG otherwise). */
 int pbf_create(const pbf_params* params, pbf_context** out);
 int pbf_destroy(pbf_context* ctx);
 

@@ -373,8 +373,10 @@ int pbf_create(const pbf_params* params, pbf_context** out) {
   PBF_TRACE("pbf_create");
   if (!params || !out) return set_error(PBF_ERR_ARG, "null argument");
   *out = nullptr;
-  if (params->max_particles_local == 0 || params->max_particles_local >= (1ull << 31))
-    return set_error(PBF_ERR_ARG, "max_particles_local must be in [1, 2^31)");
+  // slots below 2^28: the neighbour passes address particle arrays with
+  // 32-bit byte offsets (gather4)
+  if (params->max_particles_local == 0 || params->max_particles_local >= (1ull << 28))
+    return set_error(PBF_ERR_ARG, "max_particles_local must be in [1, 2^28)");
   pbf_context* c = new pbf_context();
   c->P = *params;
   int rc = compute_kparams(c->P, c->K, c->dims);

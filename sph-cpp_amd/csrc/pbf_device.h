@@ -23,6 +23,13 @@ __device__ __forceinline__ f3 ld3(const float4* a, uint32_t i) {
   return {t.x, t.y, t.z};
 }
 __device__ __forceinline__ void st3(float4* a, uint32_t i, f3 v) { a[i] = make_float4(v.x, v.y, v.z, 0.f); }
+// Element i of a particle array, addressed as the array plus a 32-bit byte
+// offset (slots stay below 2^28, pbf_create): a gather then takes the base
+// from scalar registers and one vector register for its offset, instead of
+// a 64-bit vector address.
+__device__ __forceinline__ float4 gather4(const float4* __restrict__ a, uint32_t i) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a) + (i << 4));
+}
 
 // vec.h operator-, operator+, vec*scalar, scalar*vec, vec/scalar, +=, -=
 __device__ __forceinline__ f3 sub3(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }

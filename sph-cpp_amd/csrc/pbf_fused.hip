@@ -137,7 +137,7 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
     load_batch(walk, K.stride, k0, cnt, p, q);
     float4 t[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u] & qmask];
+    for (int u = 0; u < kBatch; ++u) t[u] = gather4(xs, q[u] & qmask);
     uint32_t bits = 0;
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -530,7 +530,7 @@ __device__ __forceinline__ f3 dp_walk(const int32_t* __restrict__ walk, int cnt,
     const uint32_t bits = (uint32_t)((k0 < 64 ? m0 >> k0 : m1 >> (k0 - 64)) & 0xFFu);
     float4 t[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xt[(bits >> u) & 1u ? q[u] : p];  // masked entries read p itself
+    for (int u = 0; u < kBatch; ++u) t[u] = gather4(xt, (bits >> u) & 1u ? q[u] : p);  // masked entries read p itself
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const f3 d = sub3(xp, xyz(t[u]));
@@ -733,8 +733,8 @@ __device__ __forceinline__ f3 xsph_sum(const int32_t* __restrict__ list, uint32_
     float4 t[kBatch], vq[kBatch];  // vq.w: the neighbour's density
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      t[u] = xs[q[u]];
-      vq[u] = vin[q[u]];
+      t[u] = gather4(xs, q[u]);
+      vq[u] = gather4(vin, q[u]);
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -782,7 +782,7 @@ __global__ void __launch_bounds__(kStageBlock) k_viscosity_first(uint32_t n, KPa
     for (int u = 0; u < kBatch; ++u) q[u] &= qmask;
     float4 t[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) t[u] = xs[q[u]];
+    for (int u = 0; u < kBatch; ++u) t[u] = gather4(xs, q[u]);
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const float d2 = magsq3(sub3(xp, xyz(t[u])));
@@ -832,8 +832,8 @@ __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, c
     float4 t[kBatch], vq[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      t[u] = xs[q[u]];
-      vq[u] = vin[q[u]];
+      t[u] = gather4(xs, q[u]);
+      vq[u] = gather4(vin, q[u]);
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
@@ -863,8 +863,8 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
     float4 t[kBatch], wq[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      t[u] = xs[q[u]];
-      wq[u] = scr[q[u]];
+      t[u] = gather4(xs, q[u]);
+      wq[u] = gather4(scr, q[u]);
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {

@@ -76,6 +76,9 @@ def test_create_rejects_bad_arguments_before_touching_the_device(pbf):
     p = pbf.Params()
     p.max_particles_local = 0
     assert L.pbf_create(C.byref(p), C.byref(ctx)) == pbf.PBF_ERR_ARG
+    p.max_particles_local = 1 << 28  # slots are addressed with 32-bit byte offsets
+    assert L.pbf_create(C.byref(p), C.byref(ctx)) == pbf.PBF_ERR_ARG
+    assert b"2^28" in pbf.lib().pbf_last_error()
     p.max_particles_local = 10
     p.smoothing_radius = 0.1
     p.neighbor_bin_spacing = 0.12

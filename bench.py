@@ -376,10 +376,10 @@ def summarize(d: str, workload: str, out_dir: str = "") -> None:
                  f"x {VALU_CLOCK_GHZ} GHz).  A back-to-back stream of wave64 instructions that each take 4 cycles on a",
                  "16-lane SIMD issues 0.25.  `lanes busy` = SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU), the",
                  "share of lanes active in the VALU instructions issued.  The density and pressure-update",
-                 "passes issue at that ceiling: they are bound by VALU issue (~70 instructions per accepted",
+                 "passes issue at that ceiling: they are bound by VALU issue (~60 instructions per accepted",
                  "neighbour, half of them in the correctly rounded divide and square root).  With",
-                 "PBF_OPT_FAST_MATH (hardware square root and reciprocal) the density pass issues 43% fewer",
-                 "VALU instructions and runs 12% faster at rest, where it then waits on its neighbour gathers.",
+                 "PBF_OPT_FAST_MATH (hardware square root and reciprocal) the density pass issues about a third",
+                 "fewer VALU instructions and runs 7% faster at rest, where it then waits on its neighbour gathers.",
                  "The XSPH and vorticity passes (0.10-0.18) wait on their gathers.", "",
                  "| kernel | launches | waves | VALU instr / wave | vector loads / wave | LDS instr / wave | "
                  "mean us | VALU issue | lanes busy |",

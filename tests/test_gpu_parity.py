@@ -166,6 +166,18 @@ def test_one_step_bit_exact(pbf, scenes):
     _compare_state(g, o, "1 step")
 
 
+def test_wide_layers_steps_bit_exact(pbf, scenes):
+    """A flat block of 192 x 192 x 6 particles: one layer of cells holds about
+    110,000 particles, so list entries lie more than 2^16 slots away from
+    their particles (every list of the other small scenes spans a few
+    thousand slots)."""
+    p, pos, vel = scenes.dam_break((192, 192, 6), (9.6, 9.6, 1.2), seed=17)
+    g, o = _pair(pbf, p, pos, vel)
+    g.step(2)
+    o.step(2)
+    _compare_state(g, o, "wide layers, 2 steps")
+
+
 def test_per_pass_api_equals_oracle_passes(pbf, scenes):
     """Each ABI pass against the oracle pass of the same name, in main.cpp order."""
     p, pos, vel = _small_scene(scenes, seed=7)

@@ -193,8 +193,8 @@ HIP_KERNEL = {
     "cell_box": "pbf::k_cell_box", "fill_neighbors": "pbf::k_fill_neighbors", "fill_heavy": "pbf::k_fill_heavy",
     "density_lambda": "pbf::k_density_lambda_c<0, false, true>", "mark_dirty": "pbf::k_mark_dirty",
     "lambda_repair": "pbf::k_lambda_repair<false>", "dp_update": "pbf::k_dp_update_c<false, true>",
-    "viscosity": "pbf::k_viscosity_first", "vorticity": "pbf::k_vorticity_c",
-    "apply_vorticity": "pbf::k_apply_vorticity_c", "viscosity_positions": "pbf::k_viscosity_last",
+    "viscosity": "pbf::k_viscosity_first", "vorticity": "pbf::k_vorticity_c<true>",
+    "apply_vorticity": "pbf::k_apply_vorticity_c<true>", "viscosity_positions": "pbf::k_viscosity_last",
 }
 
 

@@ -159,7 +159,8 @@ __device__ __forceinline__ f3 spiky_in(f3 d, float r, const KParams& K) {
 // reference's.  (The XSPH and vorticity passes stay exact in both modes: with
 // the hardware operations they measured slower, 110 -> 132 us for the
 // vorticity pass at 1M, from a different register allocation.)
-// SR: the pair is one the within-h sums accept and KParams::static_ranges holds.
+// SR: KParams::static_ranges holds and, for nb_sqrt, the pair is one the
+// density sums accept (d2 >= T_hack); nb_spiky's divide needs only r <= h.
 template <bool FAST, bool SR = false>
 __device__ __forceinline__ float nb_sqrt(float d2) {
   if constexpr (FAST) return __builtin_amdgcn_sqrtf(d2);

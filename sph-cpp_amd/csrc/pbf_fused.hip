@@ -816,6 +816,7 @@ __global__ void __launch_bounds__(kBlock) k_viscosity_last(uint32_t n, KParams K
 }
 
 // compute_vorticity (particles.h:452-466) -> scr (omega, and |omega| in w)
+template <bool SR>
 __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                         const float4* __restrict__ vin, float4* __restrict__ scr,
                                                         const int32_t* __restrict__ cnb,
@@ -840,13 +841,14 @@ __global__ void __launch_bounds__(kBlock) k_vorticity_c(uint32_t n, KParams K, c
       if (k0 + u >= cnt) continue;
       const f3 d = sub3(xp, xyz(t[u]));
       const float r = sqrtf(magsq3(d));
-      acc3(w, cross3(sub3(xyz(vq[u]), vp), spiky_in(d, r, K)));
+      acc3(w, cross3(sub3(xyz(vq[u]), vp), nb_spiky<false, SR>(d, r, K)));
     }
   }
   scr[p] = make_float4(w.x, w.y, w.z, sqrtf(magsq3(w)));  // |omega| for apply_vorticity's gathers
 }
 
 // apply_vorticity (particles.h:471-487), v in place (own particle only)
+template <bool SR>
 __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParams K, const float4* __restrict__ xs,
                                                               const float4* __restrict__ scr,
                                                               float4* __restrict__ v,
@@ -872,7 +874,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_vorticity_c(uint32_t n, KParam
       const f3 d = sub3(xp, xyz(t[u]));
       const float r = sqrtf(magsq3(d));
       const float wm = wq[u].w;  // |omega_q|, computed by the vorticity pass
-      acc3(eta, smul3(wm, spiky_in(d, r, K)));
+      acc3(eta, smul3(wm, nb_spiky<false, SR>(d, r, K)));
     }
   }
   const f3 N = div3(eta, sqrtf(magsq3(eta)) + K.r_eps);
@@ -1136,14 +1138,24 @@ void launch_viscosity_first(pbf_context* c) {
              (const int32_t*)c->snb_count, c->skin_hint);
 }
 
+// the vorticity passes stay exact in both arithmetic modes; the static-range
+// instance of the divide where KParams::static_ranges holds
+#define PBF_LAUNCH_SR(c, name, KT, ...)                                     \
+  do {                                                                      \
+    if ((c)->K.static_ranges)                                               \
+      PBF_LAUNCH(c, name, (KT<true>), __VA_ARGS__);                         \
+    else                                                                    \
+      PBF_LAUNCH(c, name, (KT<false>), __VA_ARGS__);                        \
+  } while (0)
+
 void launch_vorticity_masked(pbf_context* c) {
-  PBF_LAUNCH(c, "vorticity", k_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->va, c->scr,
-             (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
+  PBF_LAUNCH_SR(c, "vorticity", k_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->va, c->scr,
+                (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 
 void launch_apply_vorticity_masked(pbf_context* c) {
-  PBF_LAUNCH(c, "apply_vorticity", k_apply_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->scr, c->va,
-             (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
+  PBF_LAUNCH_SR(c, "apply_vorticity", k_apply_vorticity_c, c->K, (const float4*)c->xs, (const float4*)c->scr,
+                c->va, (const int32_t*)c->cnb, (const int32_t*)c->cnb_count);
 }
 
 void launch_viscosity_last(pbf_context* c) {

@@ -142,10 +142,13 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32
 // calculate_bins (neighbors.h:120-125): cell key of every slot, and the
 // inverse permutation (original index -> slot) so that the keys can be
 // listed in original-index order for the stable sort.
+// Also lists the (key, slot) pairs by original index: a stable sort of that
+// sequence by key is the reference's stable sort of (key, index) pairs.
 __global__ void __launch_bounds__(kBlock) k_bin(const float4* __restrict__ xs, uint32_t n, uint32_t bin_end,
                                                 KParams K, const uint32_t* __restrict__ oid,
                                                 uint32_t* __restrict__ key, uint32_t* __restrict__ inv,
-                                                unsigned long long* __restrict__ counters) {
+                                                unsigned long long* __restrict__ counters,
+                                                uint32_t* __restrict__ sk, uint32_t* __restrict__ sv) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p == 0) {  // counted by the list build that follows
     counters[C_HEAVY] = 0ull;
@@ -161,18 +164,8 @@ __global__ void __launch_bounds__(kBlock) k_bin(const float4* __restrict__ xs, u
   }
   key[p] = k;
   inv[o] = p;
-}
-
-// (key, slot) pairs listed by original index: a stable sort of this sequence
-// by key is the reference's stable sort of (key, index) pairs.
-__global__ void __launch_bounds__(kBlock) k_keys_by_oid(uint32_t n, const uint32_t* __restrict__ key,
-                                                        const uint32_t* __restrict__ inv,
-                                                        uint32_t* __restrict__ sk, uint32_t* __restrict__ sv) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t s = inv[i];
-  sk[i] = key[s];
-  sv[i] = s;
+  sk[o] = k;
+  sv[o] = p;
 }
 
 void launch_bin(pbf_context* c, uint64_t bin_end) {
@@ -180,9 +173,8 @@ void launch_bin(pbf_context* c, uint64_t bin_end) {
   if (!n) return;
   KernelTimer* t = timer_begin(c, "bin");
   hipLaunchKernelGGL(k_bin, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, (const float4*)c->xs, n,
-                     (uint32_t)bin_end, c->K, (const uint32_t*)c->oid, c->key, c->slot, c->counters);
-  hipLaunchKernelGGL(k_keys_by_oid, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const uint32_t*)c->key,
-                     (const uint32_t*)c->slot, c->sk[0], c->sv[0]);
+                     (uint32_t)bin_end, c->K, (const uint32_t*)c->oid, c->key, c->slot, c->counters, c->sk[0],
+                     c->sv[0]);
   timer_end(c, t);
 }
 

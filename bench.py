@@ -517,7 +517,8 @@ def host_buffer_run(pbf, params, pos, vel, steps, tuning=None):
     return out
 
 
-def _solve_run(pbf, scenes, mode, model, counts, seed, steps, tol, max_slots, cap, mean, label, strict=False):
+def _solve_run(pbf, scenes, mode, model, counts, seed, steps, tol, max_slots, cap, mean, label, strict=False,
+               retry=False):
     """One trajectory of `steps` iterate-to-tolerance substeps in one warm-start
     mode, replayed from HIP graphs (pbf_solve_graph: no host round trip inside
     a substep); per substep the slots used, whether the criterion was met and
@@ -527,6 +528,8 @@ def _solve_run(pbf, scenes, mode, model, counts, seed, steps, tol, max_slots, ca
         params.options |= pbf.PBF_OPT_SOLVE_MEAN_ERROR
     if strict:
         params.options |= pbf.PBF_OPT_WARM_STRICT
+    if retry:
+        params.options |= pbf.PBF_OPT_WARM_RETRY
     n = pos.shape[0]
     g = pbf.Particles(params)
     g.add(pos, pos, vel)
@@ -547,7 +550,8 @@ def _solve_run(pbf, scenes, mode, model, counts, seed, steps, tol, max_slots, ca
     crit = emean if mean else emax
     return {"n": n, "wall": wall, "slots": np.array([r["iterations"] for r in log]),
             "met": crit <= float(np.float32(tol)), "emax": emax, "emean": emean,
-            "kept": int(sum(r["warm_start_kept"] for r in log))}
+            "kept": int(sum(r["warm_start_kept"] for r in log)),
+            "damped": int(sum(r["warm_start_damped"] for r in log))}
 
 
 def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=256, cap=128, mean=False,
@@ -575,11 +579,14 @@ def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=256, cap
            "timing": "pbf_solve_graph (graph-replayed substeps, device-side early exit)",
            "warm_fallback": "strict (max or mean error grows)" if strict else "the criterion's error grows"}
     runs = {}
-    modes = [("plain", pbf.PBF_WARM_NONE), ("previous", pbf.PBF_WARM_PREVIOUS)]
-    modes += [(name, pbf.PBF_WARM_LEARNED) for name in loaded]
-    for name, mode in modes:
-        runs[name] = _solve_run(pbf, scenes, mode, loaded.get(name), counts, seed, steps, tol, max_slots, cap, mean,
-                                f"{which} error <= {tol}, cap {cap}, {name}", strict=strict)
+    # (name, mode, network, PBF_OPT_WARM_RETRY)
+    modes = [("plain", pbf.PBF_WARM_NONE, None, False), ("previous", pbf.PBF_WARM_PREVIOUS, None, False)]
+    modes += [(name, pbf.PBF_WARM_LEARNED, name, False) for name in loaded]
+    modes += [("previous_retry", pbf.PBF_WARM_PREVIOUS, None, True),
+              ("learned_retry", pbf.PBF_WARM_LEARNED, "learned", True)]
+    for name, mode, net, retry in modes:
+        runs[name] = _solve_run(pbf, scenes, mode, loaded.get(net), counts, seed, steps, tol, max_slots, cap, mean,
+                                f"{which} error <= {tol}, cap {cap}, {name}", strict=strict, retry=retry)
     for m in loaded.values():
         m.close()
     out["models"] = {k: os.path.relpath(v, ROOT) for k, v in models.items()}
@@ -589,6 +596,7 @@ def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=256, cap
         out[name] = {"value": r["n"] * steps / r["wall"], "ms_per_step": r["wall"] * 1e3 / steps,
                      "mean_slots": float(np.mean(r["slots"])), "max_slots_used": int(np.max(r["slots"])),
                      "steps_meeting_tolerance": int(np.sum(r["met"])), "warm_start_kept": r["kept"],
+                     "warm_start_damped": r["damped"],
                      "final_max_error_median": float(np.median(r["emax"])),
                      "final_mean_error_median": float(np.median(r["emean"])),
                      "mean_slots_where_all_modes_met": (float(np.mean(r["slots"][all_met]))
@@ -603,6 +611,10 @@ def warm_start_ab(pbf, scenes, counts, seed, steps, tol=0.01, max_slots=256, cap
     out["learned_vs_previous"] = out["learned"]["value"] / out["previous"]["value"]
     if "learned_plain50" in out:
         out["learned_plain50_vs_plain"] = out["learned_plain50"]["value"] / out["plain"]["value"]
+    # PBF_OPT_WARM_RETRY: a rejected start is retried at half its multipliers before the fall back to zero
+    out["previous_retry_vs_plain"] = out["previous_retry"]["value"] / out["plain"]["value"]
+    out["learned_retry_vs_plain"] = out["learned_retry"]["value"] / out["plain"]["value"]
+    out["learned_retry_vs_previous_retry"] = out["learned_retry"]["value"] / out["previous_retry"]["value"]
     if all_met.any():
         ls = out["learned"]["mean_slots_where_all_modes_met"]
         out["slot_ratio_plain_over_learned_where_all_met"] = out["plain"]["mean_slots_where_all_modes_met"] / ls

@@ -119,6 +119,12 @@ typedef struct pbf_params {
  * launches per pass (pbf_stats.persistent_grid tells which ran).  A device without
  * cooperative launches makes pbf_step and pbf_solve fail with PBF_ERR_UNSUPPORTED. */
 #define PBF_OPT_PERSISTENT 128u
+/* A warm-started pbf_solve whose warm start grows the error at slot 0 retries it once with the
+ * warm multipliers halved (from the same position stars) before it falls back to a zero warm
+ * start: slot 1 checks the damped start against the same error (pbf_solve_info's
+ * warm_start_kept is then 2, and the halved multipliers join the sums).  Without it the solve
+ * falls back to zero at once. */
+#define PBF_OPT_WARM_RETRY 256u
 
 typedef struct pbf_context pbf_context;
 
@@ -250,7 +256,8 @@ enum { PBF_WARM_NONE = 0, PBF_WARM_GIVEN = 1, PBF_WARM_PREVIOUS = 2, PBF_WARM_LE
  * on the order of the sum. */
 typedef struct pbf_solve_info {
   uint32_t iterations;              /* slots used (<= max_iterations) */
-  int32_t warm_start_kept;          /* 1: warm-start positions kept; 0: rejected or none */
+  int32_t warm_start_kept;          /* 1: warm-start positions kept; 2: its damped retry's kept
+                                       (PBF_OPT_WARM_RETRY); 0: rejected or none */
   float max_error_before_warm_start;
   float mean_error_before_warm_start;
   float max_error_history[PBF_MAX_SOLVE_ITERATIONS];   /* of every used slot */

@@ -625,10 +625,15 @@ struct Sim {
   // a fused density pass (density_features) first records e_pre and the
   // features, then one dp pass with lam = warm moves x*; if slot 0 then
   // measures e_0 > e_pre, x* returns to its value after that first pass and
-  // slot 0 makes no update (the fallback to a zero warm start).  lam_sum
-  // (optional) receives the sum of the multipliers of every update that was
-  // applied, the warm one included; it is also kept for the next solve's
-  // features.  Returns the number of slots used (<= max_iter).
+  // slot 0 makes no update (the fallback to a zero warm start).  With the
+  // retry (use_mean bit 2, the library's PBF_OPT_WARM_RETRY) slot 0 instead
+  // returns x* to that value and applies one dp pass with the warm
+  // multipliers halved; slot 1 then checks that against e_pre the same way
+  // and falls back to zero if it grew too.  lam_sum (optional) receives the
+  // sum of the multipliers of every update that was applied, the warm one
+  // (or its halved retry) included; it is also kept for the next solve's
+  // features.  Returns the number of slots used (<= max_iter); *kept is 1
+  // for the warm start, 2 for its retry, 0 for neither.
   int solve(R tol, int max_iter, int use_mean, const R* warm, float* hist_max, float* hist_mean, R* lam_sum,
             std::vector<float>* feat, int* kept) {
     apply_external_forces(0, n);
@@ -639,8 +644,10 @@ struct Sim {
     R e_pre = (R)0, o_pre = (R)0;
     int keep = 0;
     // use_mean bit 0: the tolerance applies to the mean; bit 1: strict fallback
-    // (the other measure must not grow over the warm update either)
-    const bool mean = (use_mean & 1) != 0, strict = (use_mean & 2) != 0;
+    // (the other measure must not grow over the warm update either); bit 2:
+    // retry a rejected warm start once at half its multipliers
+    const bool mean = (use_mean & 1) != 0, strict = (use_mean & 2) != 0, retry = (use_mean & 4) != 0;
+    const R half = (R)0.5;
     auto error = [&]() -> R { return mean ? (R)emean_fixed(0, n) : emax_r(0, n); };
     auto other = [&]() -> R { return mean ? emax_r(0, n) : (R)emean_fixed(0, n); };
     if (warm) {
@@ -663,9 +670,17 @@ struct Sim {
       if (hist_max) hist_max[it] = (float)emax_r(0, n);
       if (hist_mean) hist_mean[it] = emean_fixed(0, n);
       used = it + 1;
-      if (warm && it == 0 && (e > e_pre || (strict && other() > o_pre))) {
+      const bool check = warm && (it == 0 || (it == 1 && keep == 2));
+      if (check && (e > e_pre || (strict && other() > o_pre))) {
         xs = saved;
-        keep = 0;
+        if (retry && it == 0) {
+          for (size_t p = 0; p < n; ++p) lam[p] = half * warm[p];
+          compute_pressure_dps(0, n);
+          update_position_stars(0, n);
+          keep = 2;
+        } else {
+          keep = 0;
+        }
         continue;
       }
       if (e <= tol) {
@@ -678,7 +693,7 @@ struct Sim {
       for (size_t p = 0; p < n; ++p) acc[p] += lam[p];
     }
     if (warm && keep)
-      for (size_t p = 0; p < n; ++p) acc[p] += warm[p];
+      for (size_t p = 0; p < n; ++p) acc[p] += keep == 2 ? half * warm[p] : warm[p];
     if (lam_sum) std::copy(acc.begin(), acc.end(), lam_sum);
     lsum_prev = acc;
     if (kept) *kept = keep;

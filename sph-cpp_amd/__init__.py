@@ -104,6 +104,7 @@ PBF_OPT_WARM_STRICT = 16
 PBF_OPT_FULL_LISTS = 32
 PBF_OPT_LANE_LISTS = 64
 PBF_OPT_PERSISTENT = 128
+PBF_OPT_WARM_RETRY = 256
 PBF_WARM_FEATURES = 8
 PBF_WARM_HIDDEN = 64
 PBF_WARM_PARAMS = 4801
@@ -443,6 +444,7 @@ class Particles:
         got = C.c_uint32(0)
         _check(lib().pbf_get_solve_log(self._ctx, recs, count, C.byref(got)))
         return [{"index": r.index, "iterations": r.iterations, "warm_start_kept": bool(r.warm_start_kept),
+                 "warm_start_damped": r.warm_start_kept == 2,
                  "max_error": r.max_error, "mean_error": r.mean_error,
                  "max_error_before_warm_start": r.max_error_before_warm_start,
                  "mean_error_before_warm_start": r.mean_error_before_warm_start} for r in recs[:got.value]]
@@ -469,6 +471,7 @@ class Particles:
         _check(lib().pbf_get_solve_info(self._ctx, C.byref(si)))
         k = si.iterations
         return {"iterations": k, "warm_start_kept": bool(si.warm_start_kept),
+                "warm_start_damped": si.warm_start_kept == 2,
                 "max_error_before_warm_start": si.max_error_before_warm_start,
                 "mean_error_before_warm_start": si.mean_error_before_warm_start,
                 "max_error_history": np.array(si.max_error_history[:k], dtype=np.float32),

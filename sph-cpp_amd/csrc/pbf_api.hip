@@ -942,7 +942,8 @@ static int solve_body(pbf_context* c, float tolerance, uint32_t max_iterations, 
   else if (warm_start == PBF_WARM_GIVEN) launch_scatter_in_f1(c, c->wgiven, c->wlam);
   const int mean = (c->P.options & PBF_OPT_SOLVE_MEAN_ERROR) ? 1 : 0;
   const int strict = (c->P.options & PBF_OPT_WARM_STRICT) ? 1 : 0;
-  const SolveCtl S{tolerance, warm, mean, c->ehist, c->esum, c->lsum, c->xsave, strict};
+  const int retry = (warm && (c->P.options & PBF_OPT_WARM_RETRY)) ? 1 : 0;
+  const SolveCtl S{tolerance, warm, mean, c->ehist, c->esum, c->lsum, c->xsave, strict, retry};
   if (warm) {
     // first density pass (its error is the reference for the fallback; it
     // writes the features), then the warm update
@@ -965,13 +966,14 @@ static int solve_body(pbf_context* c, float tolerance, uint32_t max_iterations, 
   }
   constexpr uint32_t kChunk = 8;
   c->coop_grid = 0;
-  if (persistent_jacobi(c)) launch_jacobi_persistent(c, max_iterations, true, false, S);
+  if (persistent_jacobi(c) && !retry) launch_jacobi_persistent(c, max_iterations, true, false, S);
   else for (uint32_t it = 0; it < max_iterations; ++it) {
     ++c->dirty_stamp;
     launch_density_lambda_masked(c, (int)it, (int)it + 1, S);
     launch_mark_dirty(c, (int)it, S);
     launch_lambda_repair(c, (int)it, S);
     launch_dp_update_masked(c, (int)it, 0, S);
+    if (retry && it == 0) launch_warm_retry(c);  // (does nothing unless slot 0 rejected the warm start)
     if (host_chunks && (it + 1) % kChunk == 0 && it + 1 < max_iterations) {
       int rc = check_hip(hipMemcpyAsync(c->done_host, c->counters + C_DONE, sizeof(unsigned long long),
                                         hipMemcpyDeviceToHost, c->stream), "pbf_solve");

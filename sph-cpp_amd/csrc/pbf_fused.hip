@@ -508,12 +508,6 @@ __device__ __forceinline__ float mean_error(unsigned long long sum, uint32_t n) 
   return (float)((double)sum / 4294967296.0 / (double)n);
 }
 
-// compute_pressure_dps (particles.h:348-358) + update_position_stars
-// (:363-370); on the last iteration also update_velocities (:375-385).
-// In an iterate-to-tolerance solve, slot `it` first checks the error its
-// density pass measured: a rejected warm start (slot 0 error above the error
-// before the warm start) restores xsave; an error <= tol ends the solve with
-// x* = xt.  Neither applies an update.
 // The pressure update's sum over one particle's list (particles.h:355-366):
 // the entries under the within-h mask, or with plain every entry within h.
 template <bool FAST, bool SR>
@@ -543,6 +537,13 @@ __device__ __forceinline__ f3 dp_walk(const int32_t* __restrict__ walk, int cnt,
   return dp;
 }
 
+// compute_pressure_dps (particles.h:348-358) + update_position_stars
+// (:363-370); on the last iteration also update_velocities (:375-385).
+// In an iterate-to-tolerance solve, slot `it` first checks the error its
+// density pass measured: a rejected warm start (slot 0 error above the error
+// before the warm start; with the retry, also slot 1's after the damped
+// start) restores xsave; an error <= tol ends the solve with x* = xt.
+// Neither applies an update.
 template <bool FAST, bool SR>
 __device__ __forceinline__ void dp_pass(uint32_t vb, bool first, uint32_t n, int it, int last, uint32_t stamp,
                                         const SolveCtl& S,
@@ -578,7 +579,8 @@ __device__ __forceinline__ void dp_pass(uint32_t vb, bool first, uint32_t n, int
     const float e = S.mean ? mean_error(S.esum[it + 1], n) : __uint_as_float(S.ehist[it + 1]);
     const float e_pre = S.mean ? mean_error(S.esum[0], n) : __uint_as_float(S.ehist[0]);
     bool grew = false;
-    if (S.warm && it == 0) {
+    // slot 0 checks the warm start; with the retry, slot 1 checks the damped one
+    if (S.warm && (it == 0 || (it == 1 && S.retry && counters[C_RETRY] != 0ull))) {
       grew = e > e_pre;
       if (S.strict) {  // the other error measure must not grow either
         const float o = S.mean ? __uint_as_float(S.ehist[1]) : mean_error(S.esum[1], n);
@@ -588,7 +590,14 @@ __device__ __forceinline__ void dp_pass(uint32_t vb, bool first, uint32_t n, int
     }
     if (grew) {
       mode = 1;
-      if (first) counters[C_WARM_KEPT] = 0ull;
+      if (first) {
+        if (S.retry && it == 0) {  // x* goes back, and k_warm_retry applies the damped start
+          counters[C_RETRY] = 1ull;
+          counters[C_WARM_KEPT] = 2ull;
+        } else {
+          counters[C_WARM_KEPT] = 0ull;
+        }
+      }
     } else if (S.tol >= 0.0f && e <= S.tol) {
       mode = 2;
       if (first) {
@@ -712,9 +721,60 @@ __global__ void __launch_bounds__(kBlock) k_warm_dp(uint32_t n, KParams K, const
 // After the Jacobi loop: the kept warm-start multipliers join the sum.
 __global__ void k_add_warm(uint32_t n, const float* __restrict__ wlam, float* __restrict__ lsum,
                            const unsigned long long* __restrict__ counters) {
-  if (counters[C_WARM_KEPT] == 0ull) return;
+  const unsigned long long kept = counters[C_WARM_KEPT];
+  if (kept == 0ull) return;
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) lsum[p] += wlam[p];
+  if (p < n) lsum[p] += kept == 2ull ? kWarmRetryScale * wlam[p] : wlam[p];
+}
+
+// PBF_OPT_WARM_RETRY, after slot 0's pressure update: when that update put x*
+// back because the warm start grew the error (C_RETRY), x* becomes the
+// position stars before the warm start plus the update with the warm
+// multipliers times kWarmRetryScale (k_warm_dp with scaled multipliers, from
+// the saved positions); slot 1 then checks it against the same error.
+template <bool FAST>
+__global__ void __launch_bounds__(kBlock) k_warm_retry(uint32_t n, KParams K, const float4* __restrict__ xsave,
+                                                       const float* __restrict__ wlam,
+                                                       const int32_t* __restrict__ nbr,
+                                                       const int32_t* __restrict__ nbr_count,
+                                                       float4* __restrict__ xs,
+                                                       unsigned long long* __restrict__ counters,
+                                                       const float4* __restrict__ xs0) {
+  if (counters[C_RETRY] == 0ull) return;
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  const bool track = K.skin_mode && n < (1u << kSkinSlotBits);
+  float mv = 0.0f;
+  if (p < n) {
+    const f3 xp = ld3(xsave, p);
+    const float lp = kWarmRetryScale * wlam[p];
+    f3 dp = {0.f, 0.f, 0.f};
+    const int cnt = nbr_count[p];
+    for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+      uint32_t q[kBatch];
+      load_batch(nbr, K.stride, k0, cnt, p, q);
+      float4 t[kBatch];
+      float lq[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        t[u] = gather4(xsave, q[u]);
+        lq[u] = kWarmRetryScale * wlam[q[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const f3 d = sub3(xp, xyz(t[u]));
+        const float d2 = magsq3(d);
+        if (k0 + u >= cnt || !within_h(d2, K)) continue;
+        const float r = nb_sqrt<FAST>(d2);
+        acc3(dp, smul3(lp + lq[u], nb_spiky<FAST>(d, r, K)));
+      }
+    }
+    f3 nw = add3(xp, smul3(K.inv_rho0, dp));
+    boundary(nw, K, p);
+    st3(xs, p, nw);
+    if (track) mv = move2(nw, xs0[p]);
+  }
+  // slot 1 reads parity 1
+  if (track) record_move(mv, counters, C_DISP + 1, K);
 }
 
 // XSPH sum (particles.h:441-444) over a compact row, from global memory
@@ -974,6 +1034,12 @@ void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S
              (const float4*)c->x, c->xs, c->v, (const int32_t*)c->nbr,
              (const int32_t*)c->nbr_count, (const uint64_t*)c->wmask, (const uint32_t*)c->dirty, c->counters,
              (const int32_t*)c->snb, (const int32_t*)c->snb_count, (const float4*)c->xs0, (const float*)c->rho);
+}
+
+void launch_warm_retry(pbf_context* c) {
+  PBF_LAUNCH_ARITH(PBF_LAUNCH, c, "warm_retry", k_warm_retry, c->K, (const float4*)c->xsave,
+                   (const float*)c->wlam, (const int32_t*)c->nbr, (const int32_t*)c->nbr_count, c->xs, c->counters,
+                   (const float4*)c->xs0);
 }
 
 void launch_warm_dp(pbf_context* c) {

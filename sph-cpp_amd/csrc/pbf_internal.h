@@ -97,12 +97,13 @@ enum Counter {
   C_MOVED_ALL = 9,  // two flags (9, 10): too many moved particles or marked cells -> treat all as dirty
   C_DONE = 11,      // iterate-to-tolerance: 0, or the slot whose error fell below the tolerance plus one
   C_ITERS = 12,     // iterate-to-tolerance: slots used
-  C_WARM_KEPT = 13, // the warm-start positions were kept
+  C_WARM_KEPT = 13, // the warm-start positions were kept (2: the damped retry's, PBF_OPT_WARM_RETRY)
   C_DISP = 14,      // two flags (14, 15): some particle has moved too far for the skin lists by the
                     // end of the previous Jacobi iteration (parity of the iteration that reads it)
   C_SKIN = 16,      // 1: this step's skin lists are built
   C_DISP_ALL = 17,  // the same over the whole step so far
   C_TILE = 18,      // particles listed by the cell-tile walk (counted with PBF_OPT_COUNT_OVERFLOW)
+  C_RETRY = 19,     // the warm start grew the error at slot 0 and its damped retry was applied
   C_NUM = 24
 };
 
@@ -121,7 +122,12 @@ struct SolveCtl {
   float* lsum;                 // sum of the applied multipliers (null: not kept)
   const float4* xsave;         // position stars to restore when the warm start is rejected
   int strict;                  // reject the warm start when the max or the mean error grows
+  int retry;                   // a rejected warm start is retried at kWarmRetryScale before the fallback to zero
 };
+
+// PBF_OPT_WARM_RETRY: the rejected warm start's multipliers are retried at this
+// scale (exact in float) before the solve falls back to a zero warm start.
+constexpr float kWarmRetryScale = 0.5f;
 
 // The first density pass of a warm-started solve: the learned warm start's
 // features, kept in device memory when feat is set, and with wlam set the
@@ -298,6 +304,7 @@ void launch_dp_update_masked(pbf_context* c, int it, int last, const SolveCtl& S
 void launch_warm_dp(pbf_context* c);      // xt, wlam -> xs (warm start), xsave = xt
 void launch_add_warm(pbf_context* c);     // lsum += wlam when the warm start was kept
 void launch_log_solve(pbf_context* c);    // one pbf_solve_record of the solve just queued
+void launch_warm_retry(pbf_context* c);   // the damped warm start after a rejected one (a no-op otherwise)
 // the Jacobi loop in one cooperative launch (PBF_OPT_PERSISTENT); advances dirty_stamp by iters
 bool persistent_available(pbf_context* c);
 uint32_t persistent_capacity(pbf_context* c);  // blocks of it the device holds at once (occupancy query)

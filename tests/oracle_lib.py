@@ -203,7 +203,7 @@ class Oracle:
     def emax(self):
         return float(self.L.orc_emax(self.h))
 
-    def solve(self, tol, max_iter, warm=None, mean=False, features=False, strict=False):
+    def solve(self, tol, max_iter, warm=None, mean=False, features=False, strict=False, retry=False):
         """Iterate-to-tolerance step.  Returns (slots used, max-error history,
         cumulative multipliers, warm start kept); with mean=True the mean
         error is compared and the history is the mean; with features=True a
@@ -215,13 +215,14 @@ class Oracle:
         feat = np.zeros((self.n, 8), np.float32)
         kept = C.c_int(0)
         w = None if warm is None else np.ascontiguousarray(warm, dtype=np.float32)
-        used = self.L.orc_solve(self.h, float(tol), int(max_iter), int(bool(mean)) | (2 if strict else 0),
+        flags = int(bool(mean)) | (2 if strict else 0) | (4 if retry else 0)
+        used = self.L.orc_solve(self.h, float(tol), int(max_iter), flags,
                                 None if w is None else w.ctypes.data_as(F), hmax.ctypes.data_as(F),
                                 hmean.ctypes.data_as(F), lsum.ctypes.data_as(F), feat.ctypes.data_as(F),
                                 C.byref(kept))
         if used < 0:
             raise RuntimeError("oracle solve failed")
-        self.last_solve = {"max": hmax[:used], "mean": hmean[:used]}
+        self.last_solve = {"max": hmax[:used], "mean": hmean[:used], "kept": int(kept.value)}
         out = (used, (hmean if mean else hmax)[:used], lsum, bool(kept.value))
         return out + (feat,) if features else out
 

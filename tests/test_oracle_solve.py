@@ -6,7 +6,9 @@ is pinned by reducing it to the reference step: with tolerance 0 and I slots
 it must be the plain step with I iterations, bit for bit; with a tolerance
 above the first error it must be the step with no pressure update.  The warm
 start is pinned by its two closed cases: zero multipliers change nothing,
-and multipliers that push the error up are rejected.
+and multipliers that push the error up are rejected.  Its retry at half the
+multipliers reduces to the warm-started solve with the halved multipliers
+one slot later, or to the plain solve two slots later.
 """
 import os
 
@@ -170,3 +172,70 @@ def test_strict_fallback_rejects_what_raises_either_error(scenes):
     assert res[True][3] == 0
     assert np.array_equal(res[True][1]["x"], pst["x"])
     assert np.array_equal(res[True][2], psums)
+
+
+def _c1_state(scenes):
+    params, pos, vel = scenes.config_c1(seed=5)
+    o = O.Oracle(O.params_from_pbf(params), mode=O.JACOBI, threads=THREADS)
+    o.set_state(pos, pos, vel)
+    o.step(10)
+    st = o.state()
+    o.close()
+    return params, st["x"].astype(np.float32), st["v"].astype(np.float32)
+
+
+def _solve(params, x, v, *a, **kw):
+    o = _oracle(params, x, v)
+    used, hist, sums, kept = o.solve(*a, **kw)
+    out = (used, hist, sums, o.last_solve["kept"], o.state(), o.counters()["hacks"])
+    o.close()
+    return out
+
+
+def test_retry_applies_the_halved_start(scenes):
+    """PBF_OPT_WARM_RETRY: a warm start that raises the mean error at slot 0
+    is retried once at half its multipliers from the same position stars.
+    When the half is kept, the solve is the warm-started solve with the halved
+    multipliers one slot later: the same positions, velocities and sums, and
+    its error history shifted by one.  Halving by a wrong factor (a quarter)
+    or keeping the full start gives other histories."""
+    params, x, v = _c1_state(scenes)
+    _, _, lsum, _, _, _ = _solve(params, x, v, 0.0, 5)
+    w = (lsum * np.float32(4.0)).astype(np.float32)
+    half = (w * np.float32(0.5)).astype(np.float32)
+    used, hist, sums, kept, st, hacks = _solve(params, x, v, 0.0, 4, warm=w, mean=True, retry=True)
+    assert used == 4 and kept == 2
+    used_n, hist_n, _, kept_n, _, _ = _solve(params, x, v, 0.0, 4, warm=w, mean=True)
+    assert kept_n == 0 and hist_n[0] == hist[0]  # without the retry the start is dropped
+    u2, h2, s2, k2, st2, hacks2 = _solve(params, x, v, 0.0, 3, warm=half, mean=True)
+    assert k2 == 1 and hacks == hacks2
+    assert np.array_equal(hist[1:], h2)
+    assert np.array_equal(sums, s2)
+    assert np.array_equal(st["x"], st2["x"]) and np.array_equal(st["v"], st2["v"])
+    quarter = (w * np.float32(0.25)).astype(np.float32)
+    _, h4, _, _, _, _ = _solve(params, x, v, 0.0, 3, warm=quarter, mean=True)
+    assert not np.array_equal(hist[1:], h4)
+    assert not np.array_equal(hist[1:], hist_n[1:])
+
+
+def test_retry_falls_back_to_zero_when_the_half_grows_too(scenes):
+    """Multipliers of the wrong sign grow the error at any scale: after the
+    retry's slot the solve is the plain one, two slots later."""
+    params, x, v = _scene(scenes, steps=8)
+    w = np.full(len(x), 50.0, np.float32)
+    used, hist, sums, kept, st, _ = _solve(params, x, v, 0.0, 4, warm=w, retry=True)
+    assert kept == 0 and used == 4
+    _, hp, sp, _, stp, _ = _solve(params, x, v, 0.0, 2)
+    assert np.array_equal(hist[2:], hp)
+    assert np.array_equal(sums, sp)
+    assert np.array_equal(st["x"], stp["x"])
+
+
+def test_retry_leaves_a_kept_start_alone(scenes):
+    params, x, v = _c1_state(scenes)
+    _, _, lsum, _, _, _ = _solve(params, x, v, 0.0, 5)
+    a = _solve(params, x, v, 0.0, 4, warm=lsum, retry=True)
+    b = _solve(params, x, v, 0.0, 4, warm=lsum)
+    assert a[3] == 1 and b[3] == 1 and a[0] == b[0]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert np.array_equal(a[4]["x"], b[4]["x"])

@@ -115,8 +115,8 @@ typedef struct pbf_params {
  * cooperative launch with grid-wide barriers between the passes, instead of one launch per
  * pass; a solve stops on the device.  Same results, bit for bit.  Used for up to 256
  * particles per compute unit (65,536 on a 256-CU device: one block of the launch per CU);
- * larger contexts, and graphs being captured (pbf_step_graph, pbf_solve_graph), keep the
- * launches per pass (pbf_stats.persistent_grid tells which ran).  A device without
+ * larger contexts, graphs being captured (pbf_step_graph, pbf_solve_graph) and solves with
+ * PBF_OPT_WARM_RETRY keep the launches per pass (pbf_stats.persistent_grid tells which ran).  A device without
  * cooperative launches makes pbf_step and pbf_solve fail with PBF_ERR_UNSUPPORTED. */
 #define PBF_OPT_PERSISTENT 128u
 /* A warm-started pbf_solve whose warm start grows the error at slot 0 retries it once with the

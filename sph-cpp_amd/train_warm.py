@@ -11,6 +11,9 @@ next.  Two sources of trajectories:
 * "tolerance" (default; sph-cpp_amd/warm_model.npy): each scene advances
   with the solves the warm start serves, iterating to a mean density error of
   ``tol`` warm-started from the previous sums.
+* "retry" (sph-cpp_amd/warm_model_retry.npy): as "tolerance" with
+  PBF_OPT_WARM_RETRY, so that substeps whose start was retried at half its
+  multipliers (and kept) are among the samples.
 * "plain50" (sph-cpp_amd/warm_model_plain50.npy): each scene advances with
   the plain solver run to 50 Jacobi iterations per substep, so Lambda is the
   multiplier the plain solve converges to.  The features come from a zero
@@ -24,7 +27,7 @@ least-squares gain of the output on samples of held-out scenes at the largest
 box scale (clamped to [0, 1]), so that only as much of the correction is
 applied as carries over to large scenes the network has not seen.
 
-    python sph-cpp_amd/train_warm.py [--pairs 2000000] [--epochs 40] [--source tolerance|plain50]
+    python sph-cpp_amd/train_warm.py [--pairs 2000000] [--epochs 40] [--source tolerance|retry|plain50]
                                      [--out sph-cpp_amd/warm_model.npy]
 
 Prints one JSON line per scene and a summary; writes the 4801 parameters
@@ -77,6 +80,8 @@ def generate(model, pairs, seed=0, steps=200, cap=60, tol=0.01, max_slots=128, p
     while model.sample_count() < pairs:
         start = model.sample_count()
         params, pos, vel, info = random_scene(rng, cap, scale)
+        if source == "retry":
+            params.options |= pbf.PBF_OPT_WARM_RETRY
         t = pbf.Particles(params)
         t.add(pos, pos, vel)
         log(json.dumps({"scene": k, **info, "particles": len(pos)}))
@@ -176,8 +181,9 @@ def main(argv=None):
     ap.add_argument("--scale", type=int, default=0, choices=(0, 1, 2, 4),
                     help="box scale of the training scenes (0: 1, 2 and 4 at random)")
     ap.add_argument("--scenes", type=int, default=8, help="training scenes (each gives pairs / scenes samples)")
-    ap.add_argument("--source", default="tolerance", choices=("tolerance", "plain50"),
-                    help="plain50: plain solves of 50 iterations; tolerance: warm-started tolerance solves")
+    ap.add_argument("--source", default="tolerance", choices=("tolerance", "retry", "plain50"),
+                    help="plain50: plain solves of 50 iterations; tolerance: warm-started tolerance solves; "
+                         "retry: the same with PBF_OPT_WARM_RETRY")
     ap.add_argument("--out", default=MODEL_PATH)
     a = ap.parse_args(argv)
     model, summary = train(a.pairs, a.epochs, a.batch, a.lr, a.seed, log=lambda s: print(s, flush=True),

@@ -128,3 +128,32 @@ def test_double_dam_break_and_batch_builders(scenes):
         assert b[0] == pytest.approx(2 * ext[0], rel=0.02)
         assert b[1] == pytest.approx(ext[1], rel=0.02) and b[2] == pytest.approx(ext[2], rel=0.02)
         assert 0.4 <= ext[0] / ext[1] <= 2.1 and 0.4 <= ext[2] / ext[1] <= 2.1
+
+
+def _header_text():
+    return re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+
+
+def test_python_constants_match_the_header(pbf):
+    """Every integer #define of include/pbf.h that the package mirrors has the
+    header's value, and every option bit is mirrored."""
+    defs = dict(re.findall(r"^#define (PBF_[A-Z0-9_]+) (\d+)u?\s*$", _header_text(), flags=re.M))
+    assert any(k.startswith("PBF_OPT_") for k in defs)
+    for name, value in defs.items():
+        if name.startswith("PBF_OPT_"):
+            assert hasattr(pbf, name), name
+        if hasattr(pbf, name):
+            assert getattr(pbf, name) == int(value), name
+    bits = [int(v) for k, v in defs.items() if k.startswith("PBF_OPT_")]
+    assert len(set(bits)) == len(bits) and all(b & (b - 1) == 0 for b in bits)  # distinct single bits
+
+
+@pytest.mark.parametrize("c_name,py_name", [("pbf_stats", "Stats"), ("pbf_solve_info", "SolveInfo"),
+                                            ("pbf_solve_record", "SolveRecord"), ("pbf_tuning", "Tuning"),
+                                            ("pbf_layout", "Layout")])
+def test_python_structs_match_the_header(pbf, c_name, py_name):
+    """The ctypes mirrors list the C structs' fields in the same order."""
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (c_name, c_name), _header_text(), flags=re.S).group(1)
+    c_fields = re.findall(r"\b([a-z_][a-z0-9_]*)(?:\[[^\]]*\])?\s*;", body)
+    py_fields = [f[0] for f in getattr(pbf, py_name)._fields_]
+    assert c_fields == py_fields, (c_fields, py_fields)

@@ -281,17 +281,17 @@ __device__ __forceinline__ void warm_network(const FeatCtl& F, uint32_t wave_fir
 template <int FM, bool FAST, bool SR>
 __device__ __forceinline__ void density_pass(uint32_t vb, bool first, uint32_t n, int it, int eslot,
                                              const SolveCtl& S, const FeatCtl& F,
-                                                             const KParams& K, const float4* __restrict__ xs,
-                                                             const float4* __restrict__ v, float4* __restrict__ xt,
-                                                             float* __restrict__ rho, float* __restrict__ lam,
-                                                             const int32_t* __restrict__ nbr,
-                                                             const int32_t* __restrict__ nbr_count,
-                                                             uint64_t* __restrict__ wmask,
-                                                             uint32_t* __restrict__ moved, uint32_t stamp,
-                                                             uint32_t* __restrict__ mflag,
-                                                             unsigned long long* __restrict__ counters,
-                                                             const int32_t* __restrict__ snb,
-                                                             const int32_t* __restrict__ snb_count) {
+                                             const KParams& K, const float4* __restrict__ xs,
+                                             const float4* __restrict__ v, float4* __restrict__ xt,
+                                             float* __restrict__ rho, float* __restrict__ lam,
+                                             const int32_t* __restrict__ nbr,
+                                             const int32_t* __restrict__ nbr_count,
+                                             uint64_t* __restrict__ wmask,
+                                             uint32_t* __restrict__ moved, uint32_t stamp,
+                                             uint32_t* __restrict__ mflag,
+                                             unsigned long long* __restrict__ counters,
+                                             const int32_t* __restrict__ snb,
+                                             const int32_t* __restrict__ snb_count) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   const uint32_t p = vb * blockDim.x + threadIdx.x;
   // this iteration's pressure update records its moves in the other slot
@@ -338,19 +338,21 @@ __device__ __forceinline__ void density_pass(uint32_t vb, bool first, uint32_t n
   }
 }
 
+// (the exact and fast instances of the plain pass fit 64 registers: 8 wavefronts per SIMD)
 template <int FM, bool FAST, bool SR>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((FM == 0 && (SR || FAST)) ? 8 : 1))) k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S, FeatCtl F,
-                                                             KParams K, const float4* __restrict__ xs,
-                                                             const float4* __restrict__ v, float4* __restrict__ xt,
-                                                             float* __restrict__ rho, float* __restrict__ lam,
-                                                             const int32_t* __restrict__ nbr,
-                                                             const int32_t* __restrict__ nbr_count,
-                                                             uint64_t* __restrict__ wmask,
-                                                             uint32_t* __restrict__ moved, uint32_t stamp,
-                                                             uint32_t* __restrict__ mflag,
-                                                             unsigned long long* __restrict__ counters,
-                                                             const int32_t* __restrict__ snb,
-                                                             const int32_t* __restrict__ snb_count) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((FM == 0 && (SR || FAST)) ? 8 : 1)))
+k_density_lambda_c(uint32_t n, int it, int eslot, SolveCtl S, FeatCtl F,
+                   KParams K, const float4* __restrict__ xs,
+                   const float4* __restrict__ v, float4* __restrict__ xt,
+                   float* __restrict__ rho, float* __restrict__ lam,
+                   const int32_t* __restrict__ nbr,
+                   const int32_t* __restrict__ nbr_count,
+                   uint64_t* __restrict__ wmask,
+                   uint32_t* __restrict__ moved, uint32_t stamp,
+                   uint32_t* __restrict__ mflag,
+                   unsigned long long* __restrict__ counters,
+                   const int32_t* __restrict__ snb,
+                   const int32_t* __restrict__ snb_count) {
   density_pass<FM, FAST, SR>(xcd_block(), blockIdx.x == 0 && threadIdx.x == 0, n, it, eslot, S, F, K, xs, v, xt, rho,
                          lam, nbr, nbr_count, wmask, moved, stamp, mflag, counters, snb, snb_count);
 }
@@ -438,13 +440,13 @@ __device__ __forceinline__ f3 ld3_words(const float4* a, uint32_t i) {
 
 template <bool FAST>
 __device__ __forceinline__ void repair_pass(uint32_t vb, uint32_t n, int it, uint32_t stamp, const SolveCtl& S,
-                                                          const KParams& K, float4* __restrict__ xt,
-                                                          const float* __restrict__ rho, float* __restrict__ lam,
-                                                          const int32_t* __restrict__ nbr,
-                                                          const int32_t* __restrict__ nbr_count,
-                                                          const uint32_t* __restrict__ mflag,
-                                                          uint32_t* __restrict__ dirty,
-                                                          const unsigned long long* __restrict__ counters) {
+                                            const KParams& K, float4* __restrict__ xt,
+                                            const float* __restrict__ rho, float* __restrict__ lam,
+                                            const int32_t* __restrict__ nbr,
+                                            const int32_t* __restrict__ nbr_count,
+                                            const uint32_t* __restrict__ mflag,
+                                            uint32_t* __restrict__ dirty,
+                                            const unsigned long long* __restrict__ counters) {
   if (S.tol >= 0.0f && counters[C_DONE] != 0ull) return;
   if (counters[C_HACK_FLAG + (it & 1)] == 0ull) return;
   const bool scan = counters[C_MOVED_ALL + (it & 1)] != 0ull;
@@ -547,17 +549,17 @@ __device__ __forceinline__ f3 dp_walk(const int32_t* __restrict__ walk, int cnt,
 template <bool FAST, bool SR>
 __device__ __forceinline__ void dp_pass(uint32_t vb, bool first, uint32_t n, int it, int last, uint32_t stamp,
                                         const SolveCtl& S,
-                                                        const KParams& K, const float4* __restrict__ xt,
-                                                        const float4* __restrict__ x, float4* __restrict__ xs,
-                                                        float4* __restrict__ v, const int32_t* __restrict__ nbr,
-                                                        const int32_t* __restrict__ nbr_count,
-                                                        const uint64_t* __restrict__ wmask,
-                                                        const uint32_t* __restrict__ dirty,
-                                                        unsigned long long* __restrict__ counters,
-                                                        const int32_t* __restrict__ snb,
-                                                        const int32_t* __restrict__ snb_count,
-                                                        const float4* __restrict__ xs0,
-                                                        const float* __restrict__ rho) {
+                                        const KParams& K, const float4* __restrict__ xt,
+                                        const float4* __restrict__ x, float4* __restrict__ xs,
+                                        float4* __restrict__ v, const int32_t* __restrict__ nbr,
+                                        const int32_t* __restrict__ nbr_count,
+                                        const uint64_t* __restrict__ wmask,
+                                        const uint32_t* __restrict__ dirty,
+                                        unsigned long long* __restrict__ counters,
+                                        const int32_t* __restrict__ snb,
+                                        const int32_t* __restrict__ snb_count,
+                                        const float4* __restrict__ xs0,
+                                        const float* __restrict__ rho) {
   // C_DONE holds the slot that ended the solve, plus one.  Only a solve that
   // ended at an earlier slot skips this launch: blocks of the ending slot that
   // start after block 0 has written the flag must still copy xt to x*.

@@ -103,7 +103,7 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return x 
 // compute_densities (particles.h:292-313) + compute_pressure_lambdas (:318-342)
 // xs -> xt (x* with lambda in w), rho, lam, and the particle's within-h mask
 // (bit k of wmask[p], of wmask[n + p] for k >= 64: list entry k lies within h
-// and is in the sums).  With FEAT, also the learned warm start's features of
+// and is in the sums; not read for a particle the density fix moved).  With FEAT, also the learned warm start's features of
 // the particle (FeatCtl).
 // Returns the particle's density error max(rho/rho0 - 1, 0).
 template <bool FEAT, bool FAST, bool SR>
@@ -128,11 +128,14 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   float dy_sum = 0.0f, div = 0.0f;
   const f3 vp = FEAT ? ld3(v, p) : f3{0.f, 0.f, 0.f};
   // one walk over the list: the entries within h go into the sums, in list
-  // order, and into the mask; the walk stops at a near-coincident entry
+  // order, and into the mask.  A near-coincident entry ends the walk after
+  // its batch, and the plain walk below redoes the particle; the mask of such
+  // a particle is not read (the pressure update walks a moved particle's
+  // full list with the test).
   uint64_t m0 = 0, m1 = 0;
-  int hack_at = -1;
+  bool hack = false;
   const uint32_t qmask = skin ? kSkinSlotMask : ~0u;  // the slot of a walked entry
-  for (int k0 = 0; k0 < cnt && hack_at < 0; k0 += kBatch) {
+  for (int k0 = 0; k0 < cnt && !hack; k0 += kBatch) {
     uint32_t q[kBatch];  // skin entries keep their place in the full list in the top bits
     load_batch(walk, K.stride, k0, cnt, p, q);
     float4 t[kBatch];
@@ -143,11 +146,11 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
     for (int u = 0; u < kBatch; ++u) {
       const f3 d = sub3(xp, xyz(t[u]));
       const float d2 = magsq3(d);
-      const bool w = (k0 + u < cnt) & (hack_at < 0) & within_h(d2, K);
+      const bool w = (k0 + u < cnt) & within_h(d2, K);
       const bool hk = w & (d2 < K.T_hack);  // particles.h:304: handled by the plain walk below
       const bool acc = w & !hk;
       bits |= (uint32_t)acc << u;
-      hack_at = hk ? (skin ? (int)(q[u] >> kSkinSlotBits) : k0 + u) : hack_at;
+      hack |= hk;
       if (acc) {
         const float r = nb_sqrt<FAST, SR>(d2);
         density += K.mass * poly6_in(r, K);
@@ -170,32 +173,46 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   wmask[p] = m0;
   if (K.cap > 64) wmask[n + p] = m1;
   int within = __popcll(m0) + __popcll(m1);
-  if (hack_at >= 0) {
-    // particles.h:304-305: move x*_p, then finish this particle with a plain walk
+  if (hack) {
+    // particles.h:304-305: a plain walk over the full list from its first
+    // entry, in list order, moves x*_p at each near-coincident entry
+    density = K.mass_W0;
+    sum_C = 0.0f;
+    sg = {0.f, 0.f, 0.f};
+    dy_sum = 0.0f;
+    div = 0.0f;
+    within = 0;
     const int full = nbr_count[p];
-    for (int k = hack_at; k < full; ++k) {
-      const uint32_t qk = (uint32_t)nbr[list_at(k, p, K.stride)];
-      const f3 xq = ld3(xs, qk);
-      f3 d = sub3(xp, xq);
-      float d2 = magsq3(d);
-      if (!within_h(d2, K)) continue;
-      if (d2 < K.T_hack) {
-        dec3(xp, div3(mul3(ld3(v, p), K.dt), 50.0f));
-        ++hacks;
-        d = sub3(xp, xq);
-        d2 = magsq3(d);
-        if (!within_h(d2, K)) continue;
-      }
-      const float r = nb_sqrt<FAST>(d2);
-      density += K.mass * poly6_in(r, K);
-      const f3 grad = smul3(K.neg_inv_rho0, nb_spiky<FAST>(d, r, K));
-      dec3(sg, grad);
-      sum_C += magsq3(grad);
-      if (FEAT) {
-        ++within;
-        dy_sum -= d.y;
-        const f3 dv = sub3(vp, ld3(v, qk));
-        div -= (dv.x * grad.x + dv.y * grad.y) + dv.z * grad.z;
+    for (int k0 = 0; k0 < full; k0 += kBatch) {
+      // a batch's positions load together; the entries are taken in order
+      uint32_t q[kBatch];
+      load_batch(nbr, K.stride, k0, full, p, q);
+      f3 t[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) t[u] = ld3(xs, q[u]);
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        f3 d = sub3(xp, t[u]);
+        float d2 = magsq3(d);
+        if (k0 + u >= full || !within_h(d2, K)) continue;
+        if (d2 < K.T_hack) {
+          dec3(xp, div3(mul3(ld3(v, p), K.dt), 50.0f));
+          ++hacks;
+          d = sub3(xp, t[u]);
+          d2 = magsq3(d);
+          if (!within_h(d2, K)) continue;
+        }
+        const float r = nb_sqrt<FAST>(d2);
+        density += K.mass * poly6_in(r, K);
+        const f3 grad = smul3(K.neg_inv_rho0, nb_spiky<FAST>(d, r, K));
+        dec3(sg, grad);
+        sum_C += magsq3(grad);
+        if (FEAT) {
+          ++within;
+          dy_sum -= d.y;
+          const f3 dv = sub3(vp, ld3(v, q[u]));
+          div -= (dv.x * grad.x + dv.y * grad.y) + dv.z * grad.z;
+        }
       }
     }
   }

@@ -135,12 +135,16 @@ __device__ __forceinline__ float density_lambda_one(uint32_t p, uint32_t n, int 
   uint64_t m0 = 0, m1 = 0;
   bool hack = false;
   const uint32_t qmask = skin ? kSkinSlotMask : ~0u;  // the slot of a walked entry
+  uint32_t qn[kBatch];  // skin entries keep their place in the full list in the top bits
+  load_batch(walk, K.stride, 0, cnt, p, qn);
   for (int k0 = 0; k0 < cnt && !hack; k0 += kBatch) {
-    uint32_t q[kBatch];  // skin entries keep their place in the full list in the top bits
-    load_batch(walk, K.stride, k0, cnt, p, q);
+    uint32_t q[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) q[u] = qn[u];
     float4 t[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) t[u] = gather4(xs, q[u] & qmask);
+    load_batch(walk, K.stride, k0 + kBatch, cnt, p, qn);  // the next batch's entries load under this one's work
     uint32_t bits = 0;
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {

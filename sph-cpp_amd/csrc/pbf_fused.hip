@@ -5,8 +5,10 @@
 //
 //  * Latency: a neighbour walk is a chain of dependent loads (list entry, then
 //    the neighbour's data).  Neighbours are taken in batches of kBatch: the
-//    list entries of a batch come in two 16-byte loads, then all gathers are
-//    issued, then the arithmetic runs in list order.
+//    batch's list entries load first (entry k of a wavefront's 64 particles
+//    is one 256-byte row), then all gathers are issued, then the arithmetic
+//    runs in list order.  The density pass loads the next batch's entries
+//    while it works on the current one.
 //  * Only about half of a particle's listed neighbours lie within h (the list
 //    radius is the 1.2 h bin spacing, neighbors.h:198); the others contribute
 //    exact zeros.  The density pass walks the whole list once, summing the
@@ -20,9 +22,10 @@
 //  * The density pass writes lambda into the w word of its x* output, so the
 //    dp pass gathers a neighbour's x* and lambda in one 16-byte load.
 //  * The near-coincidence fix of compute_densities (particles.h:303-305) moves
-//    x*_p in the middle of the walk; when it fires, the density pass finishes
-//    that particle with a plain walk and raises the iteration's flag, and the
-//    lambda and dp passes of that iteration fall back to plain walks.
+//    x*_p in the middle of the walk; when it fires, the density pass redoes
+//    that particle with a plain walk from its first entry and raises the
+//    iteration's flag, and the lambda and dp passes of that iteration fall
+//    back to plain walks for the particles next to a moved one.
 #include <hip/hip_cooperative_groups.h>
 
 #include "pbf_mlp_device.h"

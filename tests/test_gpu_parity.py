@@ -281,6 +281,29 @@ def test_density_fix_and_lambda_repair(pbf, scenes):
     _compare_state(g, o, "step with density fix")
 
 
+@pytest.mark.parametrize("lists", ["skin", "full"])
+def test_density_fix_over_steps(pbf, scenes, lists):
+    """Coincident pairs at several places of a lattice, over four steps: from
+    the second step on the Jacobi passes walk the skin lists (unless the full
+    lists are asked for), so the density pass meets a near-coincident entry in
+    either walk and redoes that particle with the plain walk.  Every step must
+    equal the oracle bit for bit, with fixes in every step."""
+    p, pos, vel = _small_scene(scenes, seed=19, counts=(10, 8, 8), box=(1.0, 0.8, 0.8))
+    pos = pos.copy()
+    for a, b in ((10, 11), (100, 163), (300, 301), (517, 518)):
+        pos[a] = pos[b]
+    vel = np.zeros_like(pos)
+    if lists == "full":
+        p = pbf.Params.from_buffer_copy(p)
+        p.options |= pbf.PBF_OPT_FULL_LISTS
+    g, o = _pair(pbf, p, pos, vel)
+    for step in range(4):
+        g.step(1)
+        o.step(1)
+        assert g.stats()["density_hacks"] > 0, f"step {step + 1}"
+        _compare_state(g, o, f"{lists} lists, density fix, step {step + 1}")
+
+
 @pytest.mark.parametrize("case", ["crowded", "many"])
 def test_density_fix_exact_scan(pbf, scenes, case):
     """The density fix (particles.h:303-305) where marking by neighbourhood

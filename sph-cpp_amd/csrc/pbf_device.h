@@ -65,10 +65,6 @@ __device__ __forceinline__ float poly6_in(float r, const KParams& K) {
   const float a = K.hh - r * r;
   return K.poly6_norm * a * a * a;
 }
-__device__ __forceinline__ float poly6(float r, const KParams& K) {
-  if (r > K.h) return 0.0f;
-  return poly6_in(r, K);
-}
 // Three correctly rounded quotients a/b with one divisor.  A correctly
 // rounded divide on gfx950 is a Markstein division (P. Markstein, IBM J. Res.
 // Dev. 34(1), 1990): reciprocal estimate, one Newton step

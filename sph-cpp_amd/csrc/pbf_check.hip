@@ -31,8 +31,9 @@ __device__ __forceinline__ bool same_bits(float x, float y) { return __float_as_
 __device__ __forceinline__ bool same3(f3 a, f3 b) { return same_bits(a.x, b.x) && same_bits(a.y, b.y) && same_bits(a.z, b.z); }
 // what the host guarantees before it selects div3_shared_static
 __device__ __forceinline__ bool static_div_ok(f3 a, float b) {
-  return (b >= 0x1p-40f) & (b <= 0x1p40f) & (fabsf(a.x) <= 0x1p40f) & (fabsf(a.y) <= 0x1p40f) &
-         (fabsf(a.z) <= 0x1p40f);
+  const unsigned ok = (unsigned)(b >= 0x1p-40f) & (unsigned)(b <= 0x1p40f) & (unsigned)(fabsf(a.x) <= 0x1p40f) &
+                      (unsigned)(fabsf(a.y) <= 0x1p40f) & (unsigned)(fabsf(a.z) <= 0x1p40f);
+  return ok != 0u;
 }
 
 __global__ void k_check_arithmetic(uint64_t samples, uint64_t seed, unsigned long long* __restrict__ out) {
